@@ -1,0 +1,248 @@
+"""examd_amd -- MI355X (gfx950) midpoint-method Lennard-Jones force path.
+
+Python view of the C ABI in include/examd_hip.h (libexamd_hip.so, built
+in-tree by ``python -m examd_amd.build``).  The Fortran drop-in layer in
+examd_amd/fortran binds the same ABI for the reference's own call surface.
+
+Loading fails loudly when the HIP library is missing: there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libexamd_hip.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+_PREC = {"fp64": PREC_FP64, "mixed": PREC_MIXED, PREC_FP64: PREC_FP64,
+         PREC_MIXED: PREC_MIXED}
+
+# status codes (include/examd_hip.h)
+OK = 0
+ERR_ARG = 1
+ERR_HIP = 2
+ERR_NOMEM = 3
+ERR_GRID = 4
+ERR_TILE_OVERFLOW = 5
+ERR_NO_DEVICE = 6
+ERR_NONFINITE = 7
+ERR_NOT_READY = 8
+ERR_FIXED_RANGE = 9
+ERR_ZERO_DISTANCE = 10
+ERR_OUT_OF_BOX = 109
+
+# every symbol include/examd_hip.h declares
+EXPORTS = (
+    "examd_hip_domain", "examd_hip_create", "examd_hip_destroy",
+    "examd_hip_set_state", "examd_hip_get_state", "examd_hip_forces",
+    "examd_hip_run", "examd_hip_block_ids", "examd_hip_home_pair_counts",
+    "examd_hip_timing", "examd_hip_generate_fcc", "examd_hip_strerror",
+)
+
+
+class ExamdError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: {strerror(status)} (status {status})")
+
+
+class Params(C.Structure):
+    _fields_ = [("natoms", C.c_int64), ("box", C.c_double * 3),
+                ("cutoff", C.c_double), ("epsilon", C.c_double),
+                ("sigma", C.c_double), ("mass", C.c_double),
+                ("tiles", C.c_int32 * 3), ("device", C.c_int32),
+                ("block_threads", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("step", C.c_int64), ("ke", C.c_double), ("pe", C.c_double),
+                ("pe_shifted", C.c_double), ("etot", C.c_double),
+                ("pairs_cand", C.c_int64), ("pairs_cut", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libexamd_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -m examd_amd.build` (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        dp, ip64, ip32 = P(C.c_double), P(C.c_int64), P(C.c_int32)
+        L.examd_hip_domain.argtypes = [dp, C.c_double, ip32, ip64]
+        L.examd_hip_create.argtypes = [P(Params), P(C.c_void_p)]
+        L.examd_hip_destroy.argtypes = [C.c_void_p]
+        L.examd_hip_set_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
+        L.examd_hip_get_state.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64]
+        L.examd_hip_forces.argtypes = [C.c_void_p, C.c_int, dp, P(Stats)]
+        L.examd_hip_run.argtypes = [C.c_void_p, C.c_int, C.c_int64,
+                                    C.c_double, C.c_int64, P(Stats),
+                                    C.c_int64, ip64]
+        L.examd_hip_block_ids.argtypes = [C.c_void_p, ip64, ip32, C.c_int64]
+        L.examd_hip_home_pair_counts.argtypes = [C.c_void_p, C.c_int, ip64,
+                                                 C.c_int64]
+        L.examd_hip_timing.argtypes = [C.c_void_p, C.c_int, dp, ip64, C.c_int]
+        L.examd_hip_generate_fcc.argtypes = [ip32, C.c_double, C.c_double,
+                                             C.c_double, C.c_double,
+                                             C.c_uint64, dp, dp, dp]
+        L.examd_hip_strerror.argtypes = [C.c_int]
+        L.examd_hip_strerror.restype = C.c_char_p
+        for name in EXPORTS:
+            if name != "examd_hip_strerror":
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def strerror(status: int) -> str:
+    return lib().examd_hip_strerror(int(status)).decode()
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(st, where, allow=()):
+    if st != OK and st not in allow:
+        raise ExamdError(st, where)
+    return st
+
+
+def domain(box, block_length):
+    """exaMD_domain_create: (nblocks_xyz, nblocks)."""
+    b = np.ascontiguousarray(box, dtype=np.float64).reshape(3)
+    nb = (C.c_int32 * 3)()
+    n = C.c_int64()
+    _check(lib().examd_hip_domain(_dp(b), float(block_length), nb,
+                                  C.byref(n)), "examd_hip_domain")
+    return tuple(nb), n.value
+
+
+def generate_fcc(nc, rho, temperature=0.0, mass=1.0, perturb=0.0, seed=0):
+    """Seeded FCC lattice + Maxwell-Boltzmann velocities: (pos, vel, box)."""
+    nc = (C.c_int32 * 3)(*[int(v) for v in (nc if np.ndim(nc) else (nc,) * 3)])
+    n = 4 * nc[0] * nc[1] * nc[2]
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 3))
+    box = np.zeros(3)
+    _check(lib().examd_hip_generate_fcc(nc, float(rho), float(temperature),
+                                        float(mass), float(perturb),
+                                        int(seed) & (2**64 - 1), _dp(pos),
+                                        _dp(vel), _dp(box)),
+           "examd_hip_generate_fcc")
+    return pos, vel, box
+
+
+@dataclass
+class RunResult:
+    stats: list
+
+
+class MidpointLJ:
+    """One LJ system on one GPU (a handle of the C ABI)."""
+
+    def __init__(self, natoms, box, cutoff, epsilon=1.0, sigma=1.0, mass=1.0,
+                 tiles=(0, 0, 0), device=0, block_threads=0):
+        p = Params()
+        p.natoms = int(natoms)
+        p.box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
+        p.cutoff, p.epsilon, p.sigma, p.mass = (float(cutoff), float(epsilon),
+                                                float(sigma), float(mass))
+        p.tiles = (C.c_int32 * 3)(*[int(t) for t in tiles])
+        p.device = int(device)
+        p.block_threads = int(block_threads)
+        self.params = p
+        self.n = int(natoms)
+        self.box = np.array(np.reshape(box, 3), dtype=np.float64)
+        self.cutoff = float(cutoff)
+        self.nb, self.nblocks = domain(self.box, 0.5 * self.cutoff)
+        h = C.c_void_p()
+        _check(lib().examd_hip_create(C.byref(p), C.byref(h)),
+               "examd_hip_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().examd_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, pos, vel=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float64)
+        assert pos.shape == (self.n, 3)
+        _check(lib().examd_hip_set_state(self._h, _dp(pos), _dp(vel), self.n),
+               "examd_hip_set_state")
+
+    def get_state(self, forces=False):
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        f = np.zeros((self.n, 3)) if forces else None
+        _check(lib().examd_hip_get_state(self._h, _dp(pos), _dp(vel), _dp(f),
+                                         self.n), "examd_hip_get_state")
+        return (pos, vel, f) if forces else (pos, vel)
+
+    def forces(self, precision="fp64", allow=()):
+        f = np.zeros((self.n, 3))
+        s = Stats()
+        st = _check(lib().examd_hip_forces(self._h, _PREC[precision], _dp(f),
+                                           C.byref(s)),
+                    "examd_hip_forces", allow)
+        return f, s, st
+
+    def run(self, nsteps, dt, precision="fp64", energy_every=0):
+        cap = max(1, int(nsteps))
+        out = (Stats * cap)()
+        nout = C.c_int64()
+        _check(lib().examd_hip_run(self._h, _PREC[precision], int(nsteps),
+                                   float(dt), int(energy_every), out, cap,
+                                   C.byref(nout)), "examd_hip_run")
+        return [out[i] for i in range(nout.value)]
+
+    def block_ids(self):
+        num = np.zeros(self.n, dtype=np.int64)
+        xyz = np.zeros((self.n, 3), dtype=np.int32)
+        _check(lib().examd_hip_block_ids(
+            self._h, num.ctypes.data_as(C.POINTER(C.c_int64)),
+            xyz.ctypes.data_as(C.POINTER(C.c_int32)), self.n),
+            "examd_hip_block_ids")
+        return num, xyz
+
+    def home_pair_counts(self, precision="fp64"):
+        c = np.zeros(self.nblocks, dtype=np.int64)
+        _check(lib().examd_hip_home_pair_counts(
+            self._h, _PREC[precision], c.ctypes.data_as(C.POINTER(C.c_int64)),
+            self.nblocks), "examd_hip_home_pair_counts")
+        return c
+
+    def timing(self, enable=True, reset=False):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(lib().examd_hip_timing(self._h, int(enable), C.byref(ms),
+                                      C.byref(n), int(reset)),
+               "examd_hip_timing")
+        return ms.value, n.value

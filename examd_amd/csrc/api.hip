@@ -1,0 +1,681 @@
+// api.hip -- C ABI of the gfx950 midpoint LJ library (include/examd_hip.h).
+//
+// Host orchestration only: device buffers, the cell-block (tile) map that
+// replaces the METIS partition (SRC/exaMD_partitioning.f90:65-455), LDS
+// sizing, the per-step kernel sequence and the seeded generators.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/examd_hip.h"
+#include "kernels.hpp"
+
+using namespace examd;
+
+struct examd_handle {
+  examd_params p{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  Grid g{};
+  int64_t nblocks = 0;
+  int64_t n = 0;
+  DevState d{};
+  StepLog *log = nullptr;
+  int64_t log_cap = 0;
+  int tile_threads = 256;
+  int maxrb = 0;
+  int cap[2] = {0, 0};      // LDS capacity per precision
+  size_t lds_limit = 0;
+  bool state_set = false;
+  int forces_valid = -1;    // precision whose forces match the positions
+  int last_prec = 0;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  double force_ms = 0.0;
+  int64_t force_launches = 0;
+};
+
+namespace {
+
+#define HIPCHK(expr)                          \
+  do {                                        \
+    hipError_t e_ = (expr);                   \
+    if (e_ != hipSuccess) return EXAMD_ERR_HIP; \
+  } while (0)
+
+int flags_to_status(unsigned long long f) {
+  if (f & kFlagOutOfBox) return EXAMD_ERR_OUT_OF_BOX;
+  if (f & kFlagNonFinite) return EXAMD_ERR_NONFINITE;
+  if (f & kFlagOverflow) return EXAMD_ERR_TILE_OVERFLOW;
+  if (f & kFlagZero) return EXAMD_ERR_ZERO_DISTANCE;
+  if (f & kFlagFixedRange) return EXAMD_ERR_FIXED_RANGE;
+  return EXAMD_OK;
+}
+
+template <class T>
+int dalloc(T **p, int64_t n) {
+  size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+  if (hipMalloc((void **)p, bytes) != hipSuccess) {
+    *p = nullptr;
+    return EXAMD_ERR_NOMEM;
+  }
+  return EXAMD_OK;
+}
+
+void free_state(examd_handle *h) {
+  DevState &d = h->d;
+  void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
+                  d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.rank,
+                  d.count, d.start, d.sorted, d.chunk, d.home, h->log};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  d = DevState{};
+  h->log = nullptr;
+  h->log_cap = 0;
+}
+
+int ensure_log(examd_handle *h, int64_t n) {
+  if (h->log_cap >= n) return EXAMD_OK;
+  if (h->log) (void)hipFree(h->log);
+  h->log = nullptr;
+  h->log_cap = 0;
+  if (dalloc(&h->log, n)) return EXAMD_ERR_NOMEM;
+  h->log_cap = n;
+  return EXAMD_OK;
+}
+
+// Largest region (tile plus one ghost block per side) in blocks.
+int max_region_blocks(const Grid &g) {
+  int m[3];
+  for (int a = 0; a < 3; ++a) {
+    int mx = 0;
+    for (int t = 0; t < g.T[a]; ++t) {
+      int lo, hi;
+      tile_range(t, g.nb[a], g.T[a], lo, hi);
+      mx = std::max(mx, hi - lo + 2);
+    }
+    m[a] = mx;
+  }
+  return m[0] * m[1] * m[2];
+}
+
+// Largest number of atoms any tile's region holds for host positions.
+int64_t max_region_atoms(const Grid &g, const double *pos, int64_t n) {
+  const int nx = g.nb[0], ny = g.nb[1], nz = g.nb[2];
+  std::vector<int> cnt((size_t)nx * ny * nz, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+      double q = pos[3 * i + a] / g.bl + 1.0;
+      int v = (int)q;
+      if (v > g.nb[a]) v = g.nb[a];
+      if (v < 1) v = 1;
+      b[a] = v - 1;
+    }
+    cnt[(size_t)b[0] + (size_t)nx * (b[1] + (size_t)ny * b[2])]++;
+  }
+  // per-axis tile boundaries
+  std::vector<int> lo[3], hi[3];
+  for (int a = 0; a < 3; ++a)
+    for (int t = 0; t < g.T[a]; ++t) {
+      int l, h2;
+      tile_range(t, g.nb[a], g.T[a], l, h2);
+      lo[a].push_back(l);
+      hi[a].push_back(h2);
+    }
+  int64_t best = 0;
+  for (int tz = 0; tz < g.T[2]; ++tz)
+    for (int ty = 0; ty < g.T[1]; ++ty)
+      for (int tx = 0; tx < g.T[0]; ++tx) {
+        int64_t s = 0;
+        for (int cz = lo[2][tz] - 1; cz <= hi[2][tz]; ++cz) {
+          int gz = (cz + nz) % nz;
+          for (int cy = lo[1][ty] - 1; cy <= hi[1][ty]; ++cy) {
+            int gy = (cy + ny) % ny;
+            const int *row = &cnt[(size_t)nx * (gy + (size_t)ny * gz)];
+            for (int cx = lo[0][tx] - 1; cx <= hi[0][tx]; ++cx)
+              s += row[(cx + nx) % nx];
+          }
+        }
+        best = std::max(best, s);
+      }
+  return best;
+}
+
+// Pick the LDS capacity of a cell block for both precisions from the
+// largest region of the given positions (30% headroom for density
+// fluctuations during a run, at least 10%).  When the tiles were chosen
+// automatically and a region does not fit the device's LDS, refine the
+// cell-block map along its coarsest axis.
+int choose_caps(examd_handle *h, const double *pos) {
+  for (int iter = 0; iter < 256; ++iter) {
+    h->maxrb = max_region_blocks(h->g);
+    int64_t mra = max_region_atoms(h->g, pos, h->n);
+    int64_t want = ((int64_t)std::ceil((double)mra * 1.3) + 64 + 63) / 64 * 64;
+    int64_t need = (int64_t)std::ceil((double)mra * 1.1) + 32;
+    bool ok = true;
+    int caps[2];
+    for (int prec = 0; prec < 2; ++prec) {
+      bool fp64 = prec == EXAMD_PREC_FP64;
+      int64_t c = want;
+      while (c > need &&
+             force_lds_bytes(fp64, (int)c, h->maxrb) > h->lds_limit)
+        c -= 16;
+      if (force_lds_bytes(fp64, (int)c, h->maxrb) > h->lds_limit) ok = false;
+      caps[prec] = (int)c;
+    }
+    if (ok) {
+      h->cap[0] = caps[0];
+      h->cap[1] = caps[1];
+      return EXAMD_OK;
+    }
+    bool fixed = h->p.tiles[0] > 0 || h->p.tiles[1] > 0 || h->p.tiles[2] > 0;
+    if (fixed) return EXAMD_ERR_TILE_OVERFLOW;
+    int ax = -1;
+    double best = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      double size = (double)h->g.nb[a] / h->g.T[a];
+      if (h->g.T[a] < h->g.nb[a] && size > best) {
+        best = size;
+        ax = a;
+      }
+    }
+    if (ax < 0) return EXAMD_ERR_TILE_OVERFLOW;
+    h->g.T[ax] += 1;
+  }
+  return EXAMD_ERR_TILE_OVERFLOW;
+}
+
+ForceConfig force_config(const examd_handle *h, int prec) {
+  ForceConfig c;
+  c.threads = h->tile_threads;
+  c.cap = h->cap[prec];
+  c.maxrb = h->maxrb;
+  return c;
+}
+
+LJParams lj(const examd_handle *h) {
+  return LJParams{h->p.epsilon, h->p.sigma, h->p.cutoff, h->p.mass};
+}
+
+hipEvent_t next_event(examd_handle *h) {
+  if (h->ev_used == h->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->ev_pool.push_back(e);
+  }
+  return h->ev_pool[h->ev_used++];
+}
+
+// Force launch, bracketed by events when timing is on.
+void forces_timed(examd_handle *h, int prec, bool energy, bool homecount,
+                  StepLog *log) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    e0 = next_event(h);
+    e1 = next_event(h);
+    if (e0) (void)hipEventRecord(e0, h->stream);
+  }
+  launch_forces(h->d, h->g, lj(h), force_config(h, prec),
+                prec == EXAMD_PREC_MIXED, energy, homecount, log, h->stream);
+  if (e1) (void)hipEventRecord(e1, h->stream);
+}
+
+// Accumulate the recorded event pairs (stream already synchronised).
+void collect_timing(examd_handle *h) {
+  for (size_t k = 0; k + 1 < h->ev_used; k += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev_pool[k], h->ev_pool[k + 1]) ==
+        hipSuccess) {
+      h->force_ms += ms;
+      h->force_launches += 1;
+    }
+  }
+  h->ev_used = 0;
+}
+
+// Bin + sort + force evaluation at the current positions into log[slot].
+void evaluate(examd_handle *h, int prec, bool energy, bool homecount,
+              StepLog *log) {
+  launch_step_bin(h->d, h->g, prec == EXAMD_PREC_MIXED, kBinOnly, 0.0, 0.0,
+                  nullptr, log, h->stream);
+  launch_sort(h->d, h->nblocks, h->stream);
+  forces_timed(h, prec, energy, homecount, log);
+}
+
+uint64_t splitmix64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double uniform01(uint64_t &s) {  // (0, 1)
+  return ((double)(splitmix64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *examd_hip_strerror(int s) {
+  switch (s) {
+    case EXAMD_OK: return "ok";
+    case EXAMD_ERR_ARG: return "invalid argument";
+    case EXAMD_ERR_HIP: return "HIP runtime error";
+    case EXAMD_ERR_NOMEM: return "device allocation failed";
+    case EXAMD_ERR_GRID:
+      return "fewer than 5 blocks per dimension or cutoff >= box/2";
+    case EXAMD_ERR_TILE_OVERFLOW:
+      return "cell-block region exceeds its LDS capacity";
+    case EXAMD_ERR_NO_DEVICE: return "no HIP device";
+    case EXAMD_ERR_NONFINITE: return "non-finite position";
+    case EXAMD_ERR_NOT_READY: return "state not set";
+    case EXAMD_ERR_FIXED_RANGE:
+      return "pair force beyond the fixed-point range";
+    case EXAMD_ERR_ZERO_DISTANCE: return "pair distance^2 below 0.001";
+    case EXAMD_ERR_OUT_OF_BOX: return "block number out of range";
+    default: return "unknown status";
+  }
+}
+
+int examd_hip_domain(const double box[3], double block_length,
+                     int32_t nblocks_xyz[3], int64_t *nblocks) {
+  if (!box || !nblocks_xyz || !nblocks || !(block_length > 0.0))
+    return EXAMD_ERR_ARG;
+  int64_t nb = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (!(box[a] > 0.0)) return EXAMD_ERR_ARG;
+    double q = box[a] / block_length;
+    if (!(q < 2.0e9)) return EXAMD_ERR_ARG;
+    nblocks_xyz[a] = (int32_t)q;  // Int(globalbox/block_length)
+    nb *= nblocks_xyz[a];
+  }
+  *nblocks = nb;
+  return EXAMD_OK;
+}
+
+int examd_hip_create(const examd_params *prm, examd_handle **out) {
+  if (!prm || !out) return EXAMD_ERR_ARG;
+  *out = nullptr;
+  if (prm->natoms < 0 || prm->natoms > (int64_t)2000000000) return EXAMD_ERR_ARG;
+  if (!(prm->cutoff > 0.0) || !(prm->mass > 0.0) || !(prm->sigma > 0.0) ||
+      !(prm->epsilon >= 0.0))
+    return EXAMD_ERR_ARG;
+  int32_t nbxyz[3];
+  int64_t nblocks;
+  int st = examd_hip_domain(prm->box, 0.5 * prm->cutoff, nbxyz, &nblocks);
+  if (st) return st;
+  for (int a = 0; a < 3; ++a)
+    if (nbxyz[a] < 5 || !(prm->cutoff < 0.5 * prm->box[a]))
+      return EXAMD_ERR_GRID;
+  if (nblocks > (int64_t)1 << 30) return EXAMD_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return EXAMD_ERR_NO_DEVICE;
+  if (prm->device < 0 || prm->device >= ndev) return EXAMD_ERR_ARG;
+
+  examd_handle *h = new (std::nothrow) examd_handle();
+  if (!h) return EXAMD_ERR_NOMEM;
+  h->p = *prm;
+  h->dev = prm->device;
+  h->n = prm->natoms;
+  h->nblocks = nblocks;
+  Grid &g = h->g;
+  for (int a = 0; a < 3; ++a) {
+    g.nb[a] = nbxyz[a];
+    g.L[a] = prm->box[a];
+    int T = prm->tiles[a] > 0 ? prm->tiles[a] : (nbxyz[a] + 6) / 7;
+    g.T[a] = std::max(1, std::min(T, (int)nbxyz[a]));
+  }
+  g.bl = 0.5 * prm->cutoff;
+  h->maxrb = max_region_blocks(g);
+  int thr = prm->block_threads;
+  h->tile_threads = (thr == 512 || thr == 1024) ? thr : 256;
+
+  if (hipSetDevice(h->dev) != hipSuccess) {
+    delete h;
+    return EXAMD_ERR_HIP;
+  }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            h->dev) != hipSuccess || lds <= 0)
+    lds = 65536;
+  h->lds_limit = (size_t)lds;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    delete h;
+    return EXAMD_ERR_HIP;
+  }
+  DevState &d = h->d;
+  d.n = h->n;
+  int64_t n = h->n;
+  int64_t nchunk = (nblocks + 2047) / 2048;
+  int rc = EXAMD_OK;
+  rc |= dalloc(&d.x, n) | dalloc(&d.y, n) | dalloc(&d.z, n);
+  rc |= dalloc(&d.vx, n) | dalloc(&d.vy, n) | dalloc(&d.vz, n);
+  rc |= dalloc(&d.fx, n) | dalloc(&d.fy, n) | dalloc(&d.fz, n);
+  rc |= dalloc(&d.qx, n) | dalloc(&d.qy, n) | dalloc(&d.qz, n);
+  rc |= dalloc(&d.cell, n) | dalloc(&d.rank, n) | dalloc(&d.sorted, n);
+  rc |= dalloc(&d.count, nblocks) | dalloc(&d.start, nblocks + 1);
+  rc |= dalloc(&d.chunk, nchunk);
+  if (rc == EXAMD_OK) rc = ensure_log(h, 2);
+  if (rc != EXAMD_OK) {
+    free_state(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return EXAMD_ERR_NOMEM;
+  }
+  launch_fill_i32(d.count, 0, nblocks, h->stream);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    free_state(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return EXAMD_ERR_HIP;
+  }
+  *out = h;
+  return EXAMD_OK;
+}
+
+int examd_hip_destroy(examd_handle *h) {
+  if (!h) return EXAMD_ERR_ARG;
+  (void)hipSetDevice(h->dev);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_state(h);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return EXAMD_OK;
+}
+
+int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
+                        int64_t n) {
+  if (!h || !pos || n != h->n) return EXAMD_ERR_ARG;
+  HIPCHK(hipSetDevice(h->dev));
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(pos[i])) return EXAMD_ERR_NONFINITE;
+  int st = choose_caps(h, pos);
+  if (st) return st;
+  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
+  double *dst[3] = {h->d.x, h->d.y, h->d.z};
+  double *vdst[3] = {h->d.vx, h->d.vy, h->d.vz};
+  for (int a = 0; a < 3; ++a) {
+    for (int64_t i = 0; i < n; ++i) buf[i] = pos[3 * i + a];
+    HIPCHK(hipMemcpy(dst[a], buf.data(), n * sizeof(double),
+                     hipMemcpyHostToDevice));
+    for (int64_t i = 0; i < n; ++i) buf[i] = vel ? vel[3 * i + a] : 0.0;
+    HIPCHK(hipMemcpy(vdst[a], buf.data(), n * sizeof(double),
+                     hipMemcpyHostToDevice));
+  }
+  h->state_set = true;
+  h->forces_valid = -1;
+  return EXAMD_OK;
+}
+
+int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
+                        double *force, int64_t n) {
+  if (!h || n != h->n) return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
+  const double *src[3] = {h->d.x, h->d.y, h->d.z};
+  const double *vsrc[3] = {h->d.vx, h->d.vy, h->d.vz};
+  for (int a = 0; a < 3; ++a) {
+    if (pos) {
+      HIPCHK(hipMemcpy(buf.data(), src[a], n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; ++i) pos[3 * i + a] = buf[i];
+    }
+    if (vel) {
+      HIPCHK(hipMemcpy(buf.data(), vsrc[a], n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; ++i) vel[3 * i + a] = buf[i];
+    }
+  }
+  if (force) {
+    if (h->forces_valid < 0) return EXAMD_ERR_NOT_READY;
+    double *tmp = nullptr;
+    if (dalloc(&tmp, 3 * n)) return EXAMD_ERR_NOMEM;
+    launch_forces_to_fp64(h->d, tmp, h->forces_valid == EXAMD_PREC_MIXED,
+                          h->stream);
+    hipError_t e = hipMemcpyAsync(force, tmp, 3 * n * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->stream);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess || e2 != hipSuccess) return EXAMD_ERR_HIP;
+  }
+  return EXAMD_OK;
+}
+
+static void fill_stats(const examd_handle *h, const StepLog &L, int prec,
+                       int64_t step, examd_stats *s) {
+  double pe = prec == EXAMD_PREC_MIXED
+                  ? (double)(long long)L.pe_fix * kFixInv
+                  : L.pe;
+  double rc = h->p.cutoff, sr2 = h->p.sigma * h->p.sigma / (rc * rc);
+  double sr6 = sr2 * sr2 * sr2;
+  double vrc = 4.0 * h->p.epsilon * (sr6 * sr6 - sr6);
+  s->step = step;
+  s->ke = 0.5 * h->p.mass * L.ke;
+  s->pe = pe;
+  s->pe_shifted = pe - (double)L.cut * vrc;
+  s->etot = s->ke + pe;
+  s->pairs_cand = (int64_t)L.cand;
+  s->pairs_cut = (int64_t)L.cut;
+}
+
+int examd_hip_forces(examd_handle *h, int precision, double *forces,
+                     examd_stats *stats) {
+  if (!h || (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  HIPCHK(hipSetDevice(h->dev));
+  launch_fill_log(h->log, 1, h->stream);
+  evaluate(h, precision, true, false, h->log);
+  // kinetic energy of the given velocities (h = 0 leaves v unchanged)
+  launch_kick(h->d, precision == EXAMD_PREC_MIXED, 0.0, h->log, h->stream);
+  HIPCHK(hipGetLastError());
+  StepLog L;
+  HIPCHK(hipMemcpyAsync(&L, h->log, sizeof(L), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->timing) collect_timing(h);
+  h->forces_valid = precision;
+  int st = flags_to_status(L.flags);
+  if (stats) fill_stats(h, L, precision, 0, stats);
+  if (st != EXAMD_OK && st != EXAMD_ERR_ZERO_DISTANCE) {
+    h->forces_valid = -1;
+    return st;
+  }
+  if (forces) {
+    double *tmp = nullptr;
+    if (dalloc(&tmp, 3 * h->n)) return EXAMD_ERR_NOMEM;
+    launch_forces_to_fp64(h->d, tmp, precision == EXAMD_PREC_MIXED,
+                          h->stream);
+    hipError_t e = hipMemcpyAsync(forces, tmp, 3 * h->n * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->stream);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess || e2 != hipSuccess) return EXAMD_ERR_HIP;
+  }
+  return st;
+}
+
+int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
+                  int64_t energy_every, examd_stats *out, int64_t out_cap,
+                  int64_t *n_out) {
+  if (!h || nsteps < 0 || !std::isfinite(dt) ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  if (n_out) *n_out = 0;
+  HIPCHK(hipSetDevice(h->dev));
+  const bool mixed = precision == EXAMD_PREC_MIXED;
+  if (ensure_log(h, nsteps + 1)) return EXAMD_ERR_NOMEM;
+  launch_fill_log(h->log, (int)(nsteps + 1), h->stream);
+  auto is_energy = [&](int64_t s) {
+    return s == nsteps || (energy_every > 0 && s % energy_every == 0);
+  };
+  if (h->forces_valid != precision) evaluate(h, precision, true, false, h->log);
+  const double hk = (0.5 * dt) / h->p.mass;
+  for (int64_t s = 1; s <= nsteps; ++s) {
+    StepLog *ke_prev = (s > 1 && is_energy(s - 1)) ? h->log + (s - 1) : nullptr;
+    launch_step_bin(h->d, h->g, mixed,
+                    s == 1 ? kKickDriftBin : kFinishKickDriftBin, hk, dt,
+                    ke_prev, h->log + s, h->stream);
+    launch_sort(h->d, h->nblocks, h->stream);
+    forces_timed(h, precision, is_energy(s), false, h->log + s);
+  }
+  if (nsteps > 0)
+    launch_kick(h->d, mixed, hk, is_energy(nsteps) ? h->log + nsteps : nullptr,
+                h->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<StepLog> L((size_t)nsteps + 1);
+  HIPCHK(hipMemcpyAsync(L.data(), h->log, (nsteps + 1) * sizeof(StepLog),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->timing) collect_timing(h);
+  h->forces_valid = precision;
+  unsigned long long fl = 0;
+  for (auto &x : L) fl |= x.flags;
+  int64_t k = 0;
+  for (int64_t s = 1; s <= nsteps && out; ++s) {
+    if (!is_energy(s) || k >= out_cap) continue;
+    fill_stats(h, L[s], precision, s, &out[k++]);
+  }
+  if (n_out) *n_out = k;
+  int st = flags_to_status(fl);
+  if (st != EXAMD_OK) h->forces_valid = -1;
+  return st;
+}
+
+int examd_hip_block_ids(examd_handle *h, int64_t *blocknum, int32_t *blockxyz,
+                        int64_t n) {
+  if (!h || !blocknum || n != h->n) return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  HIPCHK(hipSetDevice(h->dev));
+  launch_fill_log(h->log, 1, h->stream);
+  launch_step_bin(h->d, h->g, false, kBinOnly, 0.0, 0.0, nullptr, h->log,
+                  h->stream);
+  // clear the counts the binning pass left behind
+  launch_fill_i32(h->d.count, 0, h->nblocks, h->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<int> cell((size_t)std::max<int64_t>(n, 1));
+  StepLog L;
+  HIPCHK(hipMemcpyAsync(cell.data(), h->d.cell, n * sizeof(int),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(&L, h->log, sizeof(L), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->forces_valid = -1;  // force accumulators were cleared
+  const int64_t nx = h->g.nb[0], ny = h->g.nb[1];
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t b = cell[i];
+    blocknum[i] = b + 1;
+    if (blockxyz) {
+      blockxyz[3 * i + 0] = (int32_t)(b % nx + 1);
+      blockxyz[3 * i + 1] = (int32_t)((b / nx) % ny + 1);
+      blockxyz[3 * i + 2] = (int32_t)(b / (nx * ny) + 1);
+    }
+  }
+  return flags_to_status(L.flags & (kFlagOutOfBox | kFlagNonFinite));
+}
+
+int examd_hip_home_pair_counts(examd_handle *h, int precision, int64_t *counts,
+                               int64_t nblocks) {
+  if (!h || !counts || nblocks != h->nblocks ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  HIPCHK(hipSetDevice(h->dev));
+  if (!h->d.home && dalloc(&h->d.home, h->nblocks)) return EXAMD_ERR_NOMEM;
+  launch_fill_u64(h->d.home, h->nblocks, h->stream);
+  launch_fill_log(h->log, 1, h->stream);
+  evaluate(h, precision, false, true, h->log);
+  HIPCHK(hipGetLastError());
+  StepLog L;
+  HIPCHK(hipMemcpyAsync(counts, h->d.home, nblocks * sizeof(int64_t),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(&L, h->log, sizeof(L), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->timing) collect_timing(h);
+  h->forces_valid = precision;
+  return flags_to_status(L.flags & ~(unsigned long long)kFlagZero);
+}
+
+int examd_hip_timing(examd_handle *h, int enable, double *force_ms,
+                     int64_t *force_launches, int reset) {
+  if (!h) return EXAMD_ERR_ARG;
+  if (force_ms) *force_ms = h->force_ms;
+  if (force_launches) *force_launches = h->force_launches;
+  if (reset) {
+    h->force_ms = 0.0;
+    h->force_launches = 0;
+  }
+  h->timing = enable != 0;
+  return EXAMD_OK;
+}
+
+int examd_hip_generate_fcc(const int32_t nc[3], double rho, double temperature,
+                           double mass, double perturb, uint64_t seed,
+                           double *pos, double *vel, double box[3]) {
+  if (!nc || !pos || !box || !(rho > 0.0) || !(mass > 0.0) ||
+      !(temperature >= 0.0) || !(perturb >= 0.0))
+    return EXAMD_ERR_ARG;
+  for (int a = 0; a < 3; ++a)
+    if (nc[a] <= 0) return EXAMD_ERR_ARG;
+  const double cell = std::cbrt(4.0 / rho);
+  for (int a = 0; a < 3; ++a) box[a] = nc[a] * cell;
+  static const double basis[4][3] = {
+      {0.0, 0.0, 0.0}, {0.5, 0.5, 0.0}, {0.5, 0.0, 0.5}, {0.0, 0.5, 0.5}};
+  uint64_t s = seed;
+  int64_t i = 0;
+  for (int k = 0; k < nc[2]; ++k)
+    for (int j = 0; j < nc[1]; ++j)
+      for (int m = 0; m < nc[0]; ++m)
+        for (int b = 0; b < 4; ++b, ++i) {
+          const int c[3] = {m, j, k};
+          for (int a = 0; a < 3; ++a) {
+            double x = (c[a] + basis[b][a]) * cell;
+            if (perturb > 0.0) x += perturb * (2.0 * uniform01(s) - 1.0);
+            double L = box[a];
+            x = x - std::floor(x / L) * L;
+            if (x >= L) x -= L;
+            if (x < 0.0) x = 0.0;
+            pos[3 * i + a] = x;
+          }
+        }
+  const int64_t n = i;
+  if (vel) {
+    const double sd = std::sqrt(temperature / mass);
+    for (int64_t t = 0; t < 3 * n; t += 2) {
+      // Box-Muller pairs
+      double u1 = uniform01(s), u2 = uniform01(s);
+      double r = std::sqrt(-2.0 * std::log(u1));
+      vel[t] = sd * r * std::cos(2.0 * M_PI * u2);
+      if (t + 1 < 3 * n) vel[t + 1] = sd * r * std::sin(2.0 * M_PI * u2);
+    }
+    double mom[3] = {0, 0, 0};
+    for (int64_t t = 0; t < n; ++t)
+      for (int a = 0; a < 3; ++a) mom[a] += vel[3 * t + a];
+    for (int a = 0; a < 3; ++a) mom[a] /= (double)n;
+    double v2 = 0.0;
+    for (int64_t t = 0; t < n; ++t)
+      for (int a = 0; a < 3; ++a) {
+        vel[3 * t + a] -= mom[a];
+        v2 += vel[3 * t + a] * vel[3 * t + a];
+      }
+    double target = (3.0 * n - 3.0) * temperature / mass;  // sum v^2
+    double scale = (v2 > 0.0 && n > 1) ? std::sqrt(target / v2) : 0.0;
+    for (int64_t t = 0; t < 3 * n; ++t) vel[t] *= scale;
+  }
+  return EXAMD_OK;
+}
+
+}  // extern "C"

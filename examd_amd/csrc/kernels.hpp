@@ -1,0 +1,58 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace examd {
+
+// Device state of one system.  Per-atom arrays are SoA in input order; the
+// block-sorted order exists only as the index array `sorted`.
+struct DevState {
+  int64_t n = 0;
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  double *fx = nullptr, *fy = nullptr, *fz = nullptr;          // FP64 force
+  unsigned long long *qx = nullptr, *qy = nullptr, *qz = nullptr;  // 2^-32
+  int *cell = nullptr;     // 0-based block id per atom
+  int *rank = nullptr;     // provisional slot inside the block
+  int *count = nullptr;    // atoms per block (nblocks)
+  int *start = nullptr;    // exclusive scan of count (nblocks + 1)
+  int *sorted = nullptr;   // atom ids grouped by block, input order inside
+  int *chunk = nullptr;    // scan partials
+  StepLog *log = nullptr;  // per-step tallies
+  unsigned long long *home = nullptr;  // home-block pair counts (diagnostic)
+};
+
+struct LJParams {
+  double eps, sig, rc, mass;
+};
+
+// Tunables of the force kernel.
+struct ForceConfig {
+  int threads = 256;  // workgroup size: 256, 512 or 1024
+  int cap = 0;        // LDS atom capacity per cell block
+  int maxrb = 0;      // largest region (home + shell) in blocks
+};
+
+size_t force_lds_bytes(bool fp64, int cap, int maxrb);
+
+// Binning / integration modes of launch_step_bin.
+enum BinMode { kBinOnly = 0, kKickDriftBin = 1, kFinishKickDriftBin = 2 };
+
+void launch_fill_i32(int *p, int v, int64_t n, hipStream_t s);
+void launch_fill_log(StepLog *p, int n, hipStream_t s);
+void launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
+
+void launch_step_bin(const DevState &d, const Grid &g, bool mixed, int mode,
+                     double h, double dt, StepLog *ke_log, StepLog *flag_log,
+                     hipStream_t s);
+void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
+                 hipStream_t s);
+void launch_sort(const DevState &d, int64_t nblocks, hipStream_t s);
+void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
+                   const ForceConfig &c, bool mixed, bool energy,
+                   bool homecount, StepLog *log, hipStream_t s);
+void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,
+                           hipStream_t s);
+
+}  // namespace examd

@@ -1,0 +1,164 @@
+/*
+ * examd_hip.h -- C ABI of the MI355X (gfx950) midpoint-method Lennard-Jones
+ * force path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns
+ * EXAMD_OK (0) or a status code, and never aborts the process.  Host arrays
+ * are borrowed for the duration of the call.  Position, velocity and force
+ * arrays use the reference's Fortran layout posn(1:3, 1:n): x, y, z of atom
+ * 1, then of atom 2, ... (column-major (3,n)), indexed in input order.
+ *
+ * Reference interfaces each entry point stands in for are cited as
+ * file:line of exaMD v2.0.  A handle is not thread-safe; each handle owns
+ * one HIP stream on one device.
+ */
+#ifndef EXAMD_HIP_H
+#define EXAMD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The numbers 101-109 are the reference's MPI_Abort codes
+ * where the condition is the same one. */
+enum {
+    EXAMD_OK = 0,
+    EXAMD_ERR_ARG = 1,          /* bad argument or size */
+    EXAMD_ERR_HIP = 2,          /* HIP runtime error */
+    EXAMD_ERR_NOMEM = 3,        /* allocation failed (ref. 103-108) */
+    EXAMD_ERR_GRID = 4,         /* < 5 blocks per side or rc >= L/2: the
+                                   63-row table would alias */
+    EXAMD_ERR_TILE_OVERFLOW = 5,/* a cell block's region exceeded its LDS
+                                   capacity */
+    EXAMD_ERR_NO_DEVICE = 6,    /* no HIP device */
+    EXAMD_ERR_NONFINITE = 7,    /* NaN/Inf force or position */
+    EXAMD_ERR_NOT_READY = 8,    /* state not set */
+    EXAMD_ERR_FIXED_RANGE = 9,  /* a pair force exceeded the fixed-point
+                                   conversion range (|f| >= 2^19) */
+    EXAMD_ERR_ZERO_DISTANCE = 10,/* a pair closer than sqrt(0.001): the
+                                   reference prints 'ZERO at' */
+    EXAMD_ERR_OUT_OF_BOX = 109  /* block number out of range (ref. 109) */
+};
+
+/* Arithmetic of one force evaluation. */
+enum {
+    EXAMD_PREC_FP64 = 0,  /* FP64 throughout, reference arithmetic order */
+    EXAMD_PREC_MIXED = 1  /* FP32 pair arithmetic, int64 fixed-point (2^-32)
+                             force and energy accumulation, FP64 integration */
+};
+
+typedef struct examd_handle examd_handle;
+
+/* System parameters.  ε = prmvdw(1,1), σ = prmvdw(1,2), rc = prmvdw(1,5)
+ * (exaMD.f90:202); block_length = rc/2 (exaMD.f90:56). */
+typedef struct examd_params {
+    int64_t natoms;
+    double box[3];        /* globalbox (exaMD.f90:53) */
+    double cutoff;        /* rc */
+    double epsilon;
+    double sigma;
+    double mass;
+    int32_t tiles[3];     /* cell blocks per dimension (workgroup tiles);
+                             0 = choose */
+    int32_t device;       /* HIP device ordinal */
+    int32_t block_threads;/* force-kernel workgroup size (0 = choose;
+                             256, 512 or 1024) */
+    int32_t reserved[5];
+} examd_params;
+
+/* Tallies of one step or one force evaluation. */
+typedef struct examd_stats {
+    int64_t step;         /* step index (0 = the state given) */
+    double ke;            /* 1/2 m sum v^2 (FP64) */
+    double pe;            /* sum 4 eps (s^12 - s^6) over in-cutoff pairs */
+    double pe_shifted;    /* pe - pairs_cut * V(rc) */
+    double etot;          /* ke + pe */
+    int64_t pairs_cand;   /* candidate pairs visited by the midpoint stencil */
+    int64_t pairs_cut;    /* pairs with r^2 <= rc^2 */
+} examd_stats;
+
+/* Block grid of exaMD_domain_create (SRC/exaMD_utils.f90:167-190):
+ * nblocks_xyz = Int(box/block_length).  Pure host computation. */
+int examd_hip_domain(const double box[3], double block_length,
+                     int32_t nblocks_xyz[3], int64_t *nblocks);
+
+/* Set up a handle (device buffers, stream, cell-block map).  Replaces
+ * exaMD_Init + exaMD_domain_create + exaMD_partition_domain +
+ * exaMD_setup_partitioned_grid + exaMD_setup_midpoint_interactions
+ * (SRC/exaMD_utils.f90:38, :167; SRC/exaMD_partitioning.f90:65, :670;
+ * SRC/exaMD_module.f90:276). */
+int examd_hip_create(const examd_params *params, examd_handle **out);
+
+/* Release everything (SRC/exaMD_utils.f90:90 exaMD_Finalize). */
+int examd_hip_destroy(examd_handle *h);
+
+/* Upload positions (and velocities when vel != NULL; zero otherwise).
+ * Replaces the rank-0 scatter of exaMD_partition_work
+ * (SRC/exaMD_partitioning.f90:889-1121).  Positions must lie in the box as
+ * exaMD_identify_block accepts them. */
+int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
+                        int64_t n);
+
+/* Download the current state; any pointer may be NULL.  force is the last
+ * evaluated force (FP64 or fixed point converted to FP64). */
+int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
+                        double *force, int64_t n);
+
+/* One force evaluation at the current positions:
+ * exaMD_forces_midpoint_all(user_func = lennardjones_12, cutoff, noprocs)
+ * (SRC/exaMD_forces.f90:308-390, :492-551; user_func.f90:40-75), with the
+ * halo exchange of exaMD_start_comms_posn / exaMD_wait_comms_forces
+ * (SRC/exaMD_comms.f90:44, :330) done as an on-device ghost gather and
+ * ghost-force return.  forces (3,n) receives the force (overwritten, not
+ * accumulated; may be NULL); stats may be NULL. */
+int examd_hip_forces(examd_handle *h, int precision, double *forces,
+                     examd_stats *stats);
+
+/* Velocity-Verlet MD for nsteps steps of dt with rebinning every step (no
+ * reference counterpart: the reference evaluates forces once).  Energies
+ * are tallied every energy_every steps (0 = only the last step) and written
+ * to out[0..*n_out) when out != NULL (capacity out_cap). */
+int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
+                  int64_t energy_every, examd_stats *out, int64_t out_cap,
+                  int64_t *n_out);
+
+/* Cell assignment of the current positions as exaMD_identify_block gives it
+ * (SRC/exaMD_module.f90:101-125): 1-based block number per atom, and the
+ * 1-based (i,j,k) when blockxyz (3,n) is not NULL. */
+int examd_hip_block_ids(examd_handle *h, int64_t *blocknum, int32_t *blockxyz,
+                        int64_t n);
+
+/* Pair ownership: the number of in-cutoff pairs each home block computed in
+ * a force evaluation at the current positions (0-based block id, nblocks
+ * entries), as the 63-row table of SRC/exaMD_module.f90:276-346 assigns them
+ * through SRC/exaMD_forces.f90:349-378. */
+int examd_hip_home_pair_counts(examd_handle *h, int precision,
+                               int64_t *counts, int64_t nblocks);
+
+/* Force-kernel timing over the calls since the last reset: enable, read
+ * (total milliseconds of force-kernel launches measured with HIP events on
+ * the handle's stream, and the number of launches), reset. */
+int examd_hip_timing(examd_handle *h, int enable, double *force_ms,
+                     int64_t *force_launches, int reset);
+
+/* Seeded initial states (not in the reference, whose generator uses an
+ * unseeded RNG, SRC/exaMD_input_formats.f90:41-96).
+ * FCC lattice of nc[0]*nc[1]*nc[2] cells of 4 atoms at number density rho,
+ * atom order: cell x fastest, then y, then z, basis order (0,0,0),
+ * (1/2,1/2,0), (1/2,0,1/2), (0,1/2,1/2); perturbed by uniform(-p, p) when
+ * perturb > 0; box written to box[3].  Maxwell-Boltzmann velocities at
+ * temperature T (k_B = 1), zero net momentum, rescaled so that
+ * KE = (3n-3)/2 T exactly.  pos and vel are (3,n). */
+int examd_hip_generate_fcc(const int32_t nc[3], double rho, double temperature,
+                           double mass, double perturb, uint64_t seed,
+                           double *pos, double *vel, double box[3]);
+
+/* Human-readable text of a status code. */
+const char *examd_hip_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

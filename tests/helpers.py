@@ -1,0 +1,128 @@
+"""Independent mathematical references used by the tests.
+
+Nothing here follows the reference's block/table loop: forces come from an
+all-pairs minimum-image sum, ownership from a pair-centric application of
+the 63-row table, lattice energies from a direct lattice-vector sum.  They
+pin the oracle; the oracle pins the HIP library.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+# interactions(1:2, 1:63) of SRC/exaMD_module.f90:282-344, as data
+TABLE = [(14, 14), (14, 15), (14, 16), (14, 17), (14, 18), (14, 19), (14, 20),
+         (14, 21), (14, 22), (14, 23), (14, 24), (14, 25), (14, 26), (14, 27),
+         (10, 18), (10, 27), (11, 17), (11, 18), (11, 25), (11, 26), (11, 27),
+         (12, 16), (12, 17), (12, 25), (13, 15), (13, 18), (13, 24), (13, 27),
+         (15, 16), (15, 19), (15, 22), (15, 25), (16, 21), (16, 24), (17, 19),
+         (17, 20), (17, 21), (18, 19), (19, 5), (19, 9), (20, 5), (20, 7),
+         (20, 8), (20, 9), (21, 5), (21, 7), (22, 5), (22, 6), (22, 9),
+         (23, 5), (24, 5), (24, 4), (24, 7), (25, 5), (25, 3), (25, 6),
+         (26, 5), (26, 2), (26, 3), (27, 1), (27, 2), (27, 4), (27, 5)]
+
+
+def stencil(n):
+    """neighrefs index n (1..27) -> offset, ii fastest (numpy unravel)."""
+    kk, jj, ii = np.unravel_index(n - 1, (3, 3, 3))
+    return (int(ii) - 1, int(jj) - 1, int(kk) - 1)
+
+
+def fcc(nc, rho):
+    """Perfect FCC lattice, cells x fastest; returns (pos, L)."""
+    a = (4.0 / rho) ** (1.0 / 3.0)
+    basis = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    k, j, i = np.meshgrid(np.arange(nc), np.arange(nc), np.arange(nc),
+                          indexing="ij")
+    cells = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1)
+    pos = (cells[:, None, :] + basis[None]).reshape(-1, 3) * a
+    return pos, nc * a
+
+
+def brute_force(pos, box, rc, eps=1.0, sig=1.0):
+    """All-pairs minimum-image LJ: (forces, pe, n_cut)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    box = np.asarray(box, dtype=np.float64).reshape(3)
+    n = len(pos)
+    f = np.zeros((n, 3))
+    pe = 0.0
+    ncut = 0
+    blk = 512
+    for s in range(0, n, blk):
+        d = pos[None, :, :] - pos[s:s + blk, None, :]      # x_j - x_i
+        d -= np.round(d / box) * box
+        r2 = (d ** 2).sum(-1)
+        idx = np.arange(s, min(n, s + blk))
+        r2[np.arange(len(idx)), idx] = np.inf
+        m = r2 <= rc * rc
+        sr6 = np.where(m, (sig * sig / np.where(m, r2, 1.0)) ** 3, 0.0)
+        dg = np.where(m, -24 * eps * (2 * sr6 * sr6 - sr6) / np.where(m, r2, 1.0), 0.0)
+        f[idx] += (dg[:, :, None] * d).sum(1)
+        pe += 0.5 * np.where(m, 4 * eps * (sr6 * sr6 - sr6), 0.0).sum()
+        ncut += int(m.sum())
+    return f, pe, ncut // 2
+
+
+def block_of(pos, bl, nb):
+    """exaMD_identify_block as written: Int(x/bl + 1) clamped (0-based)."""
+    q = np.floor(np.asarray(pos) / bl + 1.0).astype(np.int64)
+    q = np.minimum(q, np.asarray(nb)) - 1
+    return q
+
+
+def ownership_counts(pos, box, rc):
+    """In-cutoff pairs per home block, pair by pair.
+
+    For a pair with block offset d (minimum periodic offset, |d_a| <= 2),
+    the row of the table whose o2 - o1 is d or -d fixes the home block
+    (block of the o1-side atom minus o1)."""
+    box = np.asarray(box, dtype=np.float64)
+    bl = rc / 2
+    nb = (box / bl).astype(np.int64)
+    b = block_of(pos, bl, nb)
+    rows = {}
+    for (n1, n2) in TABLE:
+        o1, o2 = np.array(stencil(n1)), np.array(stencil(n2))
+        rows[tuple(o2 - o1)] = o1
+    counts = np.zeros(int(np.prod(nb)), dtype=np.int64)
+    n = len(pos)
+    for i in range(n):
+        d = pos[i + 1:] - pos[i]
+        d -= np.round(d / box) * box
+        r2 = (d ** 2).sum(1)
+        for jj in np.nonzero(r2 <= rc * rc)[0]:
+            j = i + 1 + jj
+            db = b[j] - b[i]
+            db = (db + nb // 2) % nb - nb // 2      # periodic, |d| <= 2
+            key = tuple(int(v) for v in db)
+            if key in rows:
+                home = b[i] - rows[key]
+            else:
+                home = b[j] - rows[tuple(-v for v in key)]
+            home %= nb
+            counts[home[0] + nb[0] * (home[1] + nb[1] * home[2])] += 1
+    return counts
+
+
+def fcc_lattice_pe_per_atom(rho, rc, eps=1.0, sig=1.0):
+    """Half the sum of 4 eps((s/R)^12 - (s/R)^6) over FCC lattice vectors."""
+    a = (4.0 / rho) ** (1.0 / 3.0)
+    m = int(np.ceil(2 * rc / a)) + 1
+    e = 0.0
+    for i, j, k in itertools.product(range(-2 * m, 2 * m + 1), repeat=3):
+        if (i + j + k) % 2 or (i == j == k == 0):
+            continue
+        r2 = (a / 2) ** 2 * (i * i + j * j + k * k)
+        if r2 <= rc * rc:
+            s6 = (sig * sig / r2) ** 3
+            e += 4 * eps * (s6 * s6 - s6)
+    return 0.5 * e
+
+
+def perturbed_fcc(nc, rho, amp, seed):
+    pos, L = fcc(nc, rho)
+    rng = np.random.default_rng(seed)
+    pos = np.mod(pos + rng.uniform(-amp, amp, pos.shape), L)
+    pos[pos >= L] = 0.0
+    return pos, L

@@ -1,0 +1,199 @@
+"""HIP path against the oracle, through the C ABI, on the GPU.
+
+Bars: cell assignment, candidate/in-cutoff pair counts and per-home-block
+pair ownership bit-exact; FP64 per-atom forces within 1e-10 relative
+(normwise: max |dF_i| / RMS |F|) and energy within 1e-10 relative; the mixed
+path within the FP32 bars stated per test, with bit-exact fixed-point
+invariants (sum F = 0, order and tile-size independence).
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+RHO = 0.8442
+FP64_TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def E():
+    import examd_amd
+    examd_amd.lib()
+    return examd_amd
+
+
+def _system(E, pos, box, rc=2.5, **kw):
+    m = E.MidpointLJ(len(pos), box, rc, **kw)
+    m.set_state(pos)
+    return m
+
+
+def _rel(f, ref):
+    return np.abs(f - ref).max() / np.sqrt((ref ** 2).sum(1).mean())
+
+
+def _case(nc=10, amp=0.05, seed=1):
+    pos, L = H.perturbed_fcc(nc, RHO, amp, seed)
+    return pos, np.array([L, L, L])
+
+
+@pytest.mark.parametrize("nc,amp,seed", [(10, 0.05, 1), (12, 0.2, 2), (7, 0.1, 3)])
+def test_fp64_forces_match_oracle(E, oracle, nc, amp, seed):
+    print(f"[fp64 parity] nc={nc} amp={amp} seed={seed}", flush=True)
+    pos, box = _case(nc, amp, seed)
+    ref, t, _ = oracle.forces(pos, box, 2.5)
+    with _system(E, pos, box) as m:
+        f, s, _ = m.forces("fp64")
+    assert _rel(f, ref) < FP64_TOL
+    assert s.pairs_cand == t.cand
+    assert s.pairs_cut == t.cut
+    assert abs(s.pe - t.pe) < FP64_TOL * abs(t.pe)
+
+
+def test_fp64_config1_shape(E, oracle):
+    """Config-1 shape: 32,000 atoms, 20^3 FCC, 26^3 blocks, 4x4x4 cell blocks."""
+    print("[fp64 parity] config-1 shape 32000 atoms", flush=True)
+    pos, vel, box = E.generate_fcc(20, RHO, 0.72, perturb=0.05, seed=7)
+    ref, t, _ = oracle.forces(pos, box, 2.5)
+    with _system(E, pos, box, tiles=(4, 4, 4)) as m:
+        f, s, _ = m.forces("fp64")
+    assert _rel(f, ref) < FP64_TOL
+    assert (s.pairs_cand, s.pairs_cut) == (t.cand, t.cut)
+
+
+def test_block_ids_bit_exact(E, oracle):
+    print("[block ids]", flush=True)
+    pos, box = _case(10, 0.3, 4)
+    bl = 1.25
+    # add atoms exactly on faces, one ulp below faces, in the enlarged block
+    nb = int(box[0] / bl)
+    extra = []
+    for k in range(nb):
+        x = k * bl
+        extra.append([x, np.nextafter(x, 0) if k else 0.0, box[0] - 1e-12])
+    pos = np.vstack([pos, np.array(extra)])
+    d = oracle.domain_create(box, bl)
+    want = np.array([oracle.identify_block(d, p)[2] for p in pos])
+    with _system(E, pos, box) as m:
+        num, xyz = m.block_ids()
+    np.testing.assert_array_equal(num, want)
+    np.testing.assert_array_equal(
+        num, xyz[:, 0] + (xyz[:, 1] - 1) * nb + (xyz[:, 2] - 1) * nb * nb)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_home_pair_ownership_bit_exact(E, oracle, prec):
+    print(f"[ownership] {prec}", flush=True)
+    pos, box = _case(9, 0.1, 5)
+    _, t, hc = oracle.forces(pos, box, 2.5, home_counts=True)
+    with _system(E, pos, box, tiles=(3, 2, 4)) as m:
+        got = m.home_pair_counts(prec)
+    if prec == "fp64":
+        np.testing.assert_array_equal(got, hc)
+    else:
+        # FP32 r^2 may move a pair within ~1e-7 of rc across the cutoff
+        assert np.abs(got - hc).sum() <= max(2, int(1e-5 * t.cut))
+
+
+@pytest.mark.parametrize("tiles,threads", [((1, 1, 1), 256), ((2, 3, 5), 512),
+                                           ((13, 13, 13), 256), ((4, 4, 4), 1024)])
+def test_fp64_tile_shapes(E, oracle, tiles, threads):
+    print(f"[tiles] {tiles} threads={threads}", flush=True)
+    pos, box = _case(10, 0.1, 6)
+    ref, t, _ = oracle.forces(pos, box, 2.5)
+    with _system(E, pos, box, tiles=tiles, block_threads=threads) as m:
+        f, s, _ = m.forces("fp64")
+    assert _rel(f, ref) < FP64_TOL
+    assert (s.pairs_cand, s.pairs_cut) == (t.cand, t.cut)
+
+
+def test_mixed_forces_near_oracle(E, oracle):
+    print("[mixed parity]", flush=True)
+    pos, box = _case(12, 0.05, 8)
+    ref, t, _ = oracle.forces(pos, box, 2.5)
+    with _system(E, pos, box) as m:
+        f, s, _ = m.forces("mixed")
+    err = np.sqrt(((f - ref) ** 2).sum(1))
+    scale = np.sqrt((ref ** 2).sum(1).mean())
+    assert err.max() / scale < 2e-5
+    assert np.sqrt((err ** 2).mean()) / scale < 2e-6
+    assert s.pairs_cand == t.cand
+    assert abs(s.pairs_cut - t.cut) <= 2
+    assert abs(s.pe - t.pe) < 1e-6 * abs(t.pe)
+
+
+def test_mixed_fixed_point_invariants(E):
+    """Sum F = 0 exactly; bit-identical forces under atom permutation and
+    under a different workgroup size (order-independent fixed-point
+    accumulation).  A different cell-block map moves the FP32 coordinate
+    origin, so it agrees to FP32 rounding only."""
+    print("[mixed invariants]", flush=True)
+    pos, box = _case(11, 0.1, 9)
+    with _system(E, pos, box, tiles=(3, 5, 2)) as m:
+        f1, _, _ = m.forces("mixed")
+        f1b, _, _ = m.forces("mixed")
+    assert np.array_equal(f1, f1b)
+    q = np.round(f1 * 2.0 ** 32).astype(np.int64)
+    assert np.all(q.sum(0) == 0)
+    perm = np.random.default_rng(1).permutation(len(pos))
+    with _system(E, pos[perm], box, tiles=(3, 5, 2), block_threads=512) as m:
+        f2, _, _ = m.forces("mixed")
+    assert np.array_equal(f2, f1[perm])
+    with _system(E, pos, box, tiles=(4, 4, 4), block_threads=1024) as m:
+        f3, _, _ = m.forces("mixed")
+    assert _rel(f3, f1) < 2e-5
+
+
+def test_fp64_run_tracks_oracle(E, oracle):
+    print("[fp64 run]", flush=True)
+    pos, vel, box = E.generate_fcc(8, RHO, 0.72, perturb=0.02, seed=3)
+    p_ref, v_ref, f_ref, ke, pe, cut = oracle.run_vv(pos, vel, box, 2.5,
+                                                     0.005, 10)
+    with E.MidpointLJ(len(pos), box, 2.5) as m:
+        m.set_state(pos, vel)
+        st = m.run(10, 0.005, "fp64", energy_every=1)
+        p, v = m.get_state()
+    dp = p - p_ref
+    dp -= np.round(dp / box) * box
+    assert np.abs(dp).max() < 1e-10
+    assert np.abs(v - v_ref).max() < 1e-10
+    assert len(st) == 10
+    for k in range(10):
+        assert abs(st[k].ke - ke[k]) < 1e-10 * ke[k]
+        assert abs(st[k].pe - pe[k]) < 1e-10 * abs(pe[k])
+        assert st[k].pairs_cut == cut[k]
+
+
+def test_mixed_energy_conservation(E):
+    print("[mixed nve]", flush=True)
+    pos, vel, box = E.generate_fcc(10, RHO, 0.72, seed=11)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with E.MidpointLJ(len(pos), box, 2.5) as m:
+            m.set_state(pos, vel)
+            st = m.run(1000, 0.005, prec, energy_every=10)
+        e = np.array([s.ke + s.pe_shifted for s in st])
+        out[prec] = e
+    for prec, e in out.items():
+        assert np.abs(e - e[0]).max() < 5e-4 * abs(e[0]), prec
+
+
+def test_out_of_box_status(E):
+    print("[errors]", flush=True)
+    pos, box = _case(6, 0.05, 1)
+    pos = pos.copy()
+    pos[5] = [-3.0, -3.0, -3.0]
+    with E.MidpointLJ(len(pos), box, 2.5) as m:
+        m.set_state(pos)
+        with pytest.raises(E.ExamdError) as ei:
+            m.forces("fp64")
+        assert ei.value.status == E.ERR_OUT_OF_BOX
+
+
+def test_small_grid_rejected(E):
+    with pytest.raises(E.ExamdError) as ei:
+        E.MidpointLJ(10, [11.0, 11.0, 11.0], 2.5)     # 8 blocks: fine
+        E.MidpointLJ(10, [6.0, 6.0, 6.0], 2.5)        # 4 blocks: rejected
+    assert ei.value.status == E.ERR_GRID
