@@ -1,0 +1,196 @@
+"""Headline benchmark: atom-steps/s of the 2,048,000-atom LJ liquid on one GPU.
+
+Workload (one GPU's share at real shape): 80^3 FCC lattice, 2,048,000 atoms,
+density 0.8442, Maxwell-Boltzmann velocities at T* = 0.72 (seeded, zero net
+momentum), sigma = epsilon = mass = 1, cutoff 2.5 (unshifted), cubic periodic
+box of 107^3 rc/2 blocks split into 16^3 cell blocks, velocity Verlet
+dt = 0.005.  Each step rebins on device and evaluates the midpoint-method
+forces once.
+
+value = mixed-precision atom-steps/s (FP32 pair arithmetic, int64 fixed-point
+accumulation, FP64 integration); the FP64 path is timed on the same system
+for the mixed-vs-FP64 ratio.  Inputs are resident in HBM before the timed
+region; the run call returns after the stream is synchronised.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 vector (packed)
+FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector
+NC = 80                       # FCC cells per side -> 2,048,000 atoms
+RHO, TEMP, RC, DT = 0.8442, 0.72, 2.5, 0.005
+TILES = (16, 16, 16)
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles",
+                               "k_forces_mixed_2048000_traffic.txt")
+
+
+def algorithmic_flops(cand, cut, energy):
+    """W = 8 P_cand + 18 P_cut (+3 P_cut when energies are tallied)."""
+    return 8.0 * cand + 18.0 * cut + (3.0 * cut if energy else 0.0)
+
+
+def drift_per_1k(stats):
+    """Least-squares slope of the shifted total energy per 1,000 steps,
+    relative to |E|."""
+    if len(stats) < 3:
+        return None
+    t = np.array([s.step for s in stats], dtype=float)
+    e = np.array([s.ke + s.pe_shifted for s in stats])
+    slope = np.polyfit(t, e, 1)[0]
+    return float(slope * 1000.0 / abs(e.mean()))
+
+
+def read_traffic():
+    """HBM bytes per force launch from the committed rocprofv3 PMC summary
+    (line 'traffic_bytes_per_launch: N'), or None."""
+    try:
+        with open(PROFILE_TRAFFIC) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every):
+    with E.MidpointLJ(len(pos), box, RC, tiles=TILES) as m:
+        m.set_state(pos, vel)
+        if warmup:
+            m.run(warmup, DT, prec, energy_every=0)
+        m.timing(enable=True, reset=True)
+        t0 = time.perf_counter()
+        stats = m.run(steps, DT, prec, energy_every=energy_every)
+        t1 = time.perf_counter()
+        force_ms, launches = m.timing(enable=False, reset=True)
+    return t1 - t0, stats, force_ms / max(launches, 1), launches
+
+
+def cpu_baseline(pos, vel, box, nsteps):
+    """The oracle (a 1-core C restatement of the reference path, FP64) on a
+    bounded sample of the same workload: nsteps VV steps of the full system,
+    after one untimed force evaluation."""
+    from oracle import oracle as O
+    f0, _, _ = O.forces(pos, box, RC)
+    t0 = time.perf_counter()
+    O.run_vv(pos, vel, box, RC, DT, nsteps, forces0=f0)
+    dt = time.perf_counter() - t0
+    return len(pos) * nsteps / dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one GPU and the "
+              "path does not use a second (run with --gpus 1)",
+              file=sys.stderr)
+        return 2
+
+    import examd_amd as E
+    E.lib()
+    pos, vel, box = E.generate_fcc(NC, RHO, TEMP, seed=2048)
+    n = len(pos)
+    steps = args.steps
+    fsteps = args.fp64_steps if args.fp64_steps is not None else steps
+    energy_every = 10
+
+    wall_mx, st_mx, force_ms_mx, launches = timed_run(
+        E, pos, vel, box, "mixed", steps, args.warmup, energy_every)
+    wall_64, st_64, force_ms_64, _ = timed_run(
+        E, pos, vel, box, "fp64", fsteps, args.warmup, energy_every)
+
+    rate_mx = n * steps / wall_mx
+    rate_64 = n * fsteps / wall_64
+    # algorithmic work of one force launch: mean P_cand, P_cut over the
+    # energy steps tallied; about 1 in energy_every launches also tallies
+    # the energy (+3 P_cut)
+    cand = float(np.mean([s.pairs_cand for s in st_mx]))
+    cut = float(np.mean([s.pairs_cut for s in st_mx]))
+    w = algorithmic_flops(cand, cut, False) + 3.0 * cut / energy_every
+    achieved = w / (force_ms_mx * 1e-3) / 1e12
+    w64 = algorithmic_flops(cand, cut, False) + 3.0 * cut / energy_every
+    achieved64 = w64 / (force_ms_64 * 1e-3) / 1e12
+
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        v = cpu_baseline(pos, vel, box, args.cpu_steps)
+        cpu = {"value": v, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} velocity-Verlet steps (rebin + "
+                         f"force) of the same 2,048,000-atom system, FP64, "
+                         f"oracle C restatement of the reference path"}
+
+    traffic = read_traffic()
+    line = {
+        "metric": "atom-steps/s @2.048M-atom LJ: mixed vs FP64 GPU; "
+                  "% of FP32 vector peak",
+        "value": rate_mx,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_mx / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "LJ liquid 2,048,000 atoms (80^3 FCC, rho 0.8442, "
+                        "T* 0.72, rc 2.5, dt 0.005), 107^3 midpoint blocks "
+                        "in 16^3 cell blocks, rebin every step",
+            "natoms": n,
+            "precision": "mixed: fp32 pair arithmetic, int64 fixed-point "
+                         "(2^-32) force/energy accumulation, fp64 integration",
+            "fp64_atom_steps_per_s": rate_64,
+            "fp64_steps": fsteps,
+            "mixed_vs_fp64": rate_mx / rate_64,
+            "force_ms_mixed": force_ms_mx,
+            "force_ms_fp64": force_ms_64,
+            "force_launches_timed": launches,
+            "pairs_cand": cand,
+            "pairs_cut": cut,
+            "force_frac_of_step_mixed": force_ms_mx * steps / (1e3 * wall_mx),
+            "fp64_force_tflops": achieved64,
+            "fp64_force_frac_of_fp64_peak": achieved64 / FP64_PEAK_TFLOPS,
+            "energy_drift_per_1k_mixed": drift_per_1k(st_mx),
+            "energy_drift_per_1k_fp64": drift_per_1k(st_64),
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel": "k_forces (mixed)",
+            "flops_per_launch": w,
+            "launch_ms": force_ms_mx,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

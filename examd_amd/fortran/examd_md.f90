@@ -1,0 +1,226 @@
+!! examd_md.f90 -- Fortran driver: the reference driver's setup sequence and
+!! force check (exaMD.f90:46-366), then a velocity-Verlet MD loop on the GPU.
+!!
+!! Options:
+!!   -atoms N        FCC lattice of N = 4 nc^3 atoms (default 32000)
+!!   -readfile       read exaMD_particles.dat (zero initial velocities)
+!!   -writefile      write exaMD_particles.dat
+!!   -overlap        use exaMD_forces_midpoint_overlap
+!!   -cutoff rc      cutoff (default 2.5)
+!!   -rho r          FCC number density (default 0.8442)
+!!   -temp T         initial temperature (default 0.72)
+!!   -steps K        MD steps after the force check (default 0)
+!!   -dt h           time step (default 0.005)
+!!   -precision p    fp64 | mixed (default fp64)
+!!   -seed s         generator seed (default 1)
+
+Program examd_md
+
+  Use, Intrinsic :: iso_c_binding
+  Use exaMD_mod
+  Use exaMD_partitioning
+  Use exaMD_comms
+  Use exaMD_forces_mod
+  Use exaMD_input_formats
+  Use exaMD_utils_mod
+  Use user_func_mod
+  Use exaMD_hip_binding
+
+  Implicit None
+
+  Real(kind=PRC) :: cutoff, block_length, rho, temp, dt, ftot(3), ftot2(3)
+  Integer :: natms_tot, nblocks, nblocks_xyz(3), pid, noprocs, nc, i, j, k, b
+  Integer :: argc, ia, nnan
+  Integer(c_int64_t) :: nsteps, seed, nstats, s
+  Integer, Allocatable :: ntot(:), first_particle(:), next_particle(:)
+  Integer, Allocatable :: part(:,:,:)
+  Real(kind=PRC), Allocatable :: particle_positions(:,:), velocities(:,:)
+  Real(c_double), Allocatable :: vflat(:,:), pos_out(:,:), vel_out(:,:)
+  Type(examd_stats), Allocatable :: stats(:)
+  Character(len=32) :: arg
+  Logical :: inputcase, overlapcase, writeout
+  Integer(8) :: c0, c1, crate
+  Real(c_double) :: force_ms, e0, drift
+
+  Call exaMD_Init(pid, noprocs, 0)
+
+  natms_tot = 32000
+  cutoff = 2.5_PRC
+  rho = 0.8442_PRC
+  temp = 0.72_PRC
+  dt = 0.005_PRC
+  nsteps = 0
+  seed = 1
+  inputcase = .False.
+  overlapcase = .False.
+  writeout = .False.
+
+  argc = command_argument_count()
+  ia = 1
+  Do While (ia <= argc)
+     Call get_command_argument(ia, arg)
+     Select Case (Trim(arg))
+     Case ('-overlap')
+        overlapcase = .True.
+     Case ('-readfile')
+        inputcase = .True.
+     Case ('-writefile')
+        writeout = .True.
+     Case ('-atoms')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) natms_tot
+     Case ('-cutoff')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) cutoff
+     Case ('-rho')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) rho
+     Case ('-temp')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) temp
+     Case ('-steps')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) nsteps
+     Case ('-dt')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) dt
+     Case ('-seed')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) seed
+     Case ('-precision')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        If (Trim(arg) == 'mixed') exaMD_precision = EXAMD_PREC_MIXED
+     End Select
+     ia = ia + 1
+  End Do
+  block_length = half * cutoff
+
+  ! positions (and velocities) before the domain: the FCC box follows N, rho
+  If (inputcase) Then
+     Call exaMD_read_params(natms_tot)
+  Else
+     nc = Nint((Real(natms_tot, PRC) / 4.0_PRC)**(1.0_PRC / 3.0_PRC))
+     If (4 * nc**3 /= natms_tot) Then
+        Write (*, '(A,I0,A)') ' -atoms ', natms_tot, ' is not 4 nc^3'
+        Call exaMD_abort(103)
+     End If
+  End If
+  Allocate(particle_positions(3, natms_tot), velocities(3, natms_tot))
+  velocities = zero
+  If (.Not. inputcase) Call exaMD_generate_fcc(nc, rho, temp, seed, &
+       particle_positions, velocities)
+
+  Call exaMD_domain_create(block_length, nblocks, nblocks_xyz)
+
+  Write (*, *) '-----------------------------------------------------------------------'
+  Write (*, *) 'exaMD on MI355X (gfx950): midpoint-method Lennard-Jones'
+  Write (*, '(A,I0)') ' Number of atoms: ', natms_tot
+  Write (*, '(A,3F10.4)') ' Simulation box size:', globalbox
+  Write (*, '(A,F6.2)') ' Cut-off distance:', cutoff
+  Write (*, '(A,3(1X,I0))') ' Number of blocks in each dimension:', nblocks_xyz
+  If (exaMD_precision == EXAMD_PREC_MIXED) Then
+     Write (*, *) 'Precision: mixed (FP32 pairs, 64-bit fixed-point sums)'
+  Else
+     Write (*, *) 'Precision: FP64'
+  End If
+  Write (*, *) '-----------------------------------------------------------------------'
+
+  Allocate(ntot(nblocks), first_particle(nblocks), next_particle(natms_tot))
+  If (inputcase) Then
+     Call exaMD_read_particles(particle_positions, natms_tot, ntot, &
+          first_particle, next_particle, nblocks, nblocks_xyz, block_length)
+  Else
+     Call exaMD_chain_blocks(particle_positions, natms_tot, ntot, &
+          first_particle, next_particle, nblocks, nblocks_xyz, block_length)
+  End If
+  If (writeout) Call exaMD_write_particles(particle_positions, natms_tot)
+
+  Allocate(part(nblocks_xyz(1), nblocks_xyz(2), nblocks_xyz(3)))
+  Call exaMD_partition_domain(nblocks, nblocks_xyz, ntot, pid, noprocs, part)
+  Call exaMD_setup_partitioned_grid(part, nblocks, nblocks_xyz, pid)
+  Call exaMD_partition_work(particle_positions, ntot, first_particle, &
+       next_particle, nblocks, nblocks_xyz, pid, noprocs)
+
+  Allocate(kktype(1, 1), ltpvdw(1), prmvdw(1, 5))
+  kktype = 1
+  ltpvdw = 2
+  prmvdw(1, 1:5) = (/ one, one, zero, zero, cutoff /)
+  Call exaMD_setup_midpoint_interactions()
+
+  ! one force evaluation, timed like the reference (exaMD.f90:241-253)
+  Call system_clock(c0, crate)
+  Call exaMD_start_comms_posn()
+  If (overlapcase) Then
+     Call exaMD_forces_midpoint_overlap(lennardjones_12, cutoff, pid, noprocs)
+  Else
+     Call exaMD_forces_midpoint_all(lennardjones_12, cutoff, noprocs)
+  End If
+  Call exaMD_wait_comms_forces(noprocs)
+  Call system_clock(c1)
+
+  ! checks of exaMD.f90:269-317
+  nnan = 0
+  ftot = zero
+  ftot2 = zero
+  Do b = 1, nblocks_loc
+     i = locblocks(1, b)
+     j = locblocks(2, b)
+     k = locblocks(3, b)
+     If (grid(i, j, k)%natms == 0) Cycle
+     nnan = nnan + Count(grid(i, j, k)%forces /= grid(i, j, k)%forces)
+     ftot = ftot + Sum(grid(i, j, k)%forces, dim=2)
+     ftot2 = ftot2 + Sum(grid(i, j, k)%forces**2, dim=2)
+  End Do
+  Write (*, *) 'Checks on force calculations'
+  If (nnan > 0) Write (*, '(A,I0)') ' NaN force components: ', nnan
+  If (Abs(ftot(1)) > 1e-10_PRC) Write (*, *) 'WARNING! Suspected error'
+  Write (*, '(A,3E10.1)') '  SUM(fxx, fyy, fzz):', ftot
+  Write (*, '(A,3E10.1)') '  SUM(fxx^2, fyy^2, fzz^2):', ftot2
+  Write (*, '(A,I0,A,I0)') '  Pairs: candidate ', exaMD_last_stats%pairs_cand, &
+       ', in cutoff ', exaMD_last_stats%pairs_cut
+  Write (*, '(A,F16.6)') '  Potential energy:', exaMD_last_stats%pe
+  Write (*, '(A,F10.4)') '  Computation time (secs):', &
+       Real(c1 - c0, 8) / Real(crate, 8)
+
+  If (nsteps > 0) Then
+     ! velocities in the grid's atom order (blocks, then input order)
+     Allocate(vflat(3, natms_tot), pos_out(3, natms_tot), vel_out(3, natms_tot))
+     s = 0
+     Do b = 1, nblocks
+        j = first_particle(b)
+        Do i = 1, ntot(b)
+           s = s + 1
+           vflat(:, s) = velocities(:, j)
+           j = next_particle(j)
+        End Do
+     End Do
+     Allocate(stats(Max(1_c_int64_t, nsteps)))
+     Call system_clock(c0, crate)
+     Call exaMD_md_run(cutoff, vflat, nsteps, dt, Max(1_c_int64_t, nsteps / 10), &
+          stats, nstats, pos_out, vel_out, force_ms)
+     Call system_clock(c1)
+     Write (*, *) 'MD: step, KE, PE, E(shifted)'
+     Do s = 1, nstats
+        Write (*, '(I8,3F18.6)') stats(s)%step, stats(s)%ke, stats(s)%pe, &
+             stats(s)%ke + stats(s)%pe_shifted
+     End Do
+     e0 = stats(1)%ke + stats(1)%pe_shifted
+     drift = (stats(nstats)%ke + stats(nstats)%pe_shifted - e0) / Abs(e0)
+     Write (*, '(A,ES12.4)') ' Relative shifted-energy change:', drift
+     Write (*, '(A,ES12.4)') ' atom-steps/s:', Real(natms_tot, 8) * &
+          Real(nsteps, 8) / (Real(c1 - c0, 8) / Real(crate, 8))
+     Write (*, '(A,F10.4)') ' Force kernel ms/step:', force_ms / Real(nsteps, 8)
+  End If
+
+  Deallocate(kktype, ltpvdw, prmvdw)
+  Call exaMD_Finalize('Done')
+
+End Program examd_md
