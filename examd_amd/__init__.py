@@ -56,7 +56,8 @@ class Params(C.Structure):
                 ("cutoff", C.c_double), ("epsilon", C.c_double),
                 ("sigma", C.c_double), ("mass", C.c_double),
                 ("tiles", C.c_int32 * 3), ("device", C.c_int32),
-                ("block_threads", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("block_threads", C.c_int32), ("cap_slack_pct", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):
@@ -155,7 +156,8 @@ class MidpointLJ:
     """One LJ system on one GPU (a handle of the C ABI)."""
 
     def __init__(self, natoms, box, cutoff, epsilon=1.0, sigma=1.0, mass=1.0,
-                 tiles=(0, 0, 0), device=0, block_threads=0):
+                 tiles=(0, 0, 0), device=0, block_threads=0,
+                 cap_slack_pct=0):
         p = Params()
         p.natoms = int(natoms)
         p.box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
@@ -164,6 +166,7 @@ class MidpointLJ:
         p.tiles = (C.c_int32 * 3)(*[int(t) for t in tiles])
         p.device = int(device)
         p.block_threads = int(block_threads)
+        p.cap_slack_pct = int(cap_slack_pct)
         self.params = p
         self.n = int(natoms)
         self.box = np.array(np.reshape(box, 3), dtype=np.float64)

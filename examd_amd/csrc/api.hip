@@ -29,6 +29,7 @@ struct examd_handle {
   int maxrb = 0;
   int cap[2] = {0, 0};      // LDS capacity per precision
   size_t lds_limit = 0;
+  int64_t part_cap = 0;
   bool state_set = false;
   int forces_valid = -1;    // precision whose forces match the positions
   int last_prec = 0;
@@ -71,12 +72,25 @@ void free_state(examd_handle *h) {
   DevState &d = h->d;
   void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
                   d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.rank,
-                  d.count, d.start, d.sorted, d.chunk, d.home, h->log};
+                  d.count, d.start, d.sorted, d.chunk, d.home, d.part, h->log};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DevState{};
   h->log = nullptr;
   h->log_cap = 0;
+  h->part_cap = 0;
+}
+
+// Partial-tally slots for the current tile map.
+int ensure_partials(examd_handle *h) {
+  int64_t need = partial_slots(h->n, (int64_t)h->g.T[0] * h->g.T[1] * h->g.T[2]);
+  if (h->part_cap >= need) return EXAMD_OK;
+  if (h->d.part) (void)hipFree(h->d.part);
+  h->d.part = nullptr;
+  h->part_cap = 0;
+  if (dalloc(&h->d.part, need)) return EXAMD_ERR_NOMEM;
+  h->part_cap = need;
+  return EXAMD_OK;
 }
 
 int ensure_log(examd_handle *h, int64_t n) {
@@ -148,27 +162,36 @@ int64_t max_region_atoms(const Grid &g, const double *pos, int64_t n) {
 }
 
 // Pick the LDS capacity of a cell block for both precisions from the
-// largest region of the given positions (30% headroom for density
-// fluctuations during a run, at least 10%).  When the tiles were chosen
+// largest region of the given positions, with cap_slack_pct headroom for
+// density fluctuations during a run (at least 5%).  When the tiles were chosen
 // automatically and a region does not fit the device's LDS, refine the
 // cell-block map along its coarsest axis.
 int choose_caps(examd_handle *h, const double *pos) {
   for (int iter = 0; iter < 256; ++iter) {
     h->maxrb = max_region_blocks(h->g);
-    int64_t mra = max_region_atoms(h->g, pos, h->n);
-    int64_t want = ((int64_t)std::ceil((double)mra * 1.3) + 64 + 63) / 64 * 64;
-    int64_t need = (int64_t)std::ceil((double)mra * 1.1) + 32;
+    // region coordinates are packed 8 bits per axis on the device
+    bool dims_ok = true;
+    for (int a = 0; a < 3; ++a) {
+      int lo, hi;
+      tile_range(0, h->g.nb[a], h->g.T[a], lo, hi);
+      if ((h->g.nb[a] + h->g.T[a] - 1) / h->g.T[a] + 2 > 255) dims_ok = false;
+    }
+    int64_t mra = dims_ok ? max_region_atoms(h->g, pos, h->n) : 0;
+    const double slack = (h->p.cap_slack_pct > 0 ? h->p.cap_slack_pct : 15) / 100.0;
+    int64_t want = ((int64_t)std::ceil((double)mra * (1.0 + slack)) + 32 + 15) / 16 * 16;
+    int64_t need = (int64_t)std::ceil((double)mra * 1.05) + 16;
     bool ok = true;
     int caps[2];
     for (int prec = 0; prec < 2; ++prec) {
       bool fp64 = prec == EXAMD_PREC_FP64;
       int64_t c = want;
       while (c > need &&
-             force_lds_bytes(fp64, (int)c, h->maxrb) > h->lds_limit)
+             force_lds_bytes(fp64, (int)c, h->maxrb, h->tile_threads) > h->lds_limit)
         c -= 16;
-      if (force_lds_bytes(fp64, (int)c, h->maxrb) > h->lds_limit) ok = false;
+      if (force_lds_bytes(fp64, (int)c, h->maxrb, h->tile_threads) > h->lds_limit) ok = false;
       caps[prec] = (int)c;
     }
+    if (!dims_ok || want > 65535) ok = false;
     if (ok) {
       h->cap[0] = caps[0];
       h->cap[1] = caps[1];
@@ -345,7 +368,8 @@ int examd_hip_create(const examd_params *prm, examd_handle **out) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                             h->dev) != hipSuccess || lds <= 0)
     lds = 65536;
-  h->lds_limit = (size_t)lds;
+  // keep room for the kernels' static LDS (reduction scratch)
+  h->lds_limit = (size_t)lds > 4096 ? (size_t)lds - 4096 : (size_t)lds;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) !=
       hipSuccess) {
     delete h;
@@ -400,6 +424,7 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
     if (!std::isfinite(pos[i])) return EXAMD_ERR_NONFINITE;
   int st = choose_caps(h, pos);
   if (st) return st;
+  if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
   std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
   double *dst[3] = {h->d.x, h->d.y, h->d.z};
   double *vdst[3] = {h->d.vx, h->d.vy, h->d.vz};

@@ -97,6 +97,17 @@ struct StepLog {
   unsigned long long flags;  // Flag bits
 };
 
+// Partial tallies of one workgroup, written with plain stores and summed in
+// index order by a reduction kernel (no same-address atomics, and the sums
+// are deterministic).
+struct Partial {
+  double ke;
+  double pe;
+  unsigned long long pe_fix;
+  unsigned long long cand;
+  unsigned long long cut;
+};
+
 constexpr double kFixScale = 4294967296.0;          // 2^32
 constexpr double kFixInv = 2.3283064365386963e-10;   // 2^-32
 constexpr double kFixMagic = 6755399441055744.0;     // 1.5 * 2^52

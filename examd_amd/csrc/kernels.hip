@@ -44,6 +44,55 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
   return v;
 }
 
+__device__ __forceinline__ void add_partial(Partial &a, const Partial &b) {
+  a.ke += b.ke;
+  a.pe += b.pe;
+  a.pe_fix += b.pe_fix;
+  a.cand += b.cand;
+  a.cut += b.cut;
+}
+
+// Workgroup sum of one Partial per thread, stored at out[blockIdx.x] (all
+// threads must call it; the order of the sums is fixed).
+template <int NT>
+__device__ void write_partial(Partial v, Partial *out) {
+  __shared__ Partial s_p[NT / 64];
+  v.ke = wave_sum(v.ke);
+  v.pe = wave_sum(v.pe);
+  v.pe_fix = wave_sum(v.pe_fix);
+  v.cand = wave_sum(v.cand);
+  v.cut = wave_sum(v.cut);
+  if ((threadIdx.x & 63) == 0) s_p[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Partial t = s_p[0];
+    for (int w = 1; w < NT / 64; ++w) add_partial(t, s_p[w]);
+    out[blockIdx.x] = t;
+  }
+}
+
+enum : unsigned { kSumKE = 1u, kSumForce = 2u };
+
+// Sum n partials in index order into one step's log.
+constexpr int kReduceThreads = 256;
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_partials(
+    const Partial *p, int n, StepLog *log, unsigned what) {
+  Partial acc{0.0, 0.0, 0ull, 0ull, 0ull};
+  for (int i = threadIdx.x; i < n; i += kReduceThreads) add_partial(acc, p[i]);
+  __shared__ Partial s_tot;
+  write_partial<kReduceThreads>(acc, &s_tot);  // one workgroup: index 0
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (what & kSumKE) log->ke += s_tot.ke;
+    if (what & kSumForce) {
+      log->pe += s_tot.pe;
+      log->pe_fix += s_tot.pe_fix;
+      log->cand += s_tot.cand;
+      log->cut += s_tot.cut;
+    }
+  }
+}
+
 // ------------------------------------------------------------- fills ----
 __global__ void k_fill_i32(int *p, int v, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,8 +113,7 @@ __global__ void k_fill_log(StepLog *p, int n) {
 // zero the force accumulators and assign the block.
 template <bool MIXED, int MODE>
 __global__ __launch_bounds__(kBinThreads) void k_step_bin(
-    DevState d, Grid g, double h, double dt, StepLog *ke_log,
-    StepLog *flag_log) {
+    DevState d, Grid g, double h, double dt, int want_ke, StepLog *flag_log) {
   const int64_t i = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
   double ke = 0.0;
   unsigned flags = 0;
@@ -133,10 +181,8 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     d.rank[i] = atomicAdd(&d.count[cell], 1);
   }
   if constexpr (MODE == kFinishKickDriftBin) {
-    if (ke_log) {
-      ke = wave_sum(ke);
-      if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(&ke_log->ke, ke);
-    }
+    if (want_ke) write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull},
+                                            d.part);
   }
   flags = wave_or(flags);
   if ((threadIdx.x & 63) == 0 && flags)
@@ -145,7 +191,7 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
 
 template <bool MIXED>
 __global__ __launch_bounds__(kBinThreads) void k_kick(DevState d, double h,
-                                                      StepLog *ke_log) {
+                                                      int want_ke) {
   const int64_t i = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
   double ke = 0.0;
   if (i < d.n) {
@@ -167,10 +213,8 @@ __global__ __launch_bounds__(kBinThreads) void k_kick(DevState d, double h,
     d.vz[i] = Vz;
     ke = Vx * Vx + Vy * Vy + Vz * Vz;
   }
-  if (ke_log) {
-    ke = wave_sum(ke);
-    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(&ke_log->ke, ke);
-  }
+  if (want_ke)
+    write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull}, d.part);
 }
 
 // ------------------------------------------------------------- scan ----
@@ -298,12 +342,17 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
   return (v + a - 1) / a * a;
 }
 
-// LDS image of one cell block: region-block offsets, then atoms.
+// LDS image of one cell block: region-block offsets, then atoms, then the
+// per-wave pair-walk scratch (candidate ranges and the hit queue).
+constexpr int kUnitsPerAtom = 16;   // >= the 13 ranges an atom can own
+constexpr int kQueue = 128;         // hit ring per wave (two drains deep)
 struct LdsLayout {
-  size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, total;
+  size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_walk, total;
 };
-__host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap,
-                                                int maxrb) {
+constexpr size_t kWalkBytesPerWave =
+    64 * kUnitsPerAtom * 4 + 2 * 65 * 4 + kQueue * 4;
+__host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
+                                                int nwaves) {
   LdsLayout L;
   size_t o = (size_t)(maxrb + 1) * 4;  // s_off
   L.off_gst = o;
@@ -320,6 +369,9 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap,
   o += (size_t)cap * 4;
   L.off_bc = o;
   if (fp64) o += (size_t)cap * 4;
+  o = align_up(o, 16);
+  L.off_walk = o;
+  o += (size_t)nwaves * kWalkBytesPerWave;
   L.total = align_up(o, 16);
   return L;
 }
@@ -352,12 +404,21 @@ __device__ __forceinline__ unsigned run_mask(unsigned ox, unsigned oy,
   return mx & my & mz;
 }
 
-template <int R>
-__device__ __forceinline__ Off3 run_o1_rt(int k) {
-  constexpr Off3 a = run_o1(R, 0), b = run_o1(R, 1), c = run_o1(R, 2),
-                 e = run_o1(R, 3), f = run_o1(R, 4);
-  return k == 0 ? a : k == 1 ? b : k == 2 ? c : k == 3 ? e : f;
-}
+// o1 of the table row with offset d = o2 - o1 (d in [-2, 2]^3, table
+// orientation), packed (o1x+1) | (o1y+1) << 2 | (o1z+1) << 4; for the
+// home-block diagnostic.
+struct O1Table {
+  unsigned char v[125];
+  constexpr O1Table() : v() {
+    for (int i = 0; i < 125; ++i) {
+      int dx = i % 5 - 2, dy = (i / 5) % 5 - 2, dz = i / 25 - 2;
+      int r = row_of(dx, dy, dz);
+      Off3 o = r < 0 ? Off3{0, 0, 0} : stencil_off(kTable[r][0]);
+      v[i] = (unsigned char)((o.x + 1) | ((o.y + 1) << 2) | ((o.z + 1) << 4));
+    }
+  }
+};
+__constant__ O1Table kO1;
 
 __device__ __forceinline__ void lds_add(double *p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -369,9 +430,25 @@ __device__ __forceinline__ void lds_sub(long long *p, long long v) {
   __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Order LDS traffic between the lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Packed region coordinates of an LDS atom: 8 bits per axis, plus the
+// periodic image code (s + 1, s in {-1, 0, 1}) per axis in bits 24..29.
+__device__ __forceinline__ int bc_x(int bc) { return bc & 255; }
+__device__ __forceinline__ int bc_y(int bc) { return (bc >> 8) & 255; }
+__device__ __forceinline__ int bc_z(int bc) { return (bc >> 16) & 255; }
+__device__ __forceinline__ int bc_s(int bc, int axis) {
+  return ((bc >> (24 + 2 * axis)) & 3) - 1;
+}
+
 template <bool FP64, bool ENERGY, bool HOMECNT, int NT>
 __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NW = NT / 64;
   const int tid = threadIdx.x;
   const Grid &g = A.g;
   const int G = g.T[0] * g.T[1] * g.T[2];
@@ -388,13 +465,14 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   const int RXY = RX * RY, NRB = RXY * RZ;
   const int nbx = g.nb[0], nby = g.nb[1], nbz = g.nb[2];
 
-  const LdsLayout LL = lds_layout(FP64, A.cap, A.maxrb);
+  const LdsLayout LL = lds_layout(FP64, A.cap, A.maxrb, NW);
   int *s_off = reinterpret_cast<int *>(smem);
   int *s_gst = reinterpret_cast<int *>(smem + LL.off_gst);
   int *s_wave = reinterpret_cast<int *>(smem + LL.off_wave);
   int *s_id = reinterpret_cast<int *>(smem + LL.off_id);
   int *s_bc = reinterpret_cast<int *>(smem + LL.off_bc);
-  // FP64 image: SoA doubles; mixed image: float4 (x, y, z, packed coords)
+  // FP64 image: SoA doubles + packed coords; mixed image: float4 with the
+  // packed coords in w
   double *s_x = reinterpret_cast<double *>(smem + LL.off_pos);
   double *s_y = s_x + A.cap;
   double *s_z = s_y + A.cap;
@@ -442,206 +520,302 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   }
   const int R = s_off[NRB];
   if (R > A.cap) {
-    if (tid == 0) atomicOr(&A.log->flags, (unsigned long long)kFlagOverflow);
+    if (tid == 0) {
+      atomicOr(&A.log->flags, (unsigned long long)kFlagOverflow);
+      A.d.part[blockIdx.x] = Partial{0.0, 0.0, 0ull, 0ull, 0ull};
+    }
     return;
   }
 
   // 2. ghost gather: atoms of the region into LDS.  The mixed image is
   //    relative to the tile centre and pre-shifted by the periodic image;
-  //    the FP64 image keeps raw coordinates (minimum image is applied per
-  //    pair, as the reference does).
+  //    the FP64 image keeps raw coordinates (the minimum image is applied
+  //    per pair, as the reference does).  First every slot learns its
+  //    region block, then one thread per slot gathers.
   const bool wlx = h0x == 0, whx = h1x == nbx;
   const bool wly = h0y == 0, why = h1y == nby;
   const bool wlz = h0z == 0, whz = h1z == nbz;
+  const bool wraps = wlx || whx || wly || why || wlz || whz;
   const double ox = 0.5 * (double)(h0x + h1x) * g.bl;
   const double oy = 0.5 * (double)(h0y + h1y) * g.bl;
   const double oz = 0.5 * (double)(h0z + h1z) * g.bl;
-  for (int rb = tid; rb < NRB; rb += NT) {
-    int b0 = s_off[rb], b1 = s_off[rb + 1];
-    if (b0 == b1) continue;
-    int cx = rb % RX, cy = (rb / RX) % RY, cz = rb / RXY;
-    int gs = s_gst[rb];
-    int bc = cx | (cy << 10) | (cz << 20);
-    double shx = (wlx && cx == 0) ? -g.L[0] : ((whx && cx == RX - 1) ? g.L[0] : 0.0);
-    double shy = (wly && cy == 0) ? -g.L[1] : ((why && cy == RY - 1) ? g.L[1] : 0.0);
-    double shz = (wlz && cz == 0) ? -g.L[2] : ((whz && cz == RZ - 1) ? g.L[2] : 0.0);
-    for (int k = 0; k < b1 - b0; ++k) {
-      int id = A.d.sorted[gs + k];
-      int s = b0 + k;
-      double X = A.d.x[id], Y = A.d.y[id], Z = A.d.z[id];
-      s_id[s] = id;
-      if constexpr (FP64) {
-        s_x[s] = X;
-        s_y[s] = Y;
-        s_z[s] = Z;
-        s_bc[s] = bc;
-        s_fx[s] = 0.0;
-        s_fy[s] = 0.0;
-        s_fz[s] = 0.0;
-      } else {
-        s_p4[s] = make_float4((float)((X + shx) - ox), (float)((Y + shy) - oy),
-                              (float)((Z + shz) - oz), __int_as_float(bc));
-        s_qx[s] = 0;
-        s_qy[s] = 0;
-        s_qz[s] = 0;
-      }
+  for (int rb = tid; rb < NRB; rb += NT)
+    for (int s = s_off[rb]; s < s_off[rb + 1]; ++s) s_id[s] = rb;
+  __syncthreads();
+  for (int s = tid; s < R; s += NT) {
+    const int rb = s_id[s];
+    const int cx = rb % RX, cy = (rb / RX) % RY, cz = rb / RXY;
+    const int id = A.d.sorted[s_gst[rb] + (s - s_off[rb])];
+    const int sx = (wlx && cx == 0) ? -1 : ((whx && cx == RX - 1) ? 1 : 0);
+    const int sy = (wly && cy == 0) ? -1 : ((why && cy == RY - 1) ? 1 : 0);
+    const int sz = (wlz && cz == 0) ? -1 : ((whz && cz == RZ - 1) ? 1 : 0);
+    const int bc = cx | (cy << 8) | (cz << 16) | ((sx + 1) << 24) |
+                   ((sy + 1) << 26) | ((sz + 1) << 28);
+    const double X = A.d.x[id], Y = A.d.y[id], Z = A.d.z[id];
+    s_id[s] = id;
+    if constexpr (FP64) {
+      s_x[s] = X;
+      s_y[s] = Y;
+      s_z[s] = Z;
+      s_bc[s] = bc;
+      s_fx[s] = 0.0;
+      s_fy[s] = 0.0;
+      s_fz[s] = 0.0;
+    } else {
+      s_p4[s] = make_float4((float)((X + sx * g.L[0]) - ox),
+                            (float)((Y + sy * g.L[1]) - oy),
+                            (float)((Z + sz * g.L[2]) - oz), __int_as_float(bc));
+      s_qx[s] = 0;
+      s_qy[s] = 0;
+      s_qz[s] = 0;
     }
   }
   __syncthreads();
 
-  // 3. pairs.  One lane per i-atom a in block I of the region; for each of
-  //    the 13 runs the owned J blocks (home I - o1 inside the tile) form at
-  //    most a few column segments, each one contiguous range of LDS atoms.
+  // 3. pairs.  Each wave takes batches of 64 region atoms.  Lane l lists the
+  //    candidate ranges atom (batch + l) owns: for each of the 13 runs, the
+  //    J blocks whose home block I - o1 lies in the tile, as contiguous LDS
+  //    ranges.  The wave then splits the batch's candidates evenly over its
+  //    lanes; in-cutoff pairs go to a per-wave ring and are evaluated 64 at
+  //    a time, so every lane does the same work in both phases.
+  const int wave = tid >> 6, lane = tid & 63;
+  unsigned *w_units = reinterpret_cast<unsigned *>(
+      smem + LL.off_walk + (size_t)wave * kWalkBytesPerWave);
+  int *w_nun = reinterpret_cast<int *>(w_units + 64 * kUnitsPerAtom);
+  int *w_cst = w_nun + 65;
+  unsigned *w_q = reinterpret_cast<unsigned *>(w_cst + 65);
+
   unsigned long long cand = 0, cut = 0;
   unsigned flags = 0;
   double pe64 = 0.0;
   long long pefix = 0;
 
-  for (int a = tid; a < R; a += NT) {
-    int bc;
-    double xa, ya, za;
-    float xf, yf, zf;
+  // One in-cutoff pair (a, j) in table orientation: force on a is +f, on j
+  // is -f (SRC/exaMD_forces.f90:538-542).
+  auto eval_pair = [&](int a, int j) {
     if constexpr (FP64) {
-      xa = s_x[a];
-      ya = s_y[a];
-      za = s_z[a];
-      bc = s_bc[a];
+      double btx = 0.0, bty = 0.0, btz = 0.0;
+      if (wraps) {
+        const int ba = s_bc[a], bj = s_bc[j];
+        btx = (double)(bc_s(ba, 0) - bc_s(bj, 0)) * g.L[0];
+        bty = (double)(bc_s(ba, 1) - bc_s(bj, 1)) * g.L[1];
+        btz = (double)(bc_s(ba, 2) - bc_s(bj, 2)) * g.L[2];
+      }
+      // SRC/exaMD_forces.f90:509-523 with Nint(dx*globalboxi) = bt
+      const double dx = (s_x[j] - s_x[a]) - btx;
+      const double dy = (s_y[j] - s_y[a]) - bty;
+      const double dz = (s_z[j] - s_z[a]) - btz;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      // user_func.f90:58-69
+      const double d2i = 1.0 / d2;
+      const double sg = A.sig2 * d2i;
+      const double d6 = sg * sg * sg;
+      const double d12 = d6 * d6;
+      const double dg = -(A.eps24 * (2.0 * d12 - d6) * d2i);
+      const double f1 = dg * dx, f2 = dg * dy, f3 = dg * dz;
+      lds_add(&s_fx[a], f1);
+      lds_add(&s_fy[a], f2);
+      lds_add(&s_fz[a], f3);
+      lds_add(&s_fx[j], -f1);
+      lds_add(&s_fy[j], -f2);
+      lds_add(&s_fz[j], -f3);
+      if constexpr (ENERGY) pe64 += A.eps4 * (d12 - d6);
+      if (d2 < A.d2zero) flags |= kFlagZero;
     } else {
-      float4 p = s_p4[a];
-      xf = p.x;
-      yf = p.y;
-      zf = p.z;
-      bc = __float_as_int(p.w);
+      const float4 pa = s_p4[a], pj = s_p4[j];
+      const float dx = pj.x - pa.x, dy = pj.y - pa.y, dz = pj.z - pa.z;
+      const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+      const float r2i = __builtin_amdgcn_rcpf(d2);
+      const float s2 = A.sig2f * r2i;
+      const float s6 = s2 * s2 * s2;
+      const float fs = A.eps24f * s6 * __builtin_fmaf(2.0f, s6, -1.0f) * r2i;
+      const long long q1 = fix32(-fs * dx);
+      const long long q2 = fix32(-fs * dy);
+      const long long q3 = fix32(-fs * dz);
+      lds_add(&s_qx[a], q1);
+      lds_add(&s_qy[a], q2);
+      lds_add(&s_qz[a], q3);
+      lds_sub(&s_qx[j], q1);
+      lds_sub(&s_qy[j], q2);
+      lds_sub(&s_qz[j], q3);
+      if constexpr (ENERGY) pefix += fix32(A.eps4f * s6 * (s6 - 1.0f));
+      if (d2 < A.d2zerof) flags |= kFlagZero;
+      if (d2 < A.d2fixf) flags |= kFlagFixedRange;
     }
-    const int ix = bc & 1023, iy = (bc >> 10) & 1023, iz = bc >> 20;
-    const unsigned okx = own_bits(ix, tx), oky = own_bits(iy, ty),
-                   okz = own_bits(iz, tz);
-    const int sIx = (wlx && ix == 0) ? -1 : ((whx && ix == RX - 1) ? 1 : 0);
-    const int sIy = (wly && iy == 0) ? -1 : ((why && iy == RY - 1) ? 1 : 0);
-    const int sIz = (wlz && iz == 0) ? -1 : ((whz && iz == RZ - 1) ? 1 : 0);
-    double fax = 0.0, fay = 0.0, faz = 0.0;
-    long long qax = 0, qay = 0, qaz = 0;
+    if constexpr (HOMECNT) {
+      const int ba = FP64 ? s_bc[a] : __float_as_int(s_p4[a].w);
+      const int bj = FP64 ? s_bc[j] : __float_as_int(s_p4[j].w);
+      const int di = (bc_x(bj) - bc_x(ba) + 2) + 5 * (bc_y(bj) - bc_y(ba) + 2) +
+                     25 * (bc_z(bj) - bc_z(ba) + 2);
+      const unsigned o = kO1.v[di];
+      const int hx = h0x - 1 + bc_x(ba) - ((int)(o & 3) - 1);
+      const int hy = h0y - 1 + bc_y(ba) - ((int)((o >> 2) & 3) - 1);
+      const int hz = h0z - 1 + bc_z(ba) - ((int)((o >> 4) & 3) - 1);
+      atomicAdd(&A.d.home[hx + nbx * (hy + nby * hz)], 1ull);
+    }
+    ++cut;
+  };
 
-    // Pairs (a, j) for LDS atoms j in [j0, j1), all in the same periodic
-    // image relative to a (btx/bty/btz = (sI - sJ) L; FP64 only).
-    auto pair_range = [&](int j0, int j1, double btx, double bty, double btz,
-                          int homeb) {
-      cand += (unsigned long long)(j1 > j0 ? j1 - j0 : 0);
-      for (int j = j0; j < j1; ++j) {
-        if constexpr (FP64) {
-          // SRC/exaMD_forces.f90:509-523 with Nint(dx*globalboxi) = bt
-          double dx = (s_x[j] - xa) - btx;
-          double dy = (s_y[j] - ya) - bty;
-          double dz = (s_z[j] - za) - btz;
-          double d2 = dx * dx + dy * dy + dz * dz;
-          if (d2 <= A.rc2) {  // user_func.f90:52
-            double d2i = 1.0 / d2;
-            double s = A.sig2 * d2i;
-            double d6 = s * s * s;
-            double d12 = d6 * d6;
-            double dg = -(A.eps24 * (2.0 * d12 - d6) * d2i);
-            double f1 = dg * dx, f2 = dg * dy, f3 = dg * dz;
-            fax += f1;
-            fay += f2;
-            faz += f3;
-            lds_add(&s_fx[j], -f1);
-            lds_add(&s_fy[j], -f2);
-            lds_add(&s_fz[j], -f3);
-            ++cut;
-            if constexpr (ENERGY) pe64 += A.eps4 * (d12 - d6);
-            if (d2 < A.d2zero) flags |= kFlagZero;
-            if constexpr (HOMECNT) atomicAdd(&A.d.home[homeb], 1ull);
-          }
-        } else {
-          float4 p = s_p4[j];
-          float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
-          float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-          if (d2 <= A.rc2f) {
-            float r2i = __builtin_amdgcn_rcpf(d2);
-            float s2 = A.sig2f * r2i;
-            float s6 = s2 * s2 * s2;
-            float fs = A.eps24f * s6 * __builtin_fmaf(2.0f, s6, -1.0f) * r2i;
-            long long q1 = fix32(-fs * dx);
-            long long q2 = fix32(-fs * dy);
-            long long q3 = fix32(-fs * dz);
-            qax += q1;
-            qay += q2;
-            qaz += q3;
-            lds_sub(&s_qx[j], q1);
-            lds_sub(&s_qy[j], q2);
-            lds_sub(&s_qz[j], q3);
-            ++cut;
-            if constexpr (ENERGY)
-              pefix += fix32(A.eps4f * s6 * (s6 - 1.0f));
-            if (d2 < A.d2zerof) flags |= kFlagZero;
-            if (d2 < A.d2fixf) flags |= kFlagFixedRange;
-            if constexpr (HOMECNT) atomicAdd(&A.d.home[homeb], 1ull);
+  for (int base = wave * 64; base < R; base += NW * 64) {
+    // 3a. candidate ranges of atom a = base + lane
+    const int a = base + lane;
+    int nun = 0, ncand = 0;
+    unsigned *my = w_units + lane * kUnitsPerAtom;
+    if (a < R) {
+      const int bc = FP64 ? s_bc[a] : __float_as_int(s_p4[a].w);
+      const int ix = bc_x(bc), iy = bc_y(bc), iz = bc_z(bc);
+      const unsigned okx = own_bits(ix, tx), oky = own_bits(iy, ty),
+                     okz = own_bits(iz, tz);
+      static_for<0, kNumRuns>([&](auto RUN) {
+        constexpr int r = decltype(RUN)::value;
+        unsigned mask = run_mask<r>(okx, oky, okz);
+        const int row0 = RX * ((iy + kRunDY[r]) + RY * (iz + kRunDZ[r])) + ix - 2;
+        while (mask) {
+          const int lo = __builtin_ctz(mask);
+          const int hi = lo + __builtin_ctz(~(mask >> lo)) - 1;
+          mask &= ~((2u << hi) - 1u);
+          // self run, dx = 0: the block's own triangle, j > a
+          const int j0 = (r == 0 && lo == 2) ? a + 1 : s_off[row0 + lo];
+          const int j1 = s_off[row0 + hi + 1];
+          if (j1 > j0) {
+            my[nun++] = (unsigned)j0 | ((unsigned)j1 << 16);
+            ncand += j1 - j0;
           }
         }
+      });
+    }
+    w_nun[lane] = nun;
+    // wave prefix of candidate counts
+    int incl = ncand;
+    for (int o = 1; o < 64; o <<= 1) {
+      int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    w_cst[lane] = incl - ncand;
+    if (lane == 63) w_cst[64] = total;
+    wave_sync();
+    cand += (unsigned long long)ncand;
+
+    // 3b. balanced walk: lane takes candidates [f0, f1) of the batch
+    const int per = (total + 63) >> 6;
+    const int f0 = min(total, lane * per), f1 = min(total, f0 + per);
+    int l = 0, k = 0, j = 0, jend = 0;
+    if (f0 < f1) {
+      // owner lane of f0: last lane with cst <= f0 and units
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (w_cst[mid] <= f0) lo = mid;
+        else hi = mid - 1;
+      }
+      l = lo;
+      while (w_nun[l] == 0 || w_cst[l + 1] <= f0) ++l;
+      int off = f0 - w_cst[l];
+      for (;;) {
+        const unsigned u = w_units[l * kUnitsPerAtom + k];
+        const int len = (int)(u >> 16) - (int)(u & 0xffff);
+        if (off < len) {
+          j = (int)(u & 0xffff) + off;
+          jend = (int)(u >> 16);
+          break;
+        }
+        off -= len;
+        ++k;
+      }
+    }
+    int ia = base + l;
+    double xa = 0.0, ya = 0.0, za = 0.0;
+    float xf = 0.f, yf = 0.f, zf = 0.f;
+    int sax = 0, say = 0, saz = 0;
+    auto load_a = [&]() {
+      if constexpr (FP64) {
+        xa = s_x[ia];
+        ya = s_y[ia];
+        za = s_z[ia];
+        if (wraps) {
+          const int ba = s_bc[ia];
+          sax = bc_s(ba, 0);
+          say = bc_s(ba, 1);
+          saz = bc_s(ba, 2);
+        }
+      } else {
+        const float4 p = s_p4[ia];
+        xf = p.x;
+        yf = p.y;
+        zf = p.z;
       }
     };
+    if (f0 < f1) load_a();
 
-    static_for<0, kNumRuns>([&](auto RUN) {
-      constexpr int r = decltype(RUN)::value;
-      unsigned mask = run_mask<r>(okx, oky, okz);
-      if (!mask) return;
-      const int jy = iy + kRunDY[r], jz = iz + kRunDZ[r];
-      const int row0 = RX * (jy + RY * jz);
-      double bty = 0.0, btz = 0.0;
-      if constexpr (FP64) {
-        int sJy = (wly && jy == 0) ? -1 : ((why && jy == RY - 1) ? 1 : 0);
-        int sJz = (wlz && jz == 0) ? -1 : ((whz && jz == RZ - 1) ? 1 : 0);
-        bty = (double)(sIy - sJy) * g.L[1];
-        btz = (double)(sIz - sJz) * g.L[2];
-      }
-      // columns [c0, c1] of J's row, one periodic image
-      auto cols = [&](int c0, int c1, int sJx) {
-        double btx = 0.0;
-        if constexpr (FP64) btx = (double)(sIx - sJx) * g.L[0];
-        if constexpr (HOMECNT) {
-          for (int c = c0; c <= c1; ++c) {
-            Off3 o = run_o1_rt<r>(c - ix + 2);
-            int hx = h0x - 1 + ix - o.x, hy = h0y - 1 + iy - o.y,
-                hz = h0z - 1 + iz - o.z;
-            int homeb = hx + nbx * (hy + nby * hz);
-            int j0 = (r == 0 && c == ix) ? a + 1 : s_off[row0 + c];
-            pair_range(j0, s_off[row0 + c + 1], btx, bty, btz, homeb);
-          }
-        } else {
-          int j0 = (r == 0 && c0 == ix) ? a + 1 : s_off[row0 + c0];
-          pair_range(j0, s_off[row0 + c1 + 1], btx, bty, btz, 0);
-        }
-      };
-      while (mask) {
-        int lo = __builtin_ctz(mask);
-        int hi = lo + __builtin_ctz(~(mask >> lo)) - 1;
-        mask &= ~((2u << hi) - 1u);
-        int c0 = ix + lo - 2, c1 = ix + hi - 2;
+    int qhead = 0, qtail = 0;   // wave-uniform ring indices
+    for (int it = 0; it < per; ++it) {
+      const bool act = f0 + it < f1;
+      bool hit = false;
+      if (act) {
         if constexpr (FP64) {
-          // split at wrapped shell columns so each piece has one image
-          if (wlx && c0 == 0) {
-            cols(0, 0, -1);
-            c0 = 1;
+          double btx = 0.0, bty = 0.0, btz = 0.0;
+          if (wraps) {
+            const int bj = s_bc[j];
+            btx = (double)(sax - bc_s(bj, 0)) * g.L[0];
+            bty = (double)(say - bc_s(bj, 1)) * g.L[1];
+            btz = (double)(saz - bc_s(bj, 2)) * g.L[2];
           }
-          bool tail = whx && c1 == RX - 1;
-          if (tail) c1 = RX - 2;
-          if (c0 <= c1) cols(c0, c1, 0);
-          if (tail) cols(RX - 1, RX - 1, 1);
+          const double dx = (s_x[j] - xa) - btx;
+          const double dy = (s_y[j] - ya) - bty;
+          const double dz = (s_z[j] - za) - btz;
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          hit = d2 <= A.rc2;   // user_func.f90:52
         } else {
-          cols(c0, c1, 0);
+          const float4 p = s_p4[j];
+          const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
+          const float d2 =
+              __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+          hit = d2 <= A.rc2f;
         }
       }
-    });
-
-    if constexpr (FP64) {
-      lds_add(&s_fx[a], fax);
-      lds_add(&s_fy[a], fay);
-      lds_add(&s_fz[a], faz);
-    } else {
-      lds_add(&s_qx[a], qax);
-      lds_add(&s_qy[a], qay);
-      lds_add(&s_qz[a], qaz);
+      // append hits to the ring
+      const unsigned long long m = __ballot(hit);
+      if (hit) {
+        const int pos = qtail + (int)__builtin_amdgcn_mbcnt_hi(
+                                    (unsigned)(m >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        w_q[pos & (kQueue - 1)] = (unsigned)ia | ((unsigned)j << 16);
+      }
+      qtail += __popcll(m);
+      if (qtail - qhead >= 64) {
+        wave_sync();
+        const unsigned e = w_q[(qhead + lane) & (kQueue - 1)];
+        eval_pair((int)(e & 0xffff), (int)(e >> 16));
+        qhead += 64;
+      }
+      // advance to the next candidate
+      if (act && f0 + it + 1 < f1) {
+        if (++j == jend) {
+          const int lprev = l;
+          if (++k == w_nun[l]) {
+            k = 0;
+            do {
+              ++l;
+            } while (w_nun[l] == 0);
+          }
+          const unsigned u = w_units[l * kUnitsPerAtom + k];
+          j = (int)(u & 0xffff);
+          jend = (int)(u >> 16);
+          if (l != lprev) {
+            ia = base + l;
+            load_a();
+          }
+        }
+      }
     }
+    // drain the rest of the ring
+    wave_sync();
+    if (lane < qtail - qhead) {
+      const unsigned e = w_q[(qhead + lane) & (kQueue - 1)];
+      eval_pair((int)(e & 0xffff), (int)(e >> 16));
+    }
+    wave_sync();
   }
   __syncthreads();
 
@@ -662,22 +836,15 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
     }
   }
 
-  cand = wave_sum(cand);
-  cut = wave_sum(cut);
   flags = wave_or(flags);
+  if ((tid & 63) == 0 && flags)
+    atomicOr(&A.log->flags, (unsigned long long)flags);
+  Partial mine{0.0, 0.0, 0ull, cand, cut};
   if constexpr (ENERGY) {
-    if constexpr (FP64) pe64 = wave_sum(pe64);
-    else pefix = (long long)wave_sum((unsigned long long)pefix);
+    if constexpr (FP64) mine.pe = pe64;
+    else mine.pe_fix = (unsigned long long)pefix;
   }
-  if ((tid & 63) == 0) {
-    atomicAdd(&A.log->cand, cand);
-    atomicAdd(&A.log->cut, cut);
-    if (flags) atomicOr(&A.log->flags, (unsigned long long)flags);
-    if constexpr (ENERGY) {
-      if constexpr (FP64) unsafeAtomicAdd(&A.log->pe, pe64);
-      else atomicAdd(&A.log->pe_fix, (unsigned long long)pefix);
-    }
-  }
+  write_partial<NT>(mine, A.d.part);
 }
 
 __global__ void k_forces_to_fp64(DevState d, double *out, bool mixed) {
@@ -714,8 +881,8 @@ void launch_forces_t(const ForceArgs &A, int threads, size_t lds, int G,
 
 }  // namespace
 
-size_t force_lds_bytes(bool fp64, int cap, int maxrb) {
-  return lds_layout(fp64, cap, maxrb).total;
+size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads) {
+  return lds_layout(fp64, cap, maxrb, threads / 64).total;
 }
 
 void launch_fill_i32(int *p, int v, int64_t n, hipStream_t s) {
@@ -728,13 +895,19 @@ void launch_fill_log(StepLog *p, int n, hipStream_t s) {
   if (n > 0) k_fill_log<<<nblk(n, 64), 64, 0, s>>>(p, n);
 }
 
+int64_t partial_slots(int64_t n, int64_t tiles) {
+  int64_t b = (n + kBinThreads - 1) / kBinThreads;
+  return b > tiles ? b : tiles;
+}
+
 void launch_step_bin(const DevState &d, const Grid &g, bool mixed, int mode,
                      double h, double dt, StepLog *ke_log, StepLog *flag_log,
                      hipStream_t s) {
   unsigned nb = nblk(d.n, kBinThreads);
   if (nb == 0) return;
+  const int want = (mode == kFinishKickDriftBin && ke_log) ? 1 : 0;
 #define EXAMD_BIN(M, MD)                                               \
-  k_step_bin<M, MD><<<nb, kBinThreads, 0, s>>>(d, g, h, dt, ke_log, flag_log)
+  k_step_bin<M, MD><<<nb, kBinThreads, 0, s>>>(d, g, h, dt, want, flag_log)
   if (mixed) {
     if (mode == kBinOnly) EXAMD_BIN(true, kBinOnly);
     else if (mode == kKickDriftBin) EXAMD_BIN(true, kKickDriftBin);
@@ -745,14 +918,21 @@ void launch_step_bin(const DevState &d, const Grid &g, bool mixed, int mode,
     else EXAMD_BIN(false, kFinishKickDriftBin);
   }
 #undef EXAMD_BIN
+  if (want)
+    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, (int)nb, ke_log,
+                                                    kSumKE);
 }
 
 void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
                  hipStream_t s) {
   unsigned nb = nblk(d.n, kBinThreads);
   if (nb == 0) return;
-  if (mixed) k_kick<true><<<nb, kBinThreads, 0, s>>>(d, h, ke_log);
-  else k_kick<false><<<nb, kBinThreads, 0, s>>>(d, h, ke_log);
+  const int want = ke_log ? 1 : 0;
+  if (mixed) k_kick<true><<<nb, kBinThreads, 0, s>>>(d, h, want);
+  else k_kick<false><<<nb, kBinThreads, 0, s>>>(d, h, want);
+  if (want)
+    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, (int)nb, ke_log,
+                                                    kSumKE);
 }
 
 void launch_sort(const DevState &d, int64_t nblocks, hipStream_t s) {
@@ -792,7 +972,7 @@ void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
   A.d2zerof = (float)A.d2zero;
   A.d2fixf = (float)A.d2fix;
   const bool fp64 = !mixed;
-  const size_t lds = force_lds_bytes(fp64, c.cap, c.maxrb);
+  const size_t lds = force_lds_bytes(fp64, c.cap, c.maxrb, c.threads);
   const int G = g.T[0] * g.T[1] * g.T[2];
 #define EXAMD_F(F64, E, H) launch_forces_t<F64, E, H>(A, c.threads, lds, G, s)
   if (homecount) {
@@ -806,6 +986,7 @@ void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
     else EXAMD_F(false, false, false);
   }
 #undef EXAMD_F
+  k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, G, log, kSumForce);
 }
 
 void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,

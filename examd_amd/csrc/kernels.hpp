@@ -21,7 +21,11 @@ struct DevState {
   int *chunk = nullptr;    // scan partials
   StepLog *log = nullptr;  // per-step tallies
   unsigned long long *home = nullptr;  // home-block pair counts (diagnostic)
+  Partial *part = nullptr; // per-workgroup partial tallies
 };
+
+// Partial slots needed by a launch sequence over n atoms and G tiles.
+int64_t partial_slots(int64_t n, int64_t tiles);
 
 struct LJParams {
   double eps, sig, rc, mass;
@@ -34,7 +38,7 @@ struct ForceConfig {
   int maxrb = 0;      // largest region (home + shell) in blocks
 };
 
-size_t force_lds_bytes(bool fp64, int cap, int maxrb);
+size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads);
 
 // Binning / integration modes of launch_step_bin.
 enum BinMode { kBinOnly = 0, kKickDriftBin = 1, kFinishKickDriftBin = 2 };

@@ -29,7 +29,8 @@ Module exaMD_hip_binding
      Integer(c_int32_t) :: tiles(3)
      Integer(c_int32_t) :: device
      Integer(c_int32_t) :: block_threads
-     Integer(c_int32_t) :: reserved(5)
+     Integer(c_int32_t) :: cap_slack_pct
+     Integer(c_int32_t) :: reserved(4)
   End Type examd_params
 
   Type, Bind(C) :: examd_stats

@@ -65,7 +65,9 @@ typedef struct examd_params {
     int32_t device;       /* HIP device ordinal */
     int32_t block_threads;/* force-kernel workgroup size (0 = choose;
                              256, 512 or 1024) */
-    int32_t reserved[5];
+    int32_t cap_slack_pct;/* LDS atom capacity of a cell block above the
+                             largest region at set-up, in percent (0 = 15) */
+    int32_t reserved[4];
 } examd_params;
 
 /* Tallies of one step or one force evaluation. */

@@ -97,7 +97,7 @@ def test_home_pair_ownership_bit_exact(E, oracle, prec):
         assert np.abs(got - hc).sum() <= max(2, int(1e-5 * t.cut))
 
 
-@pytest.mark.parametrize("tiles,threads", [((1, 1, 1), 256), ((2, 3, 5), 512),
+@pytest.mark.parametrize("tiles,threads", [((2, 2, 2), 256), ((2, 3, 5), 512),
                                            ((13, 13, 13), 256), ((4, 4, 4), 1024)])
 def test_fp64_tile_shapes(E, oracle, tiles, threads):
     print(f"[tiles] {tiles} threads={threads}", flush=True)
@@ -190,6 +190,16 @@ def test_out_of_box_status(E):
         with pytest.raises(E.ExamdError) as ei:
             m.forces("fp64")
         assert ei.value.status == E.ERR_OUT_OF_BOX
+
+
+def test_fixed_tiles_too_large_rejected(E):
+    """A fixed cell-block map whose region cannot fit in LDS is refused
+    with a status, not run."""
+    pos, box = _case(10, 0.1, 6)
+    with E.MidpointLJ(len(pos), box, 2.5, tiles=(1, 1, 1)) as m:
+        with pytest.raises(E.ExamdError) as ei:
+            m.set_state(pos)
+        assert ei.value.status == E.ERR_TILE_OVERFLOW
 
 
 def test_small_grid_rejected(E):
