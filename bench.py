@@ -31,7 +31,6 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 vector (packed)
 FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector
 NC = 80                       # FCC cells per side -> 2,048,000 atoms
 RHO, TEMP, RC, DT = 0.8442, 0.72, 2.5, 0.005
-TILES = (16, 16, 16)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles",
                                "k_forces_mixed_2048000_traffic.txt")
 
@@ -65,9 +64,27 @@ def read_traffic():
     return None
 
 
-def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every):
-    with E.MidpointLJ(len(pos), box, RC, tiles=TILES) as m:
+def parse_shape(text):
+    """'Tx,Ty,Tz,threads,groups' -> dict, or None."""
+    if not text:
+        return None
+    v = [int(x) for x in text.split(",")]
+    return {"tiles": tuple(v[:3]), "block_threads": v[3],
+            "run_groups": v[4] if len(v) > 4 else 0}
+
+
+def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
+              shape=None):
+    """Autotune the launch shape for this precision on the initial state
+    (or apply the given shape), then warm up and time `steps` MD steps."""
+    with E.MidpointLJ(len(pos), box, RC) as m:
         m.set_state(pos, vel)
+        if shape is None:
+            tune = m.autotune(prec, trials=3)
+        else:
+            m.set_config(prec, shape["tiles"], shape["block_threads"],
+                         shape["run_groups"])
+            tune = m.get_config(prec)
         if warmup:
             m.run(warmup, DT, prec, energy_every=0)
         m.timing(enable=True, reset=True)
@@ -75,7 +92,7 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every):
         stats = m.run(steps, DT, prec, energy_every=energy_every)
         t1 = time.perf_counter()
         force_ms, launches = m.timing(enable=False, reset=True)
-    return t1 - t0, stats, force_ms / max(launches, 1), launches
+    return t1 - t0, stats, force_ms / max(launches, 1), launches, tune
 
 
 def cpu_baseline(pos, vel, box, nsteps):
@@ -90,6 +107,37 @@ def cpu_baseline(pos, vel, box, nsteps):
     return len(pos) * nsteps / dt
 
 
+def pmc_probe(E, shape):
+    """Short program for rocprofv3 --pmc runs: three mixed force evaluations
+    of the bench system at the autotuned shape, then three device-to-device
+    copies of 256 MiB (a plain copy kernel of known size, to calibrate the
+    FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes
+    pos, vel, box = E.generate_fcc(NC, RHO, TEMP, seed=2048)
+    with E.MidpointLJ(len(pos), box, RC) as m:
+        m.set_state(pos, vel)
+        if shape is None:
+            print("tune", m.autotune("mixed", trials=2), flush=True)
+        else:
+            m.set_config("mixed", shape["tiles"], shape["block_threads"],
+                         shape["run_groups"])
+        for _ in range(3):
+            m.forces("mixed")
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"pmc probe done: copy bytes {nbytes}", flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +146,12 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run the short counter-collection program instead")
+    ap.add_argument("--mixed-shape", default=None,
+                    help="Tx,Ty,Tz,threads,groups instead of autotuning")
+    ap.add_argument("--fp64-shape", default=None,
+                    help="Tx,Ty,Tz,threads,groups instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the "
@@ -107,16 +161,20 @@ def main():
 
     import examd_amd as E
     E.lib()
+    if args.pmc_probe:
+        return pmc_probe(E, parse_shape(args.mixed_shape))
     pos, vel, box = E.generate_fcc(NC, RHO, TEMP, seed=2048)
     n = len(pos)
     steps = args.steps
     fsteps = args.fp64_steps if args.fp64_steps is not None else steps
     energy_every = 10
 
-    wall_mx, st_mx, force_ms_mx, launches = timed_run(
-        E, pos, vel, box, "mixed", steps, args.warmup, energy_every)
-    wall_64, st_64, force_ms_64, _ = timed_run(
-        E, pos, vel, box, "fp64", fsteps, args.warmup, energy_every)
+    wall_mx, st_mx, force_ms_mx, launches, tune_mx = timed_run(
+        E, pos, vel, box, "mixed", steps, args.warmup, energy_every,
+        parse_shape(args.mixed_shape))
+    wall_64, st_64, force_ms_64, _, tune_64 = timed_run(
+        E, pos, vel, box, "fp64", fsteps, args.warmup, energy_every,
+        parse_shape(args.fp64_shape))
 
     rate_mx = n * steps / wall_mx
     rate_64 = n * fsteps / wall_64
@@ -155,8 +213,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "LJ liquid 2,048,000 atoms (80^3 FCC, rho 0.8442, "
-                        "T* 0.72, rc 2.5, dt 0.005), 107^3 midpoint blocks "
-                        "in 16^3 cell blocks, rebin every step",
+                        "T* 0.72, rc 2.5, dt 0.005), 107^3 midpoint blocks, "
+                        "cell-block map chosen by the autotuner, rebin "
+                        "every step",
             "natoms": n,
             "precision": "mixed: fp32 pair arithmetic, int64 fixed-point "
                          "(2^-32) force/energy accumulation, fp64 integration",
@@ -173,6 +232,8 @@ def main():
             "fp64_force_frac_of_fp64_peak": achieved64 / FP64_PEAK_TFLOPS,
             "energy_drift_per_1k_mixed": drift_per_1k(st_mx),
             "energy_drift_per_1k_fp64": drift_per_1k(st_64),
+            "launch_mixed": tune_mx,
+            "launch_fp64": tune_64,
             "parallelism": "1 GPU",
         },
         "roofline": {

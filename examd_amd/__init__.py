@@ -42,6 +42,8 @@ EXPORTS = (
     "examd_hip_set_state", "examd_hip_get_state", "examd_hip_forces",
     "examd_hip_run", "examd_hip_block_ids", "examd_hip_home_pair_counts",
     "examd_hip_timing", "examd_hip_generate_fcc", "examd_hip_strerror",
+    "examd_hip_phase_profile", "examd_hip_autotune", "examd_hip_get_config",
+    "examd_hip_set_config",
 )
 
 
@@ -57,7 +59,17 @@ class Params(C.Structure):
                 ("sigma", C.c_double), ("mass", C.c_double),
                 ("tiles", C.c_int32 * 3), ("device", C.c_int32),
                 ("block_threads", C.c_int32), ("cap_slack_pct", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("run_groups", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("tiles", C.c_int32 * 3), ("block_threads", C.c_int32),
+                ("run_groups", C.c_int32), ("reserved", C.c_int32),
+                ("force_ms", C.c_double)]
+
+    def as_dict(self):
+        return {"tiles": tuple(self.tiles), "block_threads": self.block_threads,
+                "run_groups": self.run_groups, "force_ms": self.force_ms}
 
 
 class Stats(C.Structure):
@@ -96,6 +108,10 @@ def lib():
         L.examd_hip_home_pair_counts.argtypes = [C.c_void_p, C.c_int, ip64,
                                                  C.c_int64]
         L.examd_hip_timing.argtypes = [C.c_void_p, C.c_int, dp, ip64, C.c_int]
+        L.examd_hip_phase_profile.argtypes = [C.c_void_p, C.c_int, dp]
+        L.examd_hip_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, P(Tune)]
+        L.examd_hip_get_config.argtypes = [C.c_void_p, C.c_int, P(Tune)]
+        L.examd_hip_set_config.argtypes = [C.c_void_p, C.c_int, P(Tune)]
         L.examd_hip_generate_fcc.argtypes = [ip32, C.c_double, C.c_double,
                                              C.c_double, C.c_double,
                                              C.c_uint64, dp, dp, dp]
@@ -157,7 +173,7 @@ class MidpointLJ:
 
     def __init__(self, natoms, box, cutoff, epsilon=1.0, sigma=1.0, mass=1.0,
                  tiles=(0, 0, 0), device=0, block_threads=0,
-                 cap_slack_pct=0):
+                 cap_slack_pct=0, run_groups=0):
         p = Params()
         p.natoms = int(natoms)
         p.box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
@@ -167,6 +183,7 @@ class MidpointLJ:
         p.device = int(device)
         p.block_threads = int(block_threads)
         p.cap_slack_pct = int(cap_slack_pct)
+        p.run_groups = int(run_groups)
         self.params = p
         self.n = int(natoms)
         self.box = np.array(np.reshape(box, 3), dtype=np.float64)
@@ -241,6 +258,35 @@ class MidpointLJ:
             self._h, _PREC[precision], c.ctypes.data_as(C.POINTER(C.c_int64)),
             self.nblocks), "examd_hip_home_pair_counts")
         return c
+
+    def autotune(self, precision="mixed", trials=3):
+        """Pick the fastest force-kernel launch shape for this system."""
+        t = Tune()
+        _check(lib().examd_hip_autotune(self._h, _PREC[precision], int(trials),
+                                        C.byref(t)), "examd_hip_autotune")
+        return t.as_dict()
+
+    def get_config(self, precision="mixed"):
+        t = Tune()
+        _check(lib().examd_hip_get_config(self._h, _PREC[precision],
+                                          C.byref(t)), "examd_hip_get_config")
+        return t.as_dict()
+
+    def set_config(self, precision, tiles, block_threads, run_groups=0):
+        t = Tune()
+        t.tiles = (C.c_int32 * 3)(*[int(v) for v in tiles])
+        t.block_threads = int(block_threads)
+        t.run_groups = int(run_groups)
+        _check(lib().examd_hip_set_config(self._h, _PREC[precision],
+                                          C.byref(t)), "examd_hip_set_config")
+
+    def phase_profile(self, precision="mixed"):
+        """Mean cycles per cell block: scan, gather, pairs, return, total."""
+        out = np.zeros(5)
+        _check(lib().examd_hip_phase_profile(self._h, _PREC[precision],
+                                             _dp(out)),
+               "examd_hip_phase_profile")
+        return out
 
     def timing(self, enable=True, reset=False):
         ms = C.c_double()

@@ -15,6 +15,17 @@
 
 using namespace examd;
 
+// Cell-block map and force-kernel launch shape of one precision.
+struct TileConfig {
+  int T[3] = {1, 1, 1};   // cell blocks per dimension
+  int threads = 256;      // workgroup size
+  int groups = 1;         // run groups per region atom
+  int cap = 0;            // LDS atom capacity
+  int maxrb = 0;          // largest region in blocks
+  bool fixed_tiles = false;
+  int fixed_groups = 0;
+};
+
 struct examd_handle {
   examd_params p{};
   int dev = 0;
@@ -25,11 +36,10 @@ struct examd_handle {
   DevState d{};
   StepLog *log = nullptr;
   int64_t log_cap = 0;
-  int tile_threads = 256;
-  int maxrb = 0;
-  int cap[2] = {0, 0};      // LDS capacity per precision
+  TileConfig tc[2];         // cell-block map and launch shape per precision
   size_t lds_limit = 0;
   int64_t part_cap = 0;
+  std::vector<double> host_pos;   // positions of the last set_state (AoS)
   bool state_set = false;
   int forces_valid = -1;    // precision whose forces match the positions
   int last_prec = 0;
@@ -81,9 +91,19 @@ void free_state(examd_handle *h) {
   h->part_cap = 0;
 }
 
-// Partial-tally slots for the current tile map.
+int64_t tiles_of(const TileConfig &c) {
+  return (int64_t)c.T[0] * c.T[1] * c.T[2];
+}
+
+Grid grid_for(const examd_handle *h, int prec) {
+  Grid g = h->g;
+  for (int a = 0; a < 3; ++a) g.T[a] = h->tc[prec].T[a];
+  return g;
+}
+
+// Partial-tally slots for the current tile maps.
 int ensure_partials(examd_handle *h) {
-  int64_t need = partial_slots(h->n, (int64_t)h->g.T[0] * h->g.T[1] * h->g.T[2]);
+  int64_t need = partial_slots(h->n, std::max(tiles_of(h->tc[0]), tiles_of(h->tc[1])));
   if (h->part_cap >= need) return EXAMD_OK;
   if (h->d.part) (void)hipFree(h->d.part);
   h->d.part = nullptr;
@@ -161,64 +181,67 @@ int64_t max_region_atoms(const Grid &g, const double *pos, int64_t n) {
   return best;
 }
 
-// Pick the LDS capacity of a cell block for both precisions from the
-// largest region of the given positions, with cap_slack_pct headroom for
-// density fluctuations during a run (at least 5%).  When the tiles were chosen
+// Pick the LDS capacity of a cell block for one precision from the largest
+// region of the given positions, with cap_slack_pct headroom for density
+// fluctuations during a run (at least 5%).  When the tiles were chosen
 // automatically and a region does not fit the device's LDS, refine the
 // cell-block map along its coarsest axis.
-int choose_caps(examd_handle *h, const double *pos) {
+int choose_caps(examd_handle *h, int prec, const double *pos) {
+  TileConfig &c = h->tc[prec];
+  const bool fp64 = prec == EXAMD_PREC_FP64;
   for (int iter = 0; iter < 256; ++iter) {
-    h->maxrb = max_region_blocks(h->g);
+    Grid g = grid_for(h, prec);
+    c.maxrb = max_region_blocks(g);
     // region coordinates are packed 8 bits per axis on the device
-    bool dims_ok = true;
-    for (int a = 0; a < 3; ++a) {
-      int lo, hi;
-      tile_range(0, h->g.nb[a], h->g.T[a], lo, hi);
-      if ((h->g.nb[a] + h->g.T[a] - 1) / h->g.T[a] + 2 > 255) dims_ok = false;
-    }
-    int64_t mra = dims_ok ? max_region_atoms(h->g, pos, h->n) : 0;
-    const double slack = (h->p.cap_slack_pct > 0 ? h->p.cap_slack_pct : 15) / 100.0;
+    bool ok = true;
+    for (int a = 0; a < 3; ++a)
+      if ((g.nb[a] + g.T[a] - 1) / g.T[a] + 2 > 255) ok = false;
+    int64_t mra = ok ? max_region_atoms(g, pos, h->n) : 0;
+    const double slack =
+        (h->p.cap_slack_pct > 0 ? h->p.cap_slack_pct : 15) / 100.0;
     int64_t want = ((int64_t)std::ceil((double)mra * (1.0 + slack)) + 32 + 15) / 16 * 16;
     int64_t need = (int64_t)std::ceil((double)mra * 1.05) + 16;
-    bool ok = true;
-    int caps[2];
-    for (int prec = 0; prec < 2; ++prec) {
-      bool fp64 = prec == EXAMD_PREC_FP64;
-      int64_t c = want;
-      while (c > need &&
-             force_lds_bytes(fp64, (int)c, h->maxrb, h->tile_threads) > h->lds_limit)
-        c -= 16;
-      if (force_lds_bytes(fp64, (int)c, h->maxrb, h->tile_threads) > h->lds_limit) ok = false;
-      caps[prec] = (int)c;
-    }
-    if (!dims_ok || want > 65535) ok = false;
+    int64_t cap = want;
+    while (cap > need &&
+           force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads) > h->lds_limit)
+      cap -= 16;
+    if (force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads) > h->lds_limit ||
+        cap > 65535)
+      ok = false;
     if (ok) {
-      h->cap[0] = caps[0];
-      h->cap[1] = caps[1];
+      c.cap = (int)cap;
+      // about three work items per lane: run groups from the mean region
+      const double vol = g.L[0] * g.L[1] * g.L[2];
+      double reg = (double)h->n / vol;
+      for (int a = 0; a < 3; ++a) reg *= ((double)g.nb[a] / g.T[a] + 2.0) * g.bl;
+      int ng = c.fixed_groups > 0
+                   ? c.fixed_groups
+                   : (int)std::lround(3.0 * c.threads / std::max(reg, 1.0));
+      c.groups = std::max(1, std::min(4, ng));
       return EXAMD_OK;
     }
-    bool fixed = h->p.tiles[0] > 0 || h->p.tiles[1] > 0 || h->p.tiles[2] > 0;
-    if (fixed) return EXAMD_ERR_TILE_OVERFLOW;
+    if (c.fixed_tiles) return EXAMD_ERR_TILE_OVERFLOW;
     int ax = -1;
     double best = 0.0;
     for (int a = 0; a < 3; ++a) {
-      double size = (double)h->g.nb[a] / h->g.T[a];
-      if (h->g.T[a] < h->g.nb[a] && size > best) {
+      double size = (double)g.nb[a] / c.T[a];
+      if (c.T[a] < g.nb[a] && size > best) {
         best = size;
         ax = a;
       }
     }
     if (ax < 0) return EXAMD_ERR_TILE_OVERFLOW;
-    h->g.T[ax] += 1;
+    c.T[ax] += 1;
   }
   return EXAMD_ERR_TILE_OVERFLOW;
 }
 
 ForceConfig force_config(const examd_handle *h, int prec) {
   ForceConfig c;
-  c.threads = h->tile_threads;
-  c.cap = h->cap[prec];
-  c.maxrb = h->maxrb;
+  c.threads = h->tc[prec].threads;
+  c.cap = h->tc[prec].cap;
+  c.maxrb = h->tc[prec].maxrb;
+  c.groups = h->tc[prec].groups;
   return c;
 }
 
@@ -244,7 +267,7 @@ void forces_timed(examd_handle *h, int prec, bool energy, bool homecount,
     e1 = next_event(h);
     if (e0) (void)hipEventRecord(e0, h->stream);
   }
-  launch_forces(h->d, h->g, lj(h), force_config(h, prec),
+  launch_forces(h->d, grid_for(h, prec), lj(h), force_config(h, prec),
                 prec == EXAMD_PREC_MIXED, energy, homecount, log, h->stream);
   if (e1) (void)hipEventRecord(e1, h->stream);
 }
@@ -349,16 +372,23 @@ int examd_hip_create(const examd_params *prm, examd_handle **out) {
   h->n = prm->natoms;
   h->nblocks = nblocks;
   Grid &g = h->g;
+  const bool fixed = prm->tiles[0] > 0 || prm->tiles[1] > 0 || prm->tiles[2] > 0;
+  const int thr = prm->block_threads;
+  for (int prec = 0; prec < 2; ++prec) {
+    TileConfig &c = h->tc[prec];
+    c.fixed_tiles = fixed;
+    c.fixed_groups = prm->run_groups;
+    c.threads = (thr == 256 || thr == 512 || thr == 1024) ? thr : 512;
+  }
   for (int a = 0; a < 3; ++a) {
     g.nb[a] = nbxyz[a];
     g.L[a] = prm->box[a];
     int T = prm->tiles[a] > 0 ? prm->tiles[a] : (nbxyz[a] + 6) / 7;
-    g.T[a] = std::max(1, std::min(T, (int)nbxyz[a]));
+    T = std::max(1, std::min(T, (int)nbxyz[a]));
+    g.T[a] = T;
+    h->tc[0].T[a] = h->tc[1].T[a] = T;
   }
   g.bl = 0.5 * prm->cutoff;
-  h->maxrb = max_region_blocks(g);
-  int thr = prm->block_threads;
-  h->tile_threads = (thr == 512 || thr == 1024) ? thr : 256;
 
   if (hipSetDevice(h->dev) != hipSuccess) {
     delete h;
@@ -422,9 +452,12 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
   HIPCHK(hipSetDevice(h->dev));
   for (int64_t i = 0; i < 3 * n; ++i)
     if (!std::isfinite(pos[i])) return EXAMD_ERR_NONFINITE;
-  int st = choose_caps(h, pos);
-  if (st) return st;
+  for (int prec = 0; prec < 2; ++prec) {
+    int st = choose_caps(h, prec, pos);
+    if (st) return st;
+  }
   if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
+  h->host_pos.assign(pos, pos + 3 * n);
   std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
   double *dst[3] = {h->d.x, h->d.y, h->d.z};
   double *vdst[3] = {h->d.vx, h->d.vy, h->d.vz};
@@ -632,6 +665,184 @@ int examd_hip_home_pair_counts(examd_handle *h, int precision, int64_t *counts,
   if (h->timing) collect_timing(h);
   h->forces_valid = precision;
   return flags_to_status(L.flags & ~(unsigned long long)kFlagZero);
+}
+
+}  // extern "C"
+
+namespace {
+
+int download_positions(examd_handle *h, std::vector<double> &aos) {
+  const int64_t n = h->n;
+  aos.assign((size_t)std::max<int64_t>(3 * n, 1), 0.0);
+  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
+  const double *src[3] = {h->d.x, h->d.y, h->d.z};
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int a = 0; a < 3; ++a) {
+    HIPCHK(hipMemcpy(buf.data(), src[a], n * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) aos[3 * i + a] = buf[i];
+  }
+  return EXAMD_OK;
+}
+
+// Apply a launch shape to one precision (cell-block map, threads, groups).
+int apply_config(examd_handle *h, int prec, const int T[3], int threads,
+                 int groups, const double *pos) {
+  TileConfig saved = h->tc[prec];
+  TileConfig &c = h->tc[prec];
+  for (int a = 0; a < 3; ++a) c.T[a] = std::max(1, std::min(T[a], h->g.nb[a]));
+  c.threads = threads;
+  c.fixed_tiles = true;
+  c.fixed_groups = groups;
+  int st = choose_caps(h, prec, pos);
+  if (st == EXAMD_OK) st = ensure_partials(h);
+  if (st != EXAMD_OK) h->tc[prec] = saved;
+  if (h->forces_valid == prec) h->forces_valid = -1;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int examd_hip_get_config(examd_handle *h, int precision, examd_tune *out) {
+  if (!h || !out ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  const TileConfig &c = h->tc[precision];
+  for (int a = 0; a < 3; ++a) out->tiles[a] = c.T[a];
+  out->block_threads = c.threads;
+  out->run_groups = c.groups;
+  out->reserved = 0;
+  out->force_ms = 0.0;
+  return EXAMD_OK;
+}
+
+int examd_hip_set_config(examd_handle *h, int precision,
+                         const examd_tune *cfg) {
+  if (!h || !cfg ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  const int thr = cfg->block_threads;
+  if (thr != 256 && thr != 512 && thr != 1024) return EXAMD_ERR_ARG;
+  if (cfg->run_groups < 0 || cfg->run_groups > 4) return EXAMD_ERR_ARG;
+  for (int a = 0; a < 3; ++a)
+    if (cfg->tiles[a] <= 0) return EXAMD_ERR_ARG;
+  HIPCHK(hipSetDevice(h->dev));
+  std::vector<double> pos;
+  int st = download_positions(h, pos);
+  if (st) return st;
+  return apply_config(h, precision, cfg->tiles, thr, cfg->run_groups,
+                      pos.data());
+}
+
+int examd_hip_autotune(examd_handle *h, int precision, int trials,
+                       examd_tune *best) {
+  if (!h || trials <= 0 ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  HIPCHK(hipSetDevice(h->dev));
+  std::vector<double> pos;
+  int st = download_positions(h, pos);
+  if (st) return st;
+  // candidate cell-block sizes (home blocks per side) and launch shapes
+  static const double sizes[] = {3.5, 4.0, 4.5, 5.0, 5.5, 6.0, 6.7, 7.5, 8.5};
+  static const int threads[] = {256, 512, 1024};
+  const bool save_timing = h->timing;
+  const double save_ms = h->force_ms;
+  const int64_t save_n = h->force_launches;
+  const TileConfig start = h->tc[precision];
+  TileConfig bestc = start;
+  double best_ms = 1e300;
+  for (double sz : sizes)
+    for (int thr : threads)
+      for (int grp = 0; grp <= (thr == 1024 ? 3 : 1); ++grp) {
+        if (grp == 1) continue;  // 0 = automatic choice covers one group
+        int T[3];
+        for (int a = 0; a < 3; ++a)
+          T[a] = std::max(1, (int)std::lround(h->g.nb[a] / sz));
+        if (apply_config(h, precision, T, thr, grp, pos.data()) != EXAMD_OK)
+          continue;
+        launch_fill_log(h->log, 1, h->stream);
+        evaluate(h, precision, false, false, h->log);  // warm-up
+        h->timing = true;
+        h->ev_used = 0;
+        h->force_ms = 0.0;
+        h->force_launches = 0;
+        for (int t = 0; t < trials; ++t) {
+          launch_fill_log(h->log, 1, h->stream);
+          evaluate(h, precision, false, false, h->log);
+        }
+        hipError_t e = hipStreamSynchronize(h->stream);
+        collect_timing(h);
+        StepLog L;
+        if (e == hipSuccess)
+          e = hipMemcpy(&L, h->log, sizeof(L), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+          h->timing = save_timing;
+          return EXAMD_ERR_HIP;
+        }
+        if (L.flags & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite))
+          continue;
+        const double ms = h->force_ms / std::max<int64_t>(h->force_launches, 1);
+        if (ms < best_ms) {
+          best_ms = ms;
+          bestc = h->tc[precision];
+        }
+      }
+  h->timing = save_timing;
+  h->force_ms = save_ms;
+  h->force_launches = save_n;
+  h->ev_used = 0;
+  if (best_ms >= 1e300) {
+    h->tc[precision] = start;
+    return EXAMD_ERR_TILE_OVERFLOW;
+  }
+  st = apply_config(h, precision, bestc.T, bestc.threads, bestc.fixed_groups,
+                    pos.data());
+  h->forces_valid = -1;
+  if (st) return st;
+  if (best) {
+    examd_hip_get_config(h, precision, best);
+    best->force_ms = best_ms;
+  }
+  return EXAMD_OK;
+}
+
+int examd_hip_phase_profile(examd_handle *h, int precision,
+                            double phase_cycles[5]) {
+  if (!h || !phase_cycles ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  if (!h->state_set) return EXAMD_ERR_NOT_READY;
+  HIPCHK(hipSetDevice(h->dev));
+  const int64_t G = tiles_of(h->tc[precision]);
+  unsigned long long *buf = nullptr;
+  if (dalloc(&buf, G * kProfSlots)) return EXAMD_ERR_NOMEM;
+  h->d.prof = buf;
+  launch_fill_log(h->log, 1, h->stream);
+  evaluate(h, precision, false, false, h->log);
+  h->d.prof = nullptr;
+  std::vector<unsigned long long> t((size_t)(G * kProfSlots));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(t.data(), buf, t.size() * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, h->stream);
+  hipError_t e2 = hipStreamSynchronize(h->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess || e2 != hipSuccess) return EXAMD_ERR_HIP;
+  if (h->timing) collect_timing(h);
+  h->forces_valid = precision;
+  for (int k = 0; k < 5; ++k) phase_cycles[k] = 0.0;
+  for (int64_t b = 0; b < G; ++b) {
+    const unsigned long long *p = &t[(size_t)(b * kProfSlots)];
+    for (int k = 0; k < 4; ++k) phase_cycles[k] += (double)(p[k + 1] - p[k]);
+    phase_cycles[4] += (double)(p[4] - p[0]);
+  }
+  for (int k = 0; k < 5; ++k) phase_cycles[k] /= (double)G;
+  return EXAMD_OK;
 }
 
 int examd_hip_timing(examd_handle *h, int enable, double *force_ms,

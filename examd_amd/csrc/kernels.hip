@@ -333,7 +333,7 @@ struct ForceArgs {
   DevState d;
   Grid g;
   StepLog *log;
-  int cap, maxrb;
+  int cap, maxrb, groups;
   double rc2, eps24, eps4, sig2, d2zero, d2fix;
   float rc2f, eps24f, eps4f, sig2f, d2zerof, d2fixf;
 };
@@ -342,17 +342,15 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
   return (v + a - 1) / a * a;
 }
 
-// LDS image of one cell block: region-block offsets, then atoms, then the
-// per-wave pair-walk scratch (candidate ranges and the hit queue).
-constexpr int kUnitsPerAtom = 16;   // >= the 13 ranges an atom can own
-constexpr int kQueue = 128;         // hit ring per wave (two drains deep)
+// LDS image of one cell block: region-block offsets, then atoms.  The atom
+// arrays hold cap + 4 entries: the pair loop reads four candidates at a
+// time and may look up to three entries past the last atom (masked off).
 struct LdsLayout {
-  size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_walk, total;
+  size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, total;
 };
-constexpr size_t kWalkBytesPerWave =
-    64 * kUnitsPerAtom * 4 + 2 * 65 * 4 + kQueue * 4;
 __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
                                                 int nwaves) {
+  cap += 4;
   LdsLayout L;
   size_t o = (size_t)(maxrb + 1) * 4;  // s_off
   L.off_gst = o;
@@ -369,10 +367,8 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   o += (size_t)cap * 4;
   L.off_bc = o;
   if (fp64) o += (size_t)cap * 4;
-  o = align_up(o, 16);
-  L.off_walk = o;
-  o += (size_t)nwaves * kWalkBytesPerWave;
   L.total = align_up(o, 16);
+  (void)nwaves;
   return L;
 }
 
@@ -403,6 +399,37 @@ __device__ __forceinline__ unsigned run_mask(unsigned ox, unsigned oy,
                 ((oz & 4u) ? z2 : 0u);
   return mx & my & mz;
 }
+
+// Per-run ownership masks for runtime run indices: kRunAxisMask[r][axis][o+1].
+struct RunAxisMasks {
+  unsigned char m[kNumRuns][3][3];
+  constexpr RunAxisMasks() : m() {
+    for (int r = 0; r < kNumRuns; ++r)
+      for (int a = 0; a < 3; ++a)
+        for (int o = -1; o <= 1; ++o)
+          m[r][a][o + 1] = (unsigned char)axis_mask(r, a, o);
+  }
+};
+__constant__ RunAxisMasks kRunMasks;
+
+__device__ __forceinline__ unsigned axis_sel(const unsigned char *m,
+                                             unsigned ok) {
+  return ((ok & 1u) ? m[0] : 0u) | ((ok & 2u) ? m[1] : 0u) |
+         ((ok & 4u) ? m[2] : 0u);
+}
+__device__ __forceinline__ unsigned run_mask_rt(int r, unsigned ox,
+                                                unsigned oy, unsigned oz) {
+  return axis_sel(kRunMasks.m[r][0], ox) & axis_sel(kRunMasks.m[r][1], oy) &
+         axis_sel(kRunMasks.m[r][2], oz);
+}
+
+// Run groups: the 13 runs split into 1..4 groups of similar candidate
+// counts (run 0 covers 2.5 blocks, the others 5).
+__constant__ int kGroupStart[5][5] = {{0, 13, 13, 13, 13},
+                                      {0, 13, 13, 13, 13},
+                                      {0, 7, 13, 13, 13},
+                                      {0, 5, 9, 13, 13},
+                                      {0, 4, 7, 10, 13}};
 
 // o1 of the table row with offset d = o2 - o1 (d in [-2, 2]^3, table
 // orientation), packed (o1x+1) | (o1y+1) << 2 | (o1z+1) << 4; for the
@@ -473,16 +500,20 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   int *s_bc = reinterpret_cast<int *>(smem + LL.off_bc);
   // FP64 image: SoA doubles + packed coords; mixed image: float4 with the
   // packed coords in w
+  const int cap4 = A.cap + 4;
   double *s_x = reinterpret_cast<double *>(smem + LL.off_pos);
-  double *s_y = s_x + A.cap;
-  double *s_z = s_y + A.cap;
+  double *s_y = s_x + cap4;
+  double *s_z = s_y + cap4;
   float4 *s_p4 = reinterpret_cast<float4 *>(smem + LL.off_pos);
   double *s_fx = reinterpret_cast<double *>(smem + LL.off_f);
-  double *s_fy = s_fx + A.cap;
-  double *s_fz = s_fy + A.cap;
+  double *s_fy = s_fx + cap4;
+  double *s_fz = s_fy + cap4;
   long long *s_qx = reinterpret_cast<long long *>(smem + LL.off_f);
-  long long *s_qy = s_qx + A.cap;
-  long long *s_qz = s_qy + A.cap;
+  long long *s_qy = s_qx + cap4;
+  long long *s_qz = s_qy + cap4;
+  unsigned long long *prof =
+      A.d.prof ? A.d.prof + (size_t)blockIdx.x * kProfSlots : nullptr;
+  if (prof && tid == 0) prof[0] = clock64();
 
   // 1. region blocks: home blocks of the tile plus a one-block ghost shell,
   //    periodic (SRC/exaMD_partitioning.f90:788-790).
@@ -519,10 +550,12 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
     __syncthreads();
   }
   const int R = s_off[NRB];
+  if (prof && tid == 0) prof[1] = clock64();
   if (R > A.cap) {
     if (tid == 0) {
       atomicOr(&A.log->flags, (unsigned long long)kFlagOverflow);
       A.d.part[blockIdx.x] = Partial{0.0, 0.0, 0ull, 0ull, 0ull};
+      if (prof) prof[2] = prof[3] = prof[4] = prof[1];
     }
     return;
   }
@@ -571,41 +604,46 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
     }
   }
   __syncthreads();
+  if (prof && tid == 0) prof[2] = clock64();
 
-  // 3. pairs.  Each wave takes batches of 64 region atoms.  Lane l lists the
-  //    candidate ranges atom (batch + l) owns: for each of the 13 runs, the
-  //    J blocks whose home block I - o1 lies in the tile, as contiguous LDS
-  //    ranges.  The wave then splits the batch's candidates evenly over its
-  //    lanes; in-cutoff pairs go to a per-wave ring and are evaluated 64 at
-  //    a time, so every lane does the same work in both phases.
-  const int wave = tid >> 6, lane = tid & 63;
-  unsigned *w_units = reinterpret_cast<unsigned *>(
-      smem + LL.off_walk + (size_t)wave * kWalkBytesPerWave);
-  int *w_nun = reinterpret_cast<int *>(w_units + 64 * kUnitsPerAtom);
-  int *w_cst = w_nun + 65;
-  unsigned *w_q = reinterpret_cast<unsigned *>(w_cst + 65);
-
+  // 3. pairs.  Work item w = (run group g, region atom a): the lane lists
+  //    the J ranges atom a owns in the group's runs (home block I - o1
+  //    inside the tile) and evaluates them four candidates at a time.  The
+  //    force on a accumulates in registers, the force on j with LDS atomics.
   unsigned long long cand = 0, cut = 0;
   unsigned flags = 0;
   double pe64 = 0.0;
   long long pefix = 0;
-
-  // One in-cutoff pair (a, j) in table orientation: force on a is +f, on j
-  // is -f (SRC/exaMD_forces.f90:538-542).
-  auto eval_pair = [&](int a, int j) {
+  const int NG = A.groups;
+  const int W = NG * R;
+  for (int w = tid; w < W; w += NT) {
+    const int grp = w / R;
+    const int a = w - grp * R;
+    int bc;
+    double xa = 0.0, ya = 0.0, za = 0.0;
+    float xf = 0.f, yf = 0.f, zf = 0.f;
     if constexpr (FP64) {
-      double btx = 0.0, bty = 0.0, btz = 0.0;
-      if (wraps) {
-        const int ba = s_bc[a], bj = s_bc[j];
-        btx = (double)(bc_s(ba, 0) - bc_s(bj, 0)) * g.L[0];
-        bty = (double)(bc_s(ba, 1) - bc_s(bj, 1)) * g.L[1];
-        btz = (double)(bc_s(ba, 2) - bc_s(bj, 2)) * g.L[2];
-      }
-      // SRC/exaMD_forces.f90:509-523 with Nint(dx*globalboxi) = bt
-      const double dx = (s_x[j] - s_x[a]) - btx;
-      const double dy = (s_y[j] - s_y[a]) - bty;
-      const double dz = (s_z[j] - s_z[a]) - btz;
-      const double d2 = dx * dx + dy * dy + dz * dz;
+      xa = s_x[a];
+      ya = s_y[a];
+      za = s_z[a];
+      bc = s_bc[a];
+    } else {
+      const float4 p = s_p4[a];
+      xf = p.x;
+      yf = p.y;
+      zf = p.z;
+      bc = __float_as_int(p.w);
+    }
+    const int ix = bc_x(bc), iy = bc_y(bc), iz = bc_z(bc);
+    const unsigned okx = own_bits(ix, tx), oky = own_bits(iy, ty),
+                   okz = own_bits(iz, tz);
+    const int sax = bc_s(bc, 0), say = bc_s(bc, 1), saz = bc_s(bc, 2);
+    double fax = 0.0, fay = 0.0, faz = 0.0;
+    long long qax = 0, qay = 0, qaz = 0;
+
+    // in-cutoff pair (a, j), table orientation: +f on a, -f on j
+    // (SRC/exaMD_forces.f90:538-542)
+    auto pair64 = [&](int j, double dx, double dy, double dz, double d2) {
       // user_func.f90:58-69
       const double d2i = 1.0 / d2;
       const double sg = A.sig2 * d2i;
@@ -613,18 +651,16 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
       const double d12 = d6 * d6;
       const double dg = -(A.eps24 * (2.0 * d12 - d6) * d2i);
       const double f1 = dg * dx, f2 = dg * dy, f3 = dg * dz;
-      lds_add(&s_fx[a], f1);
-      lds_add(&s_fy[a], f2);
-      lds_add(&s_fz[a], f3);
+      fax += f1;
+      fay += f2;
+      faz += f3;
       lds_add(&s_fx[j], -f1);
       lds_add(&s_fy[j], -f2);
       lds_add(&s_fz[j], -f3);
       if constexpr (ENERGY) pe64 += A.eps4 * (d12 - d6);
       if (d2 < A.d2zero) flags |= kFlagZero;
-    } else {
-      const float4 pa = s_p4[a], pj = s_p4[j];
-      const float dx = pj.x - pa.x, dy = pj.y - pa.y, dz = pj.z - pa.z;
-      const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    };
+    auto pair32 = [&](int j, float dx, float dy, float dz, float d2) {
       const float r2i = __builtin_amdgcn_rcpf(d2);
       const float s2 = A.sig2f * r2i;
       const float s6 = s2 * s2 * s2;
@@ -632,192 +668,138 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
       const long long q1 = fix32(-fs * dx);
       const long long q2 = fix32(-fs * dy);
       const long long q3 = fix32(-fs * dz);
-      lds_add(&s_qx[a], q1);
-      lds_add(&s_qy[a], q2);
-      lds_add(&s_qz[a], q3);
+      qax += q1;
+      qay += q2;
+      qaz += q3;
       lds_sub(&s_qx[j], q1);
       lds_sub(&s_qy[j], q2);
       lds_sub(&s_qz[j], q3);
       if constexpr (ENERGY) pefix += fix32(A.eps4f * s6 * (s6 - 1.0f));
       if (d2 < A.d2zerof) flags |= kFlagZero;
       if (d2 < A.d2fixf) flags |= kFlagFixedRange;
-    }
-    if constexpr (HOMECNT) {
-      const int ba = FP64 ? s_bc[a] : __float_as_int(s_p4[a].w);
-      const int bj = FP64 ? s_bc[j] : __float_as_int(s_p4[j].w);
-      const int di = (bc_x(bj) - bc_x(ba) + 2) + 5 * (bc_y(bj) - bc_y(ba) + 2) +
-                     25 * (bc_z(bj) - bc_z(ba) + 2);
-      const unsigned o = kO1.v[di];
-      const int hx = h0x - 1 + bc_x(ba) - ((int)(o & 3) - 1);
-      const int hy = h0y - 1 + bc_y(ba) - ((int)((o >> 2) & 3) - 1);
-      const int hz = h0z - 1 + bc_z(ba) - ((int)((o >> 4) & 3) - 1);
-      atomicAdd(&A.d.home[hx + nbx * (hy + nby * hz)], 1ull);
-    }
-    ++cut;
-  };
-
-  for (int base = wave * 64; base < R; base += NW * 64) {
-    // 3a. candidate ranges of atom a = base + lane
-    const int a = base + lane;
-    int nun = 0, ncand = 0;
-    unsigned *my = w_units + lane * kUnitsPerAtom;
-    if (a < R) {
-      const int bc = FP64 ? s_bc[a] : __float_as_int(s_p4[a].w);
-      const int ix = bc_x(bc), iy = bc_y(bc), iz = bc_z(bc);
-      const unsigned okx = own_bits(ix, tx), oky = own_bits(iy, ty),
-                     okz = own_bits(iz, tz);
-      static_for<0, kNumRuns>([&](auto RUN) {
-        constexpr int r = decltype(RUN)::value;
-        unsigned mask = run_mask<r>(okx, oky, okz);
-        const int row0 = RX * ((iy + kRunDY[r]) + RY * (iz + kRunDZ[r])) + ix - 2;
-        while (mask) {
-          const int lo = __builtin_ctz(mask);
-          const int hi = lo + __builtin_ctz(~(mask >> lo)) - 1;
-          mask &= ~((2u << hi) - 1u);
-          // self run, dx = 0: the block's own triangle, j > a
-          const int j0 = (r == 0 && lo == 2) ? a + 1 : s_off[row0 + lo];
-          const int j1 = s_off[row0 + hi + 1];
-          if (j1 > j0) {
-            my[nun++] = (unsigned)j0 | ((unsigned)j1 << 16);
-            ncand += j1 - j0;
-          }
-        }
-      });
-    }
-    w_nun[lane] = nun;
-    // wave prefix of candidate counts
-    int incl = ncand;
-    for (int o = 1; o < 64; o <<= 1) {
-      int t = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += t;
-    }
-    const int total = __shfl(incl, 63, 64);
-    w_cst[lane] = incl - ncand;
-    if (lane == 63) w_cst[64] = total;
-    wave_sync();
-    cand += (unsigned long long)ncand;
-
-    // 3b. balanced walk: lane takes candidates [f0, f1) of the batch
-    const int per = (total + 63) >> 6;
-    const int f0 = min(total, lane * per), f1 = min(total, f0 + per);
-    int l = 0, k = 0, j = 0, jend = 0;
-    if (f0 < f1) {
-      // owner lane of f0: last lane with cst <= f0 and units
-      int lo = 0, hi = 63;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (w_cst[mid] <= f0) lo = mid;
-        else hi = mid - 1;
-      }
-      l = lo;
-      while (w_nun[l] == 0 || w_cst[l + 1] <= f0) ++l;
-      int off = f0 - w_cst[l];
-      for (;;) {
-        const unsigned u = w_units[l * kUnitsPerAtom + k];
-        const int len = (int)(u >> 16) - (int)(u & 0xffff);
-        if (off < len) {
-          j = (int)(u & 0xffff) + off;
-          jend = (int)(u >> 16);
-          break;
-        }
-        off -= len;
-        ++k;
-      }
-    }
-    int ia = base + l;
-    double xa = 0.0, ya = 0.0, za = 0.0;
-    float xf = 0.f, yf = 0.f, zf = 0.f;
-    int sax = 0, say = 0, saz = 0;
-    auto load_a = [&]() {
-      if constexpr (FP64) {
-        xa = s_x[ia];
-        ya = s_y[ia];
-        za = s_z[ia];
-        if (wraps) {
-          const int ba = s_bc[ia];
-          sax = bc_s(ba, 0);
-          say = bc_s(ba, 1);
-          saz = bc_s(ba, 2);
-        }
-      } else {
-        const float4 p = s_p4[ia];
-        xf = p.x;
-        yf = p.y;
-        zf = p.z;
+    };
+    auto home_count = [&](int j) {
+      if constexpr (HOMECNT) {
+        const int bj = FP64 ? s_bc[j] : __float_as_int(s_p4[j].w);
+        const int di = (bc_x(bj) - ix + 2) + 5 * (bc_y(bj) - iy + 2) +
+                       25 * (bc_z(bj) - iz + 2);
+        const unsigned o = kO1.v[di];
+        const int hx = h0x - 1 + ix - ((int)(o & 3) - 1);
+        const int hy = h0y - 1 + iy - ((int)((o >> 2) & 3) - 1);
+        const int hz = h0z - 1 + iz - ((int)((o >> 4) & 3) - 1);
+        atomicAdd(&A.d.home[hx + nbx * (hy + nby * hz)], 1ull);
       }
     };
-    if (f0 < f1) load_a();
 
-    int qhead = 0, qtail = 0;   // wave-uniform ring indices
-    for (int it = 0; it < per; ++it) {
-      const bool act = f0 + it < f1;
-      bool hit = false;
-      if (act) {
-        if constexpr (FP64) {
-          double btx = 0.0, bty = 0.0, btz = 0.0;
-          if (wraps) {
-            const int bj = s_bc[j];
-            btx = (double)(sax - bc_s(bj, 0)) * g.L[0];
-            bty = (double)(say - bc_s(bj, 1)) * g.L[1];
-            btz = (double)(saz - bc_s(bj, 2)) * g.L[2];
-          }
-          const double dx = (s_x[j] - xa) - btx;
-          const double dy = (s_y[j] - ya) - bty;
-          const double dz = (s_z[j] - za) - btz;
-          const double d2 = dx * dx + dy * dy + dz * dz;
-          hit = d2 <= A.rc2;   // user_func.f90:52
-        } else {
-          const float4 p = s_p4[j];
-          const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
-          const float d2 =
-              __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-          hit = d2 <= A.rc2f;
+    const int r0 = kGroupStart[NG][grp], r1 = kGroupStart[NG][grp + 1];
+    for (int r = r0; r < r1; ++r) {
+      unsigned mask = run_mask_rt(r, okx, oky, okz);
+      const int jy = iy + kRunDY[r], jz = iz + kRunDZ[r];
+      const int row0 = RX * (jy + RY * jz) + ix - 2;
+      double bty = 0.0, btz = 0.0;
+      if constexpr (FP64) {
+        if (wraps) {
+          const int sjy = (wly && jy == 0) ? -1 : ((why && jy == RY - 1) ? 1 : 0);
+          const int sjz = (wlz && jz == 0) ? -1 : ((whz && jz == RZ - 1) ? 1 : 0);
+          bty = (double)(say - sjy) * g.L[1];
+          btz = (double)(saz - sjz) * g.L[2];
         }
       }
-      // append hits to the ring
-      const unsigned long long m = __ballot(hit);
-      if (hit) {
-        const int pos = qtail + (int)__builtin_amdgcn_mbcnt_hi(
-                                    (unsigned)(m >> 32),
-                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        w_q[pos & (kQueue - 1)] = (unsigned)ia | ((unsigned)j << 16);
-      }
-      qtail += __popcll(m);
-      if (qtail - qhead >= 64) {
-        wave_sync();
-        const unsigned e = w_q[(qhead + lane) & (kQueue - 1)];
-        eval_pair((int)(e & 0xffff), (int)(e >> 16));
-        qhead += 64;
-      }
-      // advance to the next candidate
-      if (act && f0 + it + 1 < f1) {
-        if (++j == jend) {
-          const int lprev = l;
-          if (++k == w_nun[l]) {
-            k = 0;
-            do {
-              ++l;
-            } while (w_nun[l] == 0);
-          }
-          const unsigned u = w_units[l * kUnitsPerAtom + k];
-          j = (int)(u & 0xffff);
-          jend = (int)(u >> 16);
-          if (l != lprev) {
-            ia = base + l;
-            load_a();
+      while (mask) {
+        const int lo = __builtin_ctz(mask);
+        const int hi = lo + __builtin_ctz(~(mask >> lo)) - 1;
+        mask &= ~((2u << hi) - 1u);
+        // self run, dx = 0: the block's own triangle, j > a
+        const int j0 = (r == 0 && lo == 2) ? a + 1 : s_off[row0 + lo];
+        const int j1 = s_off[row0 + hi + 1];
+        cand += (unsigned long long)(j1 > j0 ? j1 - j0 : 0);
+        for (int j = j0; j < j1; j += 4) {
+          if constexpr (FP64) {
+            double btx0 = 0.0, btx1 = 0.0, btx2 = 0.0, btx3 = 0.0;
+            if (wraps) {
+              auto btx_of = [&](int jj) {
+                const int cj = bc_x(s_bc[jj]);
+                const int sjx = (wlx && cj == 0) ? -1 : ((whx && cj == RX - 1) ? 1 : 0);
+                return (double)(sax - sjx) * g.L[0];
+              };
+              btx0 = btx_of(j);
+              btx1 = btx_of(j + 1);
+              btx2 = btx_of(j + 2);
+              btx3 = btx_of(j + 3);
+            }
+            // SRC/exaMD_forces.f90:509-523 with Nint(dx*globalboxi) = bt
+            const double dx0 = (s_x[j] - xa) - btx0, dy0 = (s_y[j] - ya) - bty,
+                         dz0 = (s_z[j] - za) - btz;
+            const double dx1 = (s_x[j + 1] - xa) - btx1,
+                         dy1 = (s_y[j + 1] - ya) - bty,
+                         dz1 = (s_z[j + 1] - za) - btz;
+            const double dx2 = (s_x[j + 2] - xa) - btx2,
+                         dy2 = (s_y[j + 2] - ya) - bty,
+                         dz2 = (s_z[j + 2] - za) - btz;
+            const double dx3 = (s_x[j + 3] - xa) - btx3,
+                         dy3 = (s_y[j + 3] - ya) - bty,
+                         dz3 = (s_z[j + 3] - za) - btz;
+            const double e0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+            const double e1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+            const double e2 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
+            const double e3 = dx3 * dx3 + dy3 * dy3 + dz3 * dz3;
+            unsigned hits = ((j < j1 && e0 <= A.rc2) ? 1u : 0u) |
+                            ((j + 1 < j1 && e1 <= A.rc2) ? 2u : 0u) |
+                            ((j + 2 < j1 && e2 <= A.rc2) ? 4u : 0u) |
+                            ((j + 3 < j1 && e3 <= A.rc2) ? 8u : 0u);
+            while (hits) {
+              const int k = __builtin_ctz(hits);
+              hits &= hits - 1;
+              const double dx = k == 0 ? dx0 : k == 1 ? dx1 : k == 2 ? dx2 : dx3;
+              const double dy = k == 0 ? dy0 : k == 1 ? dy1 : k == 2 ? dy2 : dy3;
+              const double dz = k == 0 ? dz0 : k == 1 ? dz1 : k == 2 ? dz2 : dz3;
+              const double d2 = k == 0 ? e0 : k == 1 ? e1 : k == 2 ? e2 : e3;
+              pair64(j + k, dx, dy, dz, d2);
+              home_count(j + k);
+              ++cut;
+            }
+          } else {
+            const float4 p0 = s_p4[j], p1 = s_p4[j + 1], p2 = s_p4[j + 2],
+                         p3 = s_p4[j + 3];
+            const float dx0 = p0.x - xf, dy0 = p0.y - yf, dz0 = p0.z - zf;
+            const float dx1 = p1.x - xf, dy1 = p1.y - yf, dz1 = p1.z - zf;
+            const float dx2 = p2.x - xf, dy2 = p2.y - yf, dz2 = p2.z - zf;
+            const float dx3 = p3.x - xf, dy3 = p3.y - yf, dz3 = p3.z - zf;
+            const float e0 = __builtin_fmaf(dx0, dx0, __builtin_fmaf(dy0, dy0, dz0 * dz0));
+            const float e1 = __builtin_fmaf(dx1, dx1, __builtin_fmaf(dy1, dy1, dz1 * dz1));
+            const float e2 = __builtin_fmaf(dx2, dx2, __builtin_fmaf(dy2, dy2, dz2 * dz2));
+            const float e3 = __builtin_fmaf(dx3, dx3, __builtin_fmaf(dy3, dy3, dz3 * dz3));
+            unsigned hits = ((j < j1 && e0 <= A.rc2f) ? 1u : 0u) |
+                            ((j + 1 < j1 && e1 <= A.rc2f) ? 2u : 0u) |
+                            ((j + 2 < j1 && e2 <= A.rc2f) ? 4u : 0u) |
+                            ((j + 3 < j1 && e3 <= A.rc2f) ? 8u : 0u);
+            while (hits) {
+              const int k = __builtin_ctz(hits);
+              hits &= hits - 1;
+              const float dx = k == 0 ? dx0 : k == 1 ? dx1 : k == 2 ? dx2 : dx3;
+              const float dy = k == 0 ? dy0 : k == 1 ? dy1 : k == 2 ? dy2 : dy3;
+              const float dz = k == 0 ? dz0 : k == 1 ? dz1 : k == 2 ? dz2 : dz3;
+              const float d2 = k == 0 ? e0 : k == 1 ? e1 : k == 2 ? e2 : e3;
+              pair32(j + k, dx, dy, dz, d2);
+              home_count(j + k);
+              ++cut;
+            }
           }
         }
       }
     }
-    // drain the rest of the ring
-    wave_sync();
-    if (lane < qtail - qhead) {
-      const unsigned e = w_q[(qhead + lane) & (kQueue - 1)];
-      eval_pair((int)(e & 0xffff), (int)(e >> 16));
+    if constexpr (FP64) {
+      lds_add(&s_fx[a], fax);
+      lds_add(&s_fy[a], fay);
+      lds_add(&s_fz[a], faz);
+    } else {
+      lds_add(&s_qx[a], qax);
+      lds_add(&s_qy[a], qay);
+      lds_add(&s_qz[a], qaz);
     }
-    wave_sync();
   }
   __syncthreads();
+  if (prof && tid == 0) prof[3] = clock64();
 
   // 4. return forces of home and ghost atoms (the force leg of the halo
   //    exchange, SRC/exaMD_comms.f90:263-376) with global atomics.
@@ -845,6 +827,7 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
     else mine.pe_fix = (unsigned long long)pefix;
   }
   write_partial<NT>(mine, A.d.part);
+  if (prof && tid == 0) prof[4] = clock64();
 }
 
 __global__ void k_forces_to_fp64(DevState d, double *out, bool mixed) {
@@ -954,6 +937,7 @@ void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
   A.log = log;
   A.cap = c.cap;
   A.maxrb = c.maxrb;
+  A.groups = c.groups < 1 ? 1 : (c.groups > 4 ? 4 : c.groups);
   A.rc2 = p.rc * p.rc;
   A.eps24 = 24.0 * p.eps;
   A.eps4 = 4.0 * p.eps;

@@ -22,7 +22,10 @@ struct DevState {
   StepLog *log = nullptr;  // per-step tallies
   unsigned long long *home = nullptr;  // home-block pair counts (diagnostic)
   Partial *part = nullptr; // per-workgroup partial tallies
+  unsigned long long *prof = nullptr;  // phase timestamps (diagnostic)
 };
+
+constexpr int kProfSlots = 5;   // timestamps per cell block
 
 // Partial slots needed by a launch sequence over n atoms and G tiles.
 int64_t partial_slots(int64_t n, int64_t tiles);
@@ -36,6 +39,7 @@ struct ForceConfig {
   int threads = 256;  // workgroup size: 256, 512 or 1024
   int cap = 0;        // LDS atom capacity per cell block
   int maxrb = 0;      // largest region (home + shell) in blocks
+  int groups = 1;     // run groups per atom (work items = groups x atoms)
 };
 
 size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads);

@@ -30,8 +30,17 @@ Module exaMD_hip_binding
      Integer(c_int32_t) :: device
      Integer(c_int32_t) :: block_threads
      Integer(c_int32_t) :: cap_slack_pct
-     Integer(c_int32_t) :: reserved(4)
+     Integer(c_int32_t) :: run_groups
+     Integer(c_int32_t) :: reserved(3)
   End Type examd_params
+
+  Type, Bind(C) :: examd_tune
+     Integer(c_int32_t) :: tiles(3)
+     Integer(c_int32_t) :: block_threads
+     Integer(c_int32_t) :: run_groups
+     Integer(c_int32_t) :: reserved
+     Real(c_double)     :: force_ms
+  End Type examd_tune
 
   Type, Bind(C) :: examd_stats
      Integer(c_int64_t) :: step
@@ -139,6 +148,38 @@ Module exaMD_hip_binding
        Real(c_double), Intent(out) :: pos(*), vel(*)
        Real(c_double), Intent(out) :: box(3)
      End Function examd_hip_generate_fcc
+
+     Integer(c_int) Function examd_hip_phase_profile(handle, precision, &
+          phase_cycles) Bind(C, name="examd_hip_phase_profile")
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: handle
+       Integer(c_int), Value :: precision
+       Real(c_double), Intent(out) :: phase_cycles(5)
+     End Function examd_hip_phase_profile
+
+     Integer(c_int) Function examd_hip_autotune(handle, precision, trials, &
+          best) Bind(C, name="examd_hip_autotune")
+       Import :: c_int, c_ptr, examd_tune
+       Type(c_ptr), Value :: handle
+       Integer(c_int), Value :: precision, trials
+       Type(examd_tune), Intent(out) :: best
+     End Function examd_hip_autotune
+
+     Integer(c_int) Function examd_hip_get_config(handle, precision, out) &
+          Bind(C, name="examd_hip_get_config")
+       Import :: c_int, c_ptr, examd_tune
+       Type(c_ptr), Value :: handle
+       Integer(c_int), Value :: precision
+       Type(examd_tune), Intent(out) :: out
+     End Function examd_hip_get_config
+
+     Integer(c_int) Function examd_hip_set_config(handle, precision, cfg) &
+          Bind(C, name="examd_hip_set_config")
+       Import :: c_int, c_ptr, examd_tune
+       Type(c_ptr), Value :: handle
+       Integer(c_int), Value :: precision
+       Type(examd_tune), Intent(in) :: cfg
+     End Function examd_hip_set_config
 
      Type(c_ptr) Function examd_hip_strerror(status) &
           Bind(C, name="examd_hip_strerror")

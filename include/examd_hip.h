@@ -67,7 +67,9 @@ typedef struct examd_params {
                              256, 512 or 1024) */
     int32_t cap_slack_pct;/* LDS atom capacity of a cell block above the
                              largest region at set-up, in percent (0 = 15) */
-    int32_t reserved[4];
+    int32_t run_groups;   /* work items per region atom in the pair loop
+                             (1-4; 0 = choose) */
+    int32_t reserved[3];
 } examd_params;
 
 /* Tallies of one step or one force evaluation. */
@@ -139,11 +141,42 @@ int examd_hip_block_ids(examd_handle *h, int64_t *blocknum, int32_t *blockxyz,
 int examd_hip_home_pair_counts(examd_handle *h, int precision,
                                int64_t *counts, int64_t nblocks);
 
+/* Launch shape of the force kernel for one precision: the cell-block map
+ * (tiles per dimension: the METIS partition of
+ * SRC/exaMD_partitioning.f90:65-455 becomes this map of cell blocks to
+ * workgroups), the workgroup size and the run groups per atom. */
+typedef struct examd_tune {
+    int32_t tiles[3];
+    int32_t block_threads;
+    int32_t run_groups;
+    int32_t reserved;
+    double force_ms;      /* mean force-kernel time measured by autotune */
+} examd_tune;
+
+/* Autotuner: time `trials` force evaluations of the current state for each
+ * candidate launch shape, keep the fastest for `precision` and report it in
+ * best (may be NULL).  The state is unchanged; forces must be recomputed. */
+int examd_hip_autotune(examd_handle *h, int precision, int trials,
+                       examd_tune *best);
+
+/* Read or apply the launch shape of one precision (apply: e.g. a shape the
+ * autotuner found earlier for the same system). */
+int examd_hip_get_config(examd_handle *h, int precision, examd_tune *out);
+int examd_hip_set_config(examd_handle *h, int precision,
+                         const examd_tune *cfg);
+
 /* Force-kernel timing over the calls since the last reset: enable, read
  * (total milliseconds of force-kernel launches measured with HIP events on
  * the handle's stream, and the number of launches), reset. */
 int examd_hip_timing(examd_handle *h, int enable, double *force_ms,
                      int64_t *force_launches, int reset);
+
+/* Diagnostic: one force evaluation with per-cell-block timestamps.
+ * phase_cycles[0..3] receive the mean shader-clock cycles per cell block of
+ * (0) region scan, (1) ghost gather into LDS, (2) pair evaluation, (3)
+ * force return; phase_cycles[4] the mean total per cell block. */
+int examd_hip_phase_profile(examd_handle *h, int precision,
+                            double phase_cycles[5]);
 
 /* Seeded initial states (not in the reference, whose generator uses an
  * unseeded RNG, SRC/exaMD_input_formats.f90:41-96).
