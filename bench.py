@@ -65,12 +65,13 @@ def read_traffic():
 
 
 def parse_shape(text):
-    """'Tx,Ty,Tz,threads,groups' -> dict, or None."""
+    """'Tx,Ty,Tz,threads[,groups[,lane_queue]]' -> dict, or None."""
     if not text:
         return None
     v = [int(x) for x in text.split(",")]
     return {"tiles": tuple(v[:3]), "block_threads": v[3],
-            "run_groups": v[4] if len(v) > 4 else 0}
+            "run_groups": v[4] if len(v) > 4 else 0,
+            "lane_queue": v[5] if len(v) > 5 else 0}
 
 
 def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
@@ -83,7 +84,7 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
             tune = m.autotune(prec, trials=3)
         else:
             m.set_config(prec, shape["tiles"], shape["block_threads"],
-                         shape["run_groups"])
+                         shape["run_groups"], shape["lane_queue"])
             tune = m.get_config(prec)
         if warmup:
             m.run(warmup, DT, prec, energy_every=0)
@@ -120,7 +121,7 @@ def pmc_probe(E, shape):
             print("tune", m.autotune("mixed", trials=2), flush=True)
         else:
             m.set_config("mixed", shape["tiles"], shape["block_threads"],
-                         shape["run_groups"])
+                         shape["run_groups"], shape["lane_queue"])
         for _ in range(3):
             m.forces("mixed")
     hip = ctypes.CDLL("libamdhip64.so")
@@ -149,9 +150,11 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short counter-collection program instead")
     ap.add_argument("--mixed-shape", default=None,
-                    help="Tx,Ty,Tz,threads,groups instead of autotuning")
+                    help="Tx,Ty,Tz,threads,groups,lane_queue instead of "
+                         "autotuning")
     ap.add_argument("--fp64-shape", default=None,
-                    help="Tx,Ty,Tz,threads,groups instead of autotuning")
+                    help="Tx,Ty,Tz,threads,groups,lane_queue instead of "
+                         "autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the "

@@ -64,12 +64,13 @@ class Params(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("tiles", C.c_int32 * 3), ("block_threads", C.c_int32),
-                ("run_groups", C.c_int32), ("reserved", C.c_int32),
+                ("run_groups", C.c_int32), ("lane_queue", C.c_int32),
                 ("force_ms", C.c_double)]
 
     def as_dict(self):
         return {"tiles": tuple(self.tiles), "block_threads": self.block_threads,
-                "run_groups": self.run_groups, "force_ms": self.force_ms}
+                "run_groups": self.run_groups, "lane_queue": self.lane_queue,
+                "force_ms": self.force_ms}
 
 
 class Stats(C.Structure):
@@ -272,11 +273,13 @@ class MidpointLJ:
                                           C.byref(t)), "examd_hip_get_config")
         return t.as_dict()
 
-    def set_config(self, precision, tiles, block_threads, run_groups=0):
+    def set_config(self, precision, tiles, block_threads, run_groups=0,
+                   lane_queue=0):
         t = Tune()
         t.tiles = (C.c_int32 * 3)(*[int(v) for v in tiles])
         t.block_threads = int(block_threads)
         t.run_groups = int(run_groups)
+        t.lane_queue = int(lane_queue)
         _check(lib().examd_hip_set_config(self._h, _PREC[precision],
                                           C.byref(t)), "examd_hip_set_config")
 

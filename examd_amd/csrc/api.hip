@@ -20,6 +20,7 @@ struct TileConfig {
   int T[3] = {1, 1, 1};   // cell blocks per dimension
   int threads = 256;      // workgroup size
   int groups = 1;         // run groups per region atom
+  int lane_queue = 0;     // per-lane queue depth of deferred pairs (0 = off)
   int cap = 0;            // LDS atom capacity
   int maxrb = 0;          // largest region in blocks
   bool fixed_tiles = false;
@@ -203,9 +204,9 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
     int64_t need = (int64_t)std::ceil((double)mra * 1.05) + 16;
     int64_t cap = want;
     while (cap > need &&
-           force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads) > h->lds_limit)
+           force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit)
       cap -= 16;
-    if (force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads) > h->lds_limit ||
+    if (force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit ||
         cap > 65535)
       ok = false;
     if (ok) {
@@ -242,6 +243,7 @@ ForceConfig force_config(const examd_handle *h, int prec) {
   c.cap = h->tc[prec].cap;
   c.maxrb = h->tc[prec].maxrb;
   c.groups = h->tc[prec].groups;
+  c.lane_queue = h->tc[prec].lane_queue;
   return c;
 }
 
@@ -687,9 +689,10 @@ int download_positions(examd_handle *h, std::vector<double> &aos) {
 
 // Apply a launch shape to one precision (cell-block map, threads, groups).
 int apply_config(examd_handle *h, int prec, const int T[3], int threads,
-                 int groups, const double *pos) {
+                 int groups, int lane_queue, const double *pos) {
   TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
+  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 8 ? 8 : 16);
   for (int a = 0; a < 3; ++a) c.T[a] = std::max(1, std::min(T[a], h->g.nb[a]));
   c.threads = threads;
   c.fixed_tiles = true;
@@ -713,7 +716,7 @@ int examd_hip_get_config(examd_handle *h, int precision, examd_tune *out) {
   for (int a = 0; a < 3; ++a) out->tiles[a] = c.T[a];
   out->block_threads = c.threads;
   out->run_groups = c.groups;
-  out->reserved = 0;
+  out->lane_queue = c.lane_queue;
   out->force_ms = 0.0;
   return EXAMD_OK;
 }
@@ -727,6 +730,7 @@ int examd_hip_set_config(examd_handle *h, int precision,
   const int thr = cfg->block_threads;
   if (thr != 256 && thr != 512 && thr != 1024) return EXAMD_ERR_ARG;
   if (cfg->run_groups < 0 || cfg->run_groups > 4) return EXAMD_ERR_ARG;
+  if (cfg->lane_queue < 0 || cfg->lane_queue > 16) return EXAMD_ERR_ARG;
   for (int a = 0; a < 3; ++a)
     if (cfg->tiles[a] <= 0) return EXAMD_ERR_ARG;
   HIPCHK(hipSetDevice(h->dev));
@@ -734,7 +738,7 @@ int examd_hip_set_config(examd_handle *h, int precision,
   int st = download_positions(h, pos);
   if (st) return st;
   return apply_config(h, precision, cfg->tiles, thr, cfg->run_groups,
-                      pos.data());
+                      cfg->lane_queue, pos.data());
 }
 
 int examd_hip_autotune(examd_handle *h, int precision, int trials,
@@ -758,12 +762,13 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
   double best_ms = 1e300;
   for (double sz : sizes)
     for (int thr : threads)
+      for (int lq = 0; lq <= 16; lq += 8)
       for (int grp = 0; grp <= (thr == 1024 ? 3 : 1); ++grp) {
         if (grp == 1) continue;  // 0 = automatic choice covers one group
         int T[3];
         for (int a = 0; a < 3; ++a)
           T[a] = std::max(1, (int)std::lround(h->g.nb[a] / sz));
-        if (apply_config(h, precision, T, thr, grp, pos.data()) != EXAMD_OK)
+        if (apply_config(h, precision, T, thr, grp, lq, pos.data()) != EXAMD_OK)
           continue;
         launch_fill_log(h->log, 1, h->stream);
         evaluate(h, precision, false, false, h->log);  // warm-up
@@ -801,7 +806,7 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
     return EXAMD_ERR_TILE_OVERFLOW;
   }
   st = apply_config(h, precision, bestc.T, bestc.threads, bestc.fixed_groups,
-                    pos.data());
+                    bestc.lane_queue, pos.data());
   h->forces_valid = -1;
   if (st) return st;
   if (best) {

@@ -333,7 +333,7 @@ struct ForceArgs {
   DevState d;
   Grid g;
   StepLog *log;
-  int cap, maxrb, groups;
+  int cap, maxrb, groups, lane_queue;
   double rc2, eps24, eps4, sig2, d2zero, d2fix;
   float rc2f, eps24f, eps4f, sig2f, d2zerof, d2fixf;
 };
@@ -345,11 +345,14 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
 // LDS image of one cell block: region-block offsets, then atoms.  The atom
 // arrays hold cap + 4 entries: the pair loop reads four candidates at a
 // time and may look up to three entries past the last atom (masked off).
+constexpr int kWorkBins = 64;   // owned J blocks of an atom: 0..63
+constexpr int kMaxLaneQueue = 16;  // deepest per-lane queue of deferred pairs
 struct LdsLayout {
-  size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, total;
+  size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_perm, off_hist,
+      off_q, total;
 };
 __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
-                                                int nwaves) {
+                                                int nwaves, int qdepth) {
   cap += 4;
   LdsLayout L;
   size_t o = (size_t)(maxrb + 1) * 4;  // s_off
@@ -367,8 +370,16 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   o += (size_t)cap * 4;
   L.off_bc = o;
   if (fp64) o += (size_t)cap * 4;
+  o = align_up(o, 4);
+  L.off_perm = o;
+  o += (size_t)cap * 2;
+  o = align_up(o, 4);
+  L.off_hist = o;
+  o += (kWorkBins + 1) * 4;
+  o = align_up(o, 16);
+  L.off_q = o;
+  o += (size_t)nwaves * 64 * qdepth * 2;
   L.total = align_up(o, 16);
-  (void)nwaves;
   return L;
 }
 
@@ -422,6 +433,32 @@ __device__ __forceinline__ unsigned run_mask_rt(int r, unsigned ox,
   return axis_sel(kRunMasks.m[r][0], ox) & axis_sel(kRunMasks.m[r][1], oy) &
          axis_sel(kRunMasks.m[r][2], oz);
 }
+
+// Owned J blocks of an atom as a function of its per-axis own_bits
+// (ox + 8 oy + 64 oz): the sum over the 13 runs of the popcount of the
+// ownership mask.  Used to sort a cell block's atoms by work.
+struct OwnedWork {
+  unsigned char v[512];
+  constexpr OwnedWork() : v() {
+    const RunAxisMasks M;
+    for (int i = 0; i < 512; ++i) {
+      unsigned ox = i & 7, oy = (i >> 3) & 7, oz = i >> 6;
+      int n = 0;
+      for (int r = 0; r < kNumRuns; ++r) {
+        unsigned mx = 0, my = 0, mz = 0;
+        for (int o = -1; o <= 1; ++o) {
+          if (ox & (1u << (o + 1))) mx |= M.m[r][0][o + 1];
+          if (oy & (1u << (o + 1))) my |= M.m[r][1][o + 1];
+          if (oz & (1u << (o + 1))) mz |= M.m[r][2][o + 1];
+        }
+        unsigned m = mx & my & mz;
+        for (int k = 0; k < 5; ++k) n += (m >> k) & 1u;
+      }
+      v[i] = (unsigned char)n;
+    }
+  }
+};
+__constant__ OwnedWork kOwnedWork;
 
 // Run groups: the 13 runs split into 1..4 groups of similar candidate
 // counts (run 0 covers 2.5 blocks, the others 5).
@@ -492,7 +529,7 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   const int RXY = RX * RY, NRB = RXY * RZ;
   const int nbx = g.nb[0], nby = g.nb[1], nbz = g.nb[2];
 
-  const LdsLayout LL = lds_layout(FP64, A.cap, A.maxrb, NW);
+  const LdsLayout LL = lds_layout(FP64, A.cap, A.maxrb, NW, A.lane_queue);
   int *s_off = reinterpret_cast<int *>(smem);
   int *s_gst = reinterpret_cast<int *>(smem + LL.off_gst);
   int *s_wave = reinterpret_cast<int *>(smem + LL.off_wave);
@@ -606,19 +643,61 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   __syncthreads();
   if (prof && tid == 0) prof[2] = clock64();
 
-  // 3. pairs.  Work item w = (run group g, region atom a): the lane lists
-  //    the J ranges atom a owns in the group's runs (home block I - o1
-  //    inside the tile) and evaluates them four candidates at a time.  The
-  //    force on a accumulates in registers, the force on j with LDS atomics.
+  // 3a. order the region atoms by the number of J blocks they own, most
+  //     first, so that the lanes of a wave carry similar work; atoms that
+  //     own nothing (outer ghost layers) drop out.
+  unsigned short *s_perm = reinterpret_cast<unsigned short *>(smem + LL.off_perm);
+  int *s_hist = reinterpret_cast<int *>(smem + LL.off_hist);
+  for (int b = tid; b <= kWorkBins; b += NT) s_hist[b] = 0;
+  __syncthreads();
+  auto work_of = [&](int a) {
+    const int bc = FP64 ? s_bc[a] : __float_as_int(s_p4[a].w);
+    const unsigned key = own_bits(bc_x(bc), tx) | (own_bits(bc_y(bc), ty) << 3) |
+                         (own_bits(bc_z(bc), tz) << 6);
+    return (int)kOwnedWork.v[key];
+  };
+  for (int a = tid; a < R; a += NT) {
+    const int w = work_of(a);
+    if (w > 0) atomicAdd(&s_hist[kWorkBins - 1 - w], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int b = 0; b < kWorkBins; ++b) {
+      const int c = s_hist[b];
+      s_hist[b] = acc;
+      acc += c;
+    }
+    s_hist[kWorkBins] = acc;
+  }
+  __syncthreads();
+  const int RW = s_hist[kWorkBins];   // atoms that own at least one block
+  for (int a = tid; a < R; a += NT) {
+    const int w = work_of(a);
+    if (w > 0) s_perm[atomicAdd(&s_hist[kWorkBins - 1 - w], 1)] = (unsigned short)a;
+  }
+  __syncthreads();
+
+  // 3. pairs.  Work item w = (run group g, atom perm[i]): the lane lists the
+  //    J ranges atom a owns in the group's runs (home block I - o1 inside
+  //    the tile) and tests them four candidates at a time.  In-cutoff pairs
+  //    are either evaluated at once or, with lane queues, deferred into a
+  //    per-lane LDS queue that the wave empties together, one pair per lane
+  //    per step.  The force on a accumulates in registers, the force on j
+  //    with LDS atomics.
   unsigned long long cand = 0, cut = 0;
   unsigned flags = 0;
   double pe64 = 0.0;
   long long pefix = 0;
   const int NG = A.groups;
-  const int W = NG * R;
+  const int W = NG * RW;
+  const int QD = A.lane_queue;   // queue depth, 0 = evaluate at once
+  const bool LQ = QD > 0;
+  unsigned short *my_q = reinterpret_cast<unsigned short *>(smem + LL.off_q) +
+                         (size_t)(tid >> 6) * 64 * QD + (tid & 63);
   for (int w = tid; w < W; w += NT) {
-    const int grp = w / R;
-    const int a = w - grp * R;
+    const int grp = w / RW;
+    const int a = (int)s_perm[w - grp * RW];
     int bc;
     double xa = 0.0, ya = 0.0, za = 0.0;
     float xf = 0.f, yf = 0.f, zf = 0.f;
@@ -640,6 +719,7 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
     const int sax = bc_s(bc, 0), say = bc_s(bc, 1), saz = bc_s(bc, 2);
     double fax = 0.0, fay = 0.0, faz = 0.0;
     long long qax = 0, qay = 0, qaz = 0;
+    int qn = 0;   // queued pairs of this lane
 
     // in-cutoff pair (a, j), table orientation: +f on a, -f on j
     // (SRC/exaMD_forces.f90:538-542)
@@ -690,6 +770,35 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
         atomicAdd(&A.d.home[hx + nbx * (hy + nby * hz)], 1ull);
       }
     };
+    // evaluate a queued pair (a, j) from positions
+    auto eval_queued = [&](int j) {
+      if constexpr (FP64) {
+        double btx = 0.0, bty = 0.0, btz = 0.0;
+        if (wraps) {
+          const int bj = s_bc[j];
+          btx = (double)(sax - bc_s(bj, 0)) * g.L[0];
+          bty = (double)(say - bc_s(bj, 1)) * g.L[1];
+          btz = (double)(saz - bc_s(bj, 2)) * g.L[2];
+        }
+        const double dx = (s_x[j] - xa) - btx;
+        const double dy = (s_y[j] - ya) - bty;
+        const double dz = (s_z[j] - za) - btz;
+        pair64(j, dx, dy, dz, dx * dx + dy * dy + dz * dz);
+      } else {
+        const float4 p = s_p4[j];
+        const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
+        pair32(j, dx, dy, dz,
+               __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+      }
+      home_count(j);
+      ++cut;
+    };
+    // empty the queues of the active lanes together, one pair per step
+    auto drain = [&]() {
+      for (int e = 0; __ballot(e < qn); ++e)
+        if (e < qn) eval_queued((int)my_q[e * 64]);
+      qn = 0;
+    };
 
     const int r0 = kGroupStart[NG][grp], r1 = kGroupStart[NG][grp + 1];
     for (int r = r0; r < r1; ++r) {
@@ -714,6 +823,9 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
         const int j1 = s_off[row0 + hi + 1];
         cand += (unsigned long long)(j1 > j0 ? j1 - j0 : 0);
         for (int j = j0; j < j1; j += 4) {
+          unsigned hits;
+          double ddx[4], ddy[4], ddz[4], dd2[4];
+          float fdx[4], fdy[4], fdz[4], fd2[4];
           if constexpr (FP64) {
             double btx0 = 0.0, btx1 = 0.0, btx2 = 0.0, btx3 = 0.0;
             if (wraps) {
@@ -728,66 +840,70 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
               btx3 = btx_of(j + 3);
             }
             // SRC/exaMD_forces.f90:509-523 with Nint(dx*globalboxi) = bt
-            const double dx0 = (s_x[j] - xa) - btx0, dy0 = (s_y[j] - ya) - bty,
-                         dz0 = (s_z[j] - za) - btz;
-            const double dx1 = (s_x[j + 1] - xa) - btx1,
-                         dy1 = (s_y[j + 1] - ya) - bty,
-                         dz1 = (s_z[j + 1] - za) - btz;
-            const double dx2 = (s_x[j + 2] - xa) - btx2,
-                         dy2 = (s_y[j + 2] - ya) - bty,
-                         dz2 = (s_z[j + 2] - za) - btz;
-            const double dx3 = (s_x[j + 3] - xa) - btx3,
-                         dy3 = (s_y[j + 3] - ya) - bty,
-                         dz3 = (s_z[j + 3] - za) - btz;
-            const double e0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-            const double e1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-            const double e2 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
-            const double e3 = dx3 * dx3 + dy3 * dy3 + dz3 * dz3;
-            unsigned hits = ((j < j1 && e0 <= A.rc2) ? 1u : 0u) |
-                            ((j + 1 < j1 && e1 <= A.rc2) ? 2u : 0u) |
-                            ((j + 2 < j1 && e2 <= A.rc2) ? 4u : 0u) |
-                            ((j + 3 < j1 && e3 <= A.rc2) ? 8u : 0u);
-            while (hits) {
-              const int k = __builtin_ctz(hits);
-              hits &= hits - 1;
-              const double dx = k == 0 ? dx0 : k == 1 ? dx1 : k == 2 ? dx2 : dx3;
-              const double dy = k == 0 ? dy0 : k == 1 ? dy1 : k == 2 ? dy2 : dy3;
-              const double dz = k == 0 ? dz0 : k == 1 ? dz1 : k == 2 ? dz2 : dz3;
-              const double d2 = k == 0 ? e0 : k == 1 ? e1 : k == 2 ? e2 : e3;
-              pair64(j + k, dx, dy, dz, d2);
-              home_count(j + k);
-              ++cut;
+            ddx[0] = (s_x[j] - xa) - btx0; ddy[0] = (s_y[j] - ya) - bty; ddz[0] = (s_z[j] - za) - btz;
+            ddx[1] = (s_x[j + 1] - xa) - btx1; ddy[1] = (s_y[j + 1] - ya) - bty; ddz[1] = (s_z[j + 1] - za) - btz;
+            ddx[2] = (s_x[j + 2] - xa) - btx2; ddy[2] = (s_y[j + 2] - ya) - bty; ddz[2] = (s_z[j + 2] - za) - btz;
+            ddx[3] = (s_x[j + 3] - xa) - btx3; ddy[3] = (s_y[j + 3] - ya) - bty; ddz[3] = (s_z[j + 3] - za) - btz;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+              dd2[k] = ddx[k] * ddx[k] + ddy[k] * ddy[k] + ddz[k] * ddz[k];
+            hits = ((j < j1 && dd2[0] <= A.rc2) ? 1u : 0u) |
+                   ((j + 1 < j1 && dd2[1] <= A.rc2) ? 2u : 0u) |
+                   ((j + 2 < j1 && dd2[2] <= A.rc2) ? 4u : 0u) |
+                   ((j + 3 < j1 && dd2[3] <= A.rc2) ? 8u : 0u);
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const float4 p = s_p4[j + k];
+              fdx[k] = p.x - xf;
+              fdy[k] = p.y - yf;
+              fdz[k] = p.z - zf;
+              fd2[k] = __builtin_fmaf(fdx[k], fdx[k],
+                                      __builtin_fmaf(fdy[k], fdy[k], fdz[k] * fdz[k]));
+            }
+            hits = ((j < j1 && fd2[0] <= A.rc2f) ? 1u : 0u) |
+                   ((j + 1 < j1 && fd2[1] <= A.rc2f) ? 2u : 0u) |
+                   ((j + 2 < j1 && fd2[2] <= A.rc2f) ? 4u : 0u) |
+                   ((j + 3 < j1 && fd2[3] <= A.rc2f) ? 8u : 0u);
+          }
+          if (LQ) {
+            // defer: write the four candidates, keep the hits
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              my_q[qn * 64] = (unsigned short)(j + k);
+              qn += (hits >> k) & 1u;
+            }
+            if (qn > QD - 4) {   // rare: this lane's queue is full
+              for (int e = 0; e < qn; ++e) eval_queued((int)my_q[e * 64]);
+              qn = 0;
             }
           } else {
-            const float4 p0 = s_p4[j], p1 = s_p4[j + 1], p2 = s_p4[j + 2],
-                         p3 = s_p4[j + 3];
-            const float dx0 = p0.x - xf, dy0 = p0.y - yf, dz0 = p0.z - zf;
-            const float dx1 = p1.x - xf, dy1 = p1.y - yf, dz1 = p1.z - zf;
-            const float dx2 = p2.x - xf, dy2 = p2.y - yf, dz2 = p2.z - zf;
-            const float dx3 = p3.x - xf, dy3 = p3.y - yf, dz3 = p3.z - zf;
-            const float e0 = __builtin_fmaf(dx0, dx0, __builtin_fmaf(dy0, dy0, dz0 * dz0));
-            const float e1 = __builtin_fmaf(dx1, dx1, __builtin_fmaf(dy1, dy1, dz1 * dz1));
-            const float e2 = __builtin_fmaf(dx2, dx2, __builtin_fmaf(dy2, dy2, dz2 * dz2));
-            const float e3 = __builtin_fmaf(dx3, dx3, __builtin_fmaf(dy3, dy3, dz3 * dz3));
-            unsigned hits = ((j < j1 && e0 <= A.rc2f) ? 1u : 0u) |
-                            ((j + 1 < j1 && e1 <= A.rc2f) ? 2u : 0u) |
-                            ((j + 2 < j1 && e2 <= A.rc2f) ? 4u : 0u) |
-                            ((j + 3 < j1 && e3 <= A.rc2f) ? 8u : 0u);
             while (hits) {
               const int k = __builtin_ctz(hits);
               hits &= hits - 1;
-              const float dx = k == 0 ? dx0 : k == 1 ? dx1 : k == 2 ? dx2 : dx3;
-              const float dy = k == 0 ? dy0 : k == 1 ? dy1 : k == 2 ? dy2 : dy3;
-              const float dz = k == 0 ? dz0 : k == 1 ? dz1 : k == 2 ? dz2 : dz3;
-              const float d2 = k == 0 ? e0 : k == 1 ? e1 : k == 2 ? e2 : e3;
-              pair32(j + k, dx, dy, dz, d2);
+              if constexpr (FP64) {
+                const double dx = k == 0 ? ddx[0] : k == 1 ? ddx[1] : k == 2 ? ddx[2] : ddx[3];
+                const double dy = k == 0 ? ddy[0] : k == 1 ? ddy[1] : k == 2 ? ddy[2] : ddy[3];
+                const double dz = k == 0 ? ddz[0] : k == 1 ? ddz[1] : k == 2 ? ddz[2] : ddz[3];
+                const double d2 = k == 0 ? dd2[0] : k == 1 ? dd2[1] : k == 2 ? dd2[2] : dd2[3];
+                pair64(j + k, dx, dy, dz, d2);
+              } else {
+                const float dx = k == 0 ? fdx[0] : k == 1 ? fdx[1] : k == 2 ? fdx[2] : fdx[3];
+                const float dy = k == 0 ? fdy[0] : k == 1 ? fdy[1] : k == 2 ? fdy[2] : fdy[3];
+                const float dz = k == 0 ? fdz[0] : k == 1 ? fdz[1] : k == 2 ? fdz[2] : fdz[3];
+                const float d2 = k == 0 ? fd2[0] : k == 1 ? fd2[1] : k == 2 ? fd2[2] : fd2[3];
+                pair32(j + k, dx, dy, dz, d2);
+              }
               home_count(j + k);
               ++cut;
             }
           }
         }
       }
+      // runs end together: empty the queues when one is half full
+      if (LQ && __ballot(qn >= QD / 2)) drain();
     }
+    if (LQ) drain();
     if constexpr (FP64) {
       lds_add(&s_fx[a], fax);
       lds_add(&s_fy[a], fay);
@@ -864,8 +980,9 @@ void launch_forces_t(const ForceArgs &A, int threads, size_t lds, int G,
 
 }  // namespace
 
-size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads) {
-  return lds_layout(fp64, cap, maxrb, threads / 64).total;
+size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads,
+                       int qdepth) {
+  return lds_layout(fp64, cap, maxrb, threads / 64, qdepth).total;
 }
 
 void launch_fill_i32(int *p, int v, int64_t n, hipStream_t s) {
@@ -938,6 +1055,7 @@ void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
   A.cap = c.cap;
   A.maxrb = c.maxrb;
   A.groups = c.groups < 1 ? 1 : (c.groups > 4 ? 4 : c.groups);
+  A.lane_queue = c.lane_queue;
   A.rc2 = p.rc * p.rc;
   A.eps24 = 24.0 * p.eps;
   A.eps4 = 4.0 * p.eps;
@@ -956,7 +1074,7 @@ void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
   A.d2zerof = (float)A.d2zero;
   A.d2fixf = (float)A.d2fix;
   const bool fp64 = !mixed;
-  const size_t lds = force_lds_bytes(fp64, c.cap, c.maxrb, c.threads);
+  const size_t lds = force_lds_bytes(fp64, c.cap, c.maxrb, c.threads, c.lane_queue);
   const int G = g.T[0] * g.T[1] * g.T[2];
 #define EXAMD_F(F64, E, H) launch_forces_t<F64, E, H>(A, c.threads, lds, G, s)
   if (homecount) {

@@ -40,9 +40,11 @@ struct ForceConfig {
   int cap = 0;        // LDS atom capacity per cell block
   int maxrb = 0;      // largest region (home + shell) in blocks
   int groups = 1;     // run groups per atom (work items = groups x atoms)
+  int lane_queue = 0; // depth of per-lane queues of deferred pairs (0, 8, 16)
 };
 
-size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads);
+size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads,
+                       int qdepth);
 
 // Binning / integration modes of launch_step_bin.
 enum BinMode { kBinOnly = 0, kKickDriftBin = 1, kFinishKickDriftBin = 2 };

@@ -38,7 +38,7 @@ Module exaMD_hip_binding
      Integer(c_int32_t) :: tiles(3)
      Integer(c_int32_t) :: block_threads
      Integer(c_int32_t) :: run_groups
-     Integer(c_int32_t) :: reserved
+     Integer(c_int32_t) :: lane_queue
      Real(c_double)     :: force_ms
   End Type examd_tune
 

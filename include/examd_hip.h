@@ -149,7 +149,8 @@ typedef struct examd_tune {
     int32_t tiles[3];
     int32_t block_threads;
     int32_t run_groups;
-    int32_t reserved;
+    int32_t lane_queue;   /* depth of per-lane queues of deferred in-cutoff
+                             pairs: 0 (evaluate at once), 8 or 16 */
     double force_ms;      /* mean force-kernel time measured by autotune */
 } examd_tune;
 

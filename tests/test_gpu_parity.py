@@ -207,3 +207,26 @@ def test_small_grid_rejected(E):
         E.MidpointLJ(10, [11.0, 11.0, 11.0], 2.5)     # 8 blocks: fine
         E.MidpointLJ(10, [6.0, 6.0, 6.0], 2.5)        # 4 blocks: rejected
     assert ei.value.status == E.ERR_GRID
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lane_queue_variant_matches(E, oracle, prec):
+    """The deferred (lane-queue) pair evaluation gives the oracle's forces
+    and pair counts, and the mixed path stays bit-identical to the direct
+    evaluation at the same cell-block map."""
+    print(f"[lane queue] {prec}", flush=True)
+    pos, box = _case(10, 0.1, 12)
+    ref, t, hc = oracle.forces(pos, box, 2.5, home_counts=True)
+    with _system(E, pos, box) as m:
+        cfg = m.get_config(prec)
+        m.set_config(prec, cfg["tiles"], cfg["block_threads"], 0, 0)
+        f0, s0, _ = m.forces(prec)
+        m.set_config(prec, cfg["tiles"], cfg["block_threads"], 0, 8)
+        f1, s1, _ = m.forces(prec)
+        got = m.home_pair_counts(prec)
+    assert (s1.pairs_cand, s1.pairs_cut) == (s0.pairs_cand, s0.pairs_cut)
+    if prec == "fp64":
+        assert _rel(f1, ref) < FP64_TOL
+        np.testing.assert_array_equal(got, hc)
+    else:
+        assert np.array_equal(f1, f0)
