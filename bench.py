@@ -75,10 +75,11 @@ def parse_shape(text):
 
 
 def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
-              shape=None):
+              shape=None, graph_chunk=100):
     """Autotune the launch shape for this precision on the initial state
-    (or apply the given shape), then warm up and time `steps` MD steps."""
-    with E.MidpointLJ(len(pos), box, RC) as m:
+    (or apply the given shape), then warm up and time `steps` MD steps,
+    replayed from captured hipGraph chunks of `graph_chunk` steps."""
+    with E.MidpointLJ(len(pos), box, RC, graph_chunk=graph_chunk) as m:
         m.set_state(pos, vel)
         if shape is None:
             tune = m.autotune(prec, trials=3)
@@ -93,6 +94,9 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
         stats = m.run(steps, DT, prec, energy_every=energy_every)
         t1 = time.perf_counter()
         force_ms, launches = m.timing(enable=False, reset=True)
+    if launches != steps:
+        raise RuntimeError(f"{prec}: {launches} timed force launches for "
+                           f"{steps} steps")
     return t1 - t0, stats, force_ms / max(launches, 1), launches, tune
 
 
@@ -147,6 +151,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph-chunk", type=int, default=100,
+                    help="MD steps per captured hipGraph chunk (0 = eager)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short counter-collection program instead")
     ap.add_argument("--mixed-shape", default=None,
@@ -174,10 +180,10 @@ def main():
 
     wall_mx, st_mx, force_ms_mx, launches, tune_mx = timed_run(
         E, pos, vel, box, "mixed", steps, args.warmup, energy_every,
-        parse_shape(args.mixed_shape))
+        parse_shape(args.mixed_shape), args.graph_chunk)
     wall_64, st_64, force_ms_64, _, tune_64 = timed_run(
         E, pos, vel, box, "fp64", fsteps, args.warmup, energy_every,
-        parse_shape(args.fp64_shape))
+        parse_shape(args.fp64_shape), args.graph_chunk)
 
     rate_mx = n * steps / wall_mx
     rate_64 = n * fsteps / wall_64
@@ -237,6 +243,7 @@ def main():
             "energy_drift_per_1k_fp64": drift_per_1k(st_64),
             "launch_mixed": tune_mx,
             "launch_fp64": tune_64,
+            "graph_chunk_steps": args.graph_chunk,
             "parallelism": "1 GPU",
         },
         "roofline": {

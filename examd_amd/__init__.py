@@ -43,7 +43,7 @@ EXPORTS = (
     "examd_hip_run", "examd_hip_block_ids", "examd_hip_home_pair_counts",
     "examd_hip_timing", "examd_hip_generate_fcc", "examd_hip_strerror",
     "examd_hip_phase_profile", "examd_hip_autotune", "examd_hip_get_config",
-    "examd_hip_set_config",
+    "examd_hip_set_config", "examd_hip_set_graph_chunk",
 )
 
 
@@ -59,7 +59,8 @@ class Params(C.Structure):
                 ("sigma", C.c_double), ("mass", C.c_double),
                 ("tiles", C.c_int32 * 3), ("device", C.c_int32),
                 ("block_threads", C.c_int32), ("cap_slack_pct", C.c_int32),
-                ("run_groups", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("run_groups", C.c_int32), ("graph_chunk", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class Tune(C.Structure):
@@ -174,7 +175,7 @@ class MidpointLJ:
 
     def __init__(self, natoms, box, cutoff, epsilon=1.0, sigma=1.0, mass=1.0,
                  tiles=(0, 0, 0), device=0, block_threads=0,
-                 cap_slack_pct=0, run_groups=0):
+                 cap_slack_pct=0, run_groups=0, graph_chunk=0):
         p = Params()
         p.natoms = int(natoms)
         p.box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
@@ -185,6 +186,7 @@ class MidpointLJ:
         p.block_threads = int(block_threads)
         p.cap_slack_pct = int(cap_slack_pct)
         p.run_groups = int(run_groups)
+        p.graph_chunk = int(graph_chunk)
         self.params = p
         self.n = int(natoms)
         self.box = np.array(np.reshape(box, 3), dtype=np.float64)
@@ -243,6 +245,11 @@ class MidpointLJ:
                                    float(dt), int(energy_every), out, cap,
                                    C.byref(nout)), "examd_hip_run")
         return [out[i] for i in range(nout.value)]
+
+    def set_graph_chunk(self, chunk):
+        """Steps per captured hipGraph chunk of run() (0 = launch eagerly)."""
+        _check(lib().examd_hip_set_graph_chunk(self._h, C.c_int64(int(chunk))),
+               "examd_hip_set_graph_chunk")
 
     def block_ids(self):
         num = np.zeros(self.n, dtype=np.int64)

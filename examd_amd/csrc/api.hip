@@ -27,6 +27,15 @@ struct TileConfig {
   int fixed_groups = 0;
 };
 
+// What a captured chunk depends on.
+struct GraphKey {
+  int prec = -1;
+  int64_t chunk = 0, every = 0;
+  double dt = 0.0;
+  bool timing = false;
+  TileConfig cfg;
+};
+
 struct examd_handle {
   examd_params p{};
   int dev = 0;
@@ -41,6 +50,12 @@ struct examd_handle {
   size_t lds_limit = 0;
   int64_t part_cap = 0;
   std::vector<double> host_pos;   // positions of the last set_state (AoS)
+  // captured multi-step chunk
+  int64_t graph_chunk = 0;        // steps per captured chunk (0 = eager)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  GraphKey graph_key{};
+  size_t graph_events = 0;        // timing events recorded inside the graph
   bool state_set = false;
   int forces_valid = -1;    // precision whose forces match the positions
   int last_prec = 0;
@@ -102,10 +117,13 @@ Grid grid_for(const examd_handle *h, int prec) {
   return g;
 }
 
+void drop_graph(examd_handle *h);
+
 // Partial-tally slots for the current tile maps.
 int ensure_partials(examd_handle *h) {
   int64_t need = partial_slots(h->n, std::max(tiles_of(h->tc[0]), tiles_of(h->tc[1])));
   if (h->part_cap >= need) return EXAMD_OK;
+  drop_graph(h);   // a captured chunk holds the old address
   if (h->d.part) (void)hipFree(h->d.part);
   h->d.part = nullptr;
   h->part_cap = 0;
@@ -116,6 +134,7 @@ int ensure_partials(examd_handle *h) {
 
 int ensure_log(examd_handle *h, int64_t n) {
   if (h->log_cap >= n) return EXAMD_OK;
+  drop_graph(h);   // a captured chunk holds the old address
   if (h->log) (void)hipFree(h->log);
   h->log = nullptr;
   h->log_cap = 0;
@@ -260,6 +279,28 @@ hipEvent_t next_event(examd_handle *h) {
   return h->ev_pool[h->ev_used++];
 }
 
+// Record an event on the handle's stream.  Under stream capture HIP does
+// not turn hipEventRecord into a graph node, so the node is added to the
+// capture graph explicitly and made the stream's new dependency.
+void record_event(examd_handle *h, hipEvent_t e) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusActive) {
+    (void)hipEventRecord(e, h->stream);
+    return;
+  }
+  hipGraph_t g = nullptr;
+  const hipGraphNode_t *deps = nullptr;
+  size_t nd = 0;
+  if (hipStreamGetCaptureInfo_v2(h->stream, &cs, nullptr, &g, &deps, &nd) !=
+      hipSuccess || !g)
+    return;
+  hipGraphNode_t node = nullptr;
+  if (hipGraphAddEventRecordNode(&node, g, deps, nd, e) != hipSuccess) return;
+  (void)hipStreamUpdateCaptureDependencies(h->stream, &node, 1,
+                                           hipStreamSetCaptureDependencies);
+}
+
 // Force launch, bracketed by events when timing is on.
 void forces_timed(examd_handle *h, int prec, bool energy, bool homecount,
                   StepLog *log) {
@@ -267,11 +308,11 @@ void forces_timed(examd_handle *h, int prec, bool energy, bool homecount,
   if (h->timing) {
     e0 = next_event(h);
     e1 = next_event(h);
-    if (e0) (void)hipEventRecord(e0, h->stream);
+    if (e0) record_event(h, e0);
   }
   launch_forces(h->d, grid_for(h, prec), lj(h), force_config(h, prec),
                 prec == EXAMD_PREC_MIXED, energy, homecount, log, h->stream);
-  if (e1) (void)hipEventRecord(e1, h->stream);
+  if (e1) record_event(h, e1);
 }
 
 // Accumulate the recorded event pairs (stream already synchronised).
@@ -294,6 +335,79 @@ void evaluate(examd_handle *h, int prec, bool energy, bool homecount,
                   nullptr, log, h->stream);
   launch_sort(h->d, h->nblocks, h->stream);
   forces_timed(h, prec, energy, homecount, log);
+}
+
+// Enqueue `count` velocity-Verlet steps on the handle's stream, tallies of
+// chunk step c in log[c]: kick-drift-bin, sort, force; then
+// (finish-kick-drift-bin, sort, force) x (count - 1); then the closing kick.
+// Chunks of any length compose into the same arithmetic as one long run.
+template <class IsEnergy>
+void enqueue_steps(examd_handle *h, int prec, int64_t count, double dt,
+                   IsEnergy is_energy) {
+  const bool mixed = prec == EXAMD_PREC_MIXED;
+  const double hk = (0.5 * dt) / h->p.mass;
+  launch_fill_log(h->log + 1, (int)count, h->stream);
+  for (int64_t c = 1; c <= count; ++c) {
+    StepLog *ke_prev = (c > 1 && is_energy(c - 1)) ? h->log + (c - 1) : nullptr;
+    launch_step_bin(h->d, h->g, mixed, c == 1 ? kKickDriftBin : kFinishKickDriftBin,
+                    hk, dt, ke_prev, h->log + c, h->stream);
+    launch_sort(h->d, h->nblocks, h->stream);
+    forces_timed(h, prec, is_energy(c), false, h->log + c);
+  }
+  if (count > 0)
+    launch_kick(h->d, mixed, hk, is_energy(count) ? h->log + count : nullptr,
+                h->stream);
+}
+
+bool same_key(const GraphKey &a, const GraphKey &b) {
+  if (a.prec != b.prec || a.chunk != b.chunk || a.every != b.every ||
+      a.dt != b.dt || a.timing != b.timing)
+    return false;
+  const TileConfig &x = a.cfg, &y = b.cfg;
+  return x.T[0] == y.T[0] && x.T[1] == y.T[1] && x.T[2] == y.T[2] &&
+         x.threads == y.threads && x.groups == y.groups &&
+         x.lane_queue == y.lane_queue && x.cap == y.cap && x.maxrb == y.maxrb;
+}
+
+void drop_graph(examd_handle *h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+  h->graph_events = 0;
+}
+
+// Capture one chunk of `key.chunk` steps into an executable graph (reused
+// while the key matches).  Event records of the force timing are captured
+// too, so replays report each force launch.
+int ensure_graph(examd_handle *h, const GraphKey &key) {
+  if (h->graph_exec && same_key(h->graph_key, key)) return EXAMD_OK;
+  drop_graph(h);
+  h->ev_used = 0;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const int64_t every = key.every;
+  const int64_t chunk = key.chunk;
+  enqueue_steps(h, key.prec, chunk, key.dt, [&](int64_t c) {
+    return c == chunk || (every > 0 && c % every == 0);
+  });
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess || !g) {
+    (void)hipGetLastError();
+    h->ev_used = 0;
+    return EXAMD_ERR_HIP;
+  }
+  hipGraphExec_t ge = nullptr;
+  if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    h->ev_used = 0;
+    return EXAMD_ERR_HIP;
+  }
+  h->graph = g;
+  h->graph_exec = ge;
+  h->graph_key = key;
+  h->graph_events = h->ev_used;
+  return EXAMD_OK;
 }
 
 uint64_t splitmix64(uint64_t &s) {
@@ -391,6 +505,7 @@ int examd_hip_create(const examd_params *prm, examd_handle **out) {
     h->tc[0].T[a] = h->tc[1].T[a] = T;
   }
   g.bl = 0.5 * prm->cutoff;
+  h->graph_chunk = prm->graph_chunk > 0 ? prm->graph_chunk : 0;
 
   if (hipSetDevice(h->dev) != hipSuccess) {
     delete h;
@@ -441,6 +556,7 @@ int examd_hip_destroy(examd_handle *h) {
   if (!h) return EXAMD_ERR_ARG;
   (void)hipSetDevice(h->dev);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
   free_state(h);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -460,6 +576,7 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
   }
   if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
   h->host_pos.assign(pos, pos + 3 * n);
+  drop_graph(h);
   std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
   double *dst[3] = {h->d.x, h->d.y, h->d.z};
   double *vdst[3] = {h->d.vx, h->d.vy, h->d.vz};
@@ -575,43 +692,79 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   if (n_out) *n_out = 0;
   HIPCHK(hipSetDevice(h->dev));
-  const bool mixed = precision == EXAMD_PREC_MIXED;
-  if (ensure_log(h, nsteps + 1)) return EXAMD_ERR_NOMEM;
-  launch_fill_log(h->log, (int)(nsteps + 1), h->stream);
   auto is_energy = [&](int64_t s) {
     return s == nsteps || (energy_every > 0 && s % energy_every == 0);
   };
-  if (h->forces_valid != precision) evaluate(h, precision, true, false, h->log);
-  const double hk = (0.5 * dt) / h->p.mass;
-  for (int64_t s = 1; s <= nsteps; ++s) {
-    StepLog *ke_prev = (s > 1 && is_energy(s - 1)) ? h->log + (s - 1) : nullptr;
-    launch_step_bin(h->d, h->g, mixed,
-                    s == 1 ? kKickDriftBin : kFinishKickDriftBin, hk, dt,
-                    ke_prev, h->log + s, h->stream);
-    launch_sort(h->d, h->nblocks, h->stream);
-    forces_timed(h, precision, is_energy(s), false, h->log + s);
+  // chunks of C steps replayed from a captured graph; C a multiple of the
+  // tally stride so every chunk has the same tally pattern
+  int64_t C = 0;
+  if (h->graph_chunk > 0 && nsteps > 0) {
+    C = h->graph_chunk;
+    if (energy_every > 0) C = std::max<int64_t>(energy_every, C / energy_every * energy_every);
+    if (C > nsteps) C = 0;
   }
-  if (nsteps > 0)
-    launch_kick(h->d, mixed, hk, is_energy(nsteps) ? h->log + nsteps : nullptr,
-                h->stream);
-  HIPCHK(hipGetLastError());
-  std::vector<StepLog> L((size_t)nsteps + 1);
-  HIPCHK(hipMemcpyAsync(L.data(), h->log, (nsteps + 1) * sizeof(StepLog),
-                        hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  if (h->timing) collect_timing(h);
-  h->forces_valid = precision;
+  const int64_t nchunks = C > 0 ? nsteps / C : 0;
+  const int64_t done_by_graph = nchunks * C;
+  const int64_t rem = nsteps - done_by_graph;
+  if (ensure_log(h, std::max<int64_t>(C, rem) + 1)) return EXAMD_ERR_NOMEM;
+  // log[0]: flags of the evaluation at the starting positions, if one runs
+  launch_fill_log(h->log, 1, h->stream);
+  if (h->forces_valid != precision) {
+    const bool t = h->timing;   // only the step launches are timed
+    h->timing = false;
+    evaluate(h, precision, true, false, h->log);
+    h->timing = t;
+  }
   unsigned long long fl = 0;
-  for (auto &x : L) fl |= x.flags;
   int64_t k = 0;
-  for (int64_t s = 1; s <= nsteps && out; ++s) {
-    if (!is_energy(s) || k >= out_cap) continue;
-    fill_stats(h, L[s], precision, s, &out[k++]);
+  std::vector<StepLog> L;
+  // read back the tallies of steps base+1..base+count and check flags
+  auto harvest = [&](int64_t base, int64_t count) -> int {
+    L.resize((size_t)count + 1);
+    HIPCHK(hipMemcpyAsync(L.data(), h->log, (count + 1) * sizeof(StepLog),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->timing) collect_timing(h);
+    fl |= L[0].flags;
+    for (int64_t c = 1; c <= count; ++c) {
+      fl |= L[c].flags;
+      const int64_t s = base + c;
+      if (out && is_energy(s) && k < out_cap)
+        fill_stats(h, L[c], precision, s, &out[k++]);
+    }
+    return EXAMD_OK;
+  };
+  if (nchunks > 0) {
+    GraphKey key{precision, C, energy_every > 0 ? energy_every : 0, dt,
+                 h->timing, h->tc[precision]};
+    int st = ensure_graph(h, key);
+    if (st) return st;
+    for (int64_t c = 0; c < nchunks; ++c) {
+      HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
+      h->ev_used = h->graph_events;   // the events the replay recorded
+      st = harvest(c * C, C);
+      if (st) return st;
+      if (fl & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite)) break;
+    }
   }
+  if (rem > 0 && !(fl & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite))) {
+    enqueue_steps(h, precision, rem, dt,
+                  [&](int64_t c) { return is_energy(done_by_graph + c); });
+    HIPCHK(hipGetLastError());
+    int st = harvest(done_by_graph, rem);
+    if (st) return st;
+  }
+  h->forces_valid = precision;
   if (n_out) *n_out = k;
   int st = flags_to_status(fl);
   if (st != EXAMD_OK) h->forces_valid = -1;
   return st;
+}
+
+int examd_hip_set_graph_chunk(examd_handle *h, int64_t chunk) {
+  if (!h || chunk < 0) return EXAMD_ERR_ARG;
+  h->graph_chunk = chunk;
+  return EXAMD_OK;
 }
 
 int examd_hip_block_ids(examd_handle *h, int64_t *blocknum, int32_t *blockxyz,

@@ -31,7 +31,8 @@ Module exaMD_hip_binding
      Integer(c_int32_t) :: block_threads
      Integer(c_int32_t) :: cap_slack_pct
      Integer(c_int32_t) :: run_groups
-     Integer(c_int32_t) :: reserved(3)
+     Integer(c_int32_t) :: graph_chunk
+     Integer(c_int32_t) :: reserved(2)
   End Type examd_params
 
   Type, Bind(C) :: examd_tune
@@ -180,6 +181,13 @@ Module exaMD_hip_binding
        Integer(c_int), Value :: precision
        Type(examd_tune), Intent(in) :: cfg
      End Function examd_hip_set_config
+
+     Integer(c_int) Function examd_hip_set_graph_chunk(handle, chunk) &
+          Bind(C, name="examd_hip_set_graph_chunk")
+       Import :: c_int, c_ptr, c_int64_t
+       Type(c_ptr), Value :: handle
+       Integer(c_int64_t), Value :: chunk
+     End Function examd_hip_set_graph_chunk
 
      Type(c_ptr) Function examd_hip_strerror(status) &
           Bind(C, name="examd_hip_strerror")

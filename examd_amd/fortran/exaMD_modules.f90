@@ -439,6 +439,7 @@ Contains
     p%block_threads = 0
     p%cap_slack_pct = 0
     p%run_groups = 0
+    p%graph_chunk = 100
     p%reserved = 0
     st = examd_hip_create(p, handle)
     Call check_status(st, 'examd_hip_create')

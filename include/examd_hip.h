@@ -69,7 +69,9 @@ typedef struct examd_params {
                              largest region at set-up, in percent (0 = 15) */
     int32_t run_groups;   /* work items per region atom in the pair loop
                              (1-4; 0 = choose) */
-    int32_t reserved[3];
+    int32_t graph_chunk;  /* MD steps per captured hipGraph chunk (0 = launch
+                             kernels one by one) */
+    int32_t reserved[2];
 } examd_params;
 
 /* Tallies of one step or one force evaluation. */
@@ -127,6 +129,11 @@ int examd_hip_forces(examd_handle *h, int precision, double *forces,
 int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
                   int64_t energy_every, examd_stats *out, int64_t out_cap,
                   int64_t *n_out);
+
+/* Steps per hipGraph chunk of examd_hip_run (0 = no graphs).  A chunk's
+ * kernel sequence is captured once and replayed; the chunk length is
+ * rounded to a multiple of energy_every. */
+int examd_hip_set_graph_chunk(examd_handle *h, int64_t chunk);
 
 /* Cell assignment of the current positions as exaMD_identify_block gives it
  * (SRC/exaMD_module.f90:101-125): 1-based block number per atom, and the

@@ -230,3 +230,58 @@ def test_lane_queue_variant_matches(E, oracle, prec):
         np.testing.assert_array_equal(got, hc)
     else:
         assert np.array_equal(f1, f0)
+
+
+def _stats_tuple(s):
+    return (s.step, s.ke, s.pe, s.pe_shifted, s.pairs_cand, s.pairs_cut)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_graph_chunks_match_eager(E, prec):
+    """run() replaying a captured chunk of 10 steps equals the kernel-by-kernel
+    run: every replay's tallies, the state after each run() call (eager
+    copies and synchronisations in between), and a trailing partial chunk
+    run eagerly.  Mixed precision is bit-identical (order-independent
+    fixed-point sums); FP64 sums forces with floating-point atomics, so it
+    is compared within a tolerance."""
+    print(f"[graph vs eager {prec}]", flush=True)
+    pos, vel, box = E.generate_fcc(10, RHO, 0.72, seed=21)
+    states = {}
+    tallies = {}
+    for chunk in (0, 10):
+        with E.MidpointLJ(len(pos), box, 2.5) as m:
+            m.set_state(pos, vel)
+            m.set_graph_chunk(chunk)
+            seq, st = [], []
+            for nsteps in (30, 30, 25):
+                st += [_stats_tuple(s) for s in
+                       m.run(nsteps, 0.005, prec, energy_every=10)]
+                seq.append(m.get_state())
+            states[chunk], tallies[chunk] = seq, st
+    assert [t[0] for t in tallies[10]] == [t[0] for t in tallies[0]]
+    assert len(tallies[10]) == 9
+    if prec == "mixed":
+        assert tallies[10] == tallies[0]
+        for (p0, v0), (p1, v1) in zip(states[0], states[10]):
+            assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+    else:
+        a, b = np.array(tallies[0]), np.array(tallies[10])
+        assert np.allclose(a[:, 1:4], b[:, 1:4], rtol=1e-9, atol=0)
+        assert np.array_equal(a[:, 4:], b[:, 4:])
+        for (p0, v0), (p1, v1) in zip(states[0], states[10]):
+            assert np.abs(p0 - p1).max() < 1e-9
+            assert np.abs(v0 - v1).max() < 1e-8
+
+
+def test_graph_chunk_timing_counts_launches(E):
+    """Force-kernel timing events captured in a chunk are read back after
+    every replay: one timed launch per step."""
+    pos, vel, box = E.generate_fcc(10, RHO, 0.72, seed=22)
+    with E.MidpointLJ(len(pos), box, 2.5, graph_chunk=20) as m:
+        m.set_state(pos, vel)
+        m.run(5, 0.005, "mixed")
+        m.timing(enable=True, reset=True)
+        m.run(60, 0.005, "mixed", energy_every=20)
+        ms, launches = m.timing(enable=False, reset=True)
+    assert launches == 60
+    assert ms > 0.0
