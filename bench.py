@@ -4,8 +4,9 @@ Workload (one GPU's share at real shape): 80^3 FCC lattice, 2,048,000 atoms,
 density 0.8442, Maxwell-Boltzmann velocities at T* = 0.72 (seeded, zero net
 momentum), sigma = epsilon = mass = 1, cutoff 2.5 (unshifted), cubic periodic
 box of 107^3 rc/2 blocks split into 16^3 cell blocks, velocity Verlet
-dt = 0.005.  Each step rebins on device and evaluates the midpoint-method
-forces once.
+dt = 0.005, 10,000 steps per precision by default (energies every 10 steps),
+replayed from captured 100-step hipGraph chunks.  Each step rebins on
+device and evaluates the midpoint-method forces once.
 
 value = mixed-precision atom-steps/s (FP32 pair arithmetic, int64 fixed-point
 accumulation, FP64 integration); the FP64 path is timed on the same system
@@ -51,6 +52,14 @@ def drift_per_1k(stats):
     return float(slope * 1000.0 / abs(e.mean()))
 
 
+def max_dev(stats):
+    """max |E(t) - E(0)| / |E(0)| of the shifted total energy."""
+    if not stats:
+        return None
+    e = np.array([s.ke + s.pe_shifted for s in stats])
+    return float(np.abs(e - e[0]).max() / abs(e[0]))
+
+
 def read_traffic():
     """HBM bytes per force launch from the committed rocprofv3 PMC summary
     (line 'traffic_bytes_per_launch: N'), or None."""
@@ -74,11 +83,27 @@ def parse_shape(text):
             "lane_queue": v[5] if len(v) > 5 else 0}
 
 
+def force_error(m):
+    """Per-atom force error of the mixed path against the FP64 path at the
+    handle's current positions: max and RMS of |f_mixed - f_fp64| (epsilon /
+    sigma units) and both relative to the RMS FP64 force."""
+    f64, _, _ = m.forces("fp64")
+    fmx, _, _ = m.forces("mixed")
+    d = np.linalg.norm(fmx - f64, axis=1)
+    frms = float(np.sqrt(np.mean(np.sum(f64 * f64, axis=1))))
+    return {"max_abs": float(d.max()), "rms_abs": float(np.sqrt(np.mean(d * d))),
+            "rms_force_fp64": frms, "max_rel_to_rms_force": float(d.max()) / frms,
+            "rms_rel_to_rms_force": float(np.sqrt(np.mean(d * d))) / frms}
+
+
 def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
-              shape=None, graph_chunk=100):
+              shape=None, graph_chunk=100, report_error=False):
     """Autotune the launch shape for this precision on the initial state
     (or apply the given shape), then warm up and time `steps` MD steps,
-    replayed from captured hipGraph chunks of `graph_chunk` steps."""
+    replayed from captured hipGraph chunks of `graph_chunk` steps.  With
+    report_error, the mixed-vs-FP64 force error is taken at the final
+    positions (outside the timed region)."""
+    err = None
     with E.MidpointLJ(len(pos), box, RC, graph_chunk=graph_chunk) as m:
         m.set_state(pos, vel)
         if shape is None:
@@ -94,10 +119,12 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
         stats = m.run(steps, DT, prec, energy_every=energy_every)
         t1 = time.perf_counter()
         force_ms, launches = m.timing(enable=False, reset=True)
+        if report_error:
+            err = force_error(m)
     if launches != steps:
         raise RuntimeError(f"{prec}: {launches} timed force launches for "
                            f"{steps} steps")
-    return t1 - t0, stats, force_ms / max(launches, 1), launches, tune
+    return t1 - t0, stats, force_ms / max(launches, 1), launches, tune, err
 
 
 def cpu_baseline(pos, vel, box, nsteps):
@@ -146,7 +173,7 @@ def pmc_probe(E, shape):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -178,10 +205,10 @@ def main():
     fsteps = args.fp64_steps if args.fp64_steps is not None else steps
     energy_every = 10
 
-    wall_mx, st_mx, force_ms_mx, launches, tune_mx = timed_run(
+    wall_mx, st_mx, force_ms_mx, launches, tune_mx, ferr = timed_run(
         E, pos, vel, box, "mixed", steps, args.warmup, energy_every,
-        parse_shape(args.mixed_shape), args.graph_chunk)
-    wall_64, st_64, force_ms_64, _, tune_64 = timed_run(
+        parse_shape(args.mixed_shape), args.graph_chunk, report_error=True)
+    wall_64, st_64, force_ms_64, _, tune_64, _ = timed_run(
         E, pos, vel, box, "fp64", fsteps, args.warmup, energy_every,
         parse_shape(args.fp64_shape), args.graph_chunk)
 
@@ -241,6 +268,10 @@ def main():
             "fp64_force_frac_of_fp64_peak": achieved64 / FP64_PEAK_TFLOPS,
             "energy_drift_per_1k_mixed": drift_per_1k(st_mx),
             "energy_drift_per_1k_fp64": drift_per_1k(st_64),
+            "energy_drift_steps": [steps, fsteps],
+            "energy_max_dev_rel_mixed": max_dev(st_mx),
+            "energy_max_dev_rel_fp64": max_dev(st_64),
+            "mixed_force_error_vs_fp64_at_end": ferr,
             "launch_mixed": tune_mx,
             "launch_fp64": tune_64,
             "graph_chunk_steps": args.graph_chunk,

@@ -10,7 +10,6 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from dataclasses import dataclass
 
 import numpy as np
 
@@ -44,6 +43,7 @@ EXPORTS = (
     "examd_hip_timing", "examd_hip_generate_fcc", "examd_hip_strerror",
     "examd_hip_phase_profile", "examd_hip_autotune", "examd_hip_get_config",
     "examd_hip_set_config", "examd_hip_set_graph_chunk",
+    "examd_hip_batch_create", "examd_hip_batch_info",
 )
 
 
@@ -66,12 +66,18 @@ class Params(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("tiles", C.c_int32 * 3), ("block_threads", C.c_int32),
                 ("run_groups", C.c_int32), ("lane_queue", C.c_int32),
+                ("tile_edge", C.c_int32), ("workgroups", C.c_int32),
                 ("force_ms", C.c_double)]
 
     def as_dict(self):
         return {"tiles": tuple(self.tiles), "block_threads": self.block_threads,
                 "run_groups": self.run_groups, "lane_queue": self.lane_queue,
+                "tile_edge": self.tile_edge, "workgroups": self.workgroups,
                 "force_ms": self.force_ms}
+
+
+class System(C.Structure):
+    _fields_ = [("natoms", C.c_int64), ("box", C.c_double * 3)]
 
 
 class Stats(C.Structure):
@@ -99,6 +105,10 @@ def lib():
         dp, ip64, ip32 = P(C.c_double), P(C.c_int64), P(C.c_int32)
         L.examd_hip_domain.argtypes = [dp, C.c_double, ip32, ip64]
         L.examd_hip_create.argtypes = [P(Params), P(C.c_void_p)]
+        L.examd_hip_batch_create.argtypes = [P(Params), P(System), C.c_int32,
+                                             P(C.c_void_p)]
+        L.examd_hip_batch_info.argtypes = [C.c_void_p, ip32, ip64, ip64]
+        L.examd_hip_set_graph_chunk.argtypes = [C.c_void_p, C.c_int64]
         L.examd_hip_destroy.argtypes = [C.c_void_p]
         L.examd_hip_set_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
         L.examd_hip_get_state.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64]
@@ -165,37 +175,11 @@ def generate_fcc(nc, rho, temperature=0.0, mass=1.0, perturb=0.0, seed=0):
     return pos, vel, box
 
 
-@dataclass
-class RunResult:
-    stats: list
+class _Handle:
+    """Common methods of a handle of the C ABI (one system or a batch)."""
 
-
-class MidpointLJ:
-    """One LJ system on one GPU (a handle of the C ABI)."""
-
-    def __init__(self, natoms, box, cutoff, epsilon=1.0, sigma=1.0, mass=1.0,
-                 tiles=(0, 0, 0), device=0, block_threads=0,
-                 cap_slack_pct=0, run_groups=0, graph_chunk=0):
-        p = Params()
-        p.natoms = int(natoms)
-        p.box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
-        p.cutoff, p.epsilon, p.sigma, p.mass = (float(cutoff), float(epsilon),
-                                                float(sigma), float(mass))
-        p.tiles = (C.c_int32 * 3)(*[int(t) for t in tiles])
-        p.device = int(device)
-        p.block_threads = int(block_threads)
-        p.cap_slack_pct = int(cap_slack_pct)
-        p.run_groups = int(run_groups)
-        p.graph_chunk = int(graph_chunk)
-        self.params = p
-        self.n = int(natoms)
-        self.box = np.array(np.reshape(box, 3), dtype=np.float64)
-        self.cutoff = float(cutoff)
-        self.nb, self.nblocks = domain(self.box, 0.5 * self.cutoff)
-        h = C.c_void_p()
-        _check(lib().examd_hip_create(C.byref(p), C.byref(h)),
-               "examd_hip_create")
-        self._h = h
+    n: int
+    nsys: int
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -214,6 +198,12 @@ class MidpointLJ:
     def __exit__(self, *a):
         self.close()
 
+    def _info(self):
+        ns, na, nb = C.c_int32(), C.c_int64(), C.c_int64()
+        _check(lib().examd_hip_batch_info(self._h, C.byref(ns), C.byref(na),
+                                          C.byref(nb)), "examd_hip_batch_info")
+        self.nsys, self.n, self.nblocks = ns.value, na.value, nb.value
+
     def set_state(self, pos, vel=None):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         vel = None if vel is None else np.ascontiguousarray(vel, np.float64)
@@ -229,16 +219,18 @@ class MidpointLJ:
                                          self.n), "examd_hip_get_state")
         return (pos, vel, f) if forces else (pos, vel)
 
-    def forces(self, precision="fp64", allow=()):
+    def _forces(self, precision, allow):
         f = np.zeros((self.n, 3))
-        s = Stats()
+        s = (Stats * self.nsys)()
         st = _check(lib().examd_hip_forces(self._h, _PREC[precision], _dp(f),
-                                           C.byref(s)),
-                    "examd_hip_forces", allow)
-        return f, s, st
+                                           s), "examd_hip_forces", allow)
+        return f, [s[i] for i in range(self.nsys)], st
 
-    def run(self, nsteps, dt, precision="fp64", energy_every=0):
-        cap = max(1, int(nsteps))
+    def _run(self, nsteps, dt, precision, energy_every):
+        rows = max(1, int(nsteps) if energy_every else 1)
+        if energy_every:
+            rows = int(nsteps) // int(energy_every) + 1
+        cap = rows * self.nsys
         out = (Stats * cap)()
         nout = C.c_int64()
         _check(lib().examd_hip_run(self._h, _PREC[precision], int(nsteps),
@@ -248,7 +240,7 @@ class MidpointLJ:
 
     def set_graph_chunk(self, chunk):
         """Steps per captured hipGraph chunk of run() (0 = launch eagerly)."""
-        _check(lib().examd_hip_set_graph_chunk(self._h, C.c_int64(int(chunk))),
+        _check(lib().examd_hip_set_graph_chunk(self._h, int(chunk)),
                "examd_hip_set_graph_chunk")
 
     def block_ids(self):
@@ -268,7 +260,7 @@ class MidpointLJ:
         return c
 
     def autotune(self, precision="mixed", trials=3):
-        """Pick the fastest force-kernel launch shape for this system."""
+        """Pick the fastest force-kernel launch shape for this state."""
         t = Tune()
         _check(lib().examd_hip_autotune(self._h, _PREC[precision], int(trials),
                                         C.byref(t)), "examd_hip_autotune")
@@ -280,13 +272,15 @@ class MidpointLJ:
                                           C.byref(t)), "examd_hip_get_config")
         return t.as_dict()
 
-    def set_config(self, precision, tiles, block_threads, run_groups=0,
-                   lane_queue=0):
+    def set_config(self, precision, tiles=(0, 0, 0), block_threads=512,
+                   run_groups=0, lane_queue=0, tile_edge=0, workgroups=0):
         t = Tune()
         t.tiles = (C.c_int32 * 3)(*[int(v) for v in tiles])
         t.block_threads = int(block_threads)
         t.run_groups = int(run_groups)
         t.lane_queue = int(lane_queue)
+        t.tile_edge = int(tile_edge)
+        t.workgroups = int(workgroups)
         _check(lib().examd_hip_set_config(self._h, _PREC[precision],
                                           C.byref(t)), "examd_hip_set_config")
 
@@ -305,3 +299,85 @@ class MidpointLJ:
                                       C.byref(n), int(reset)),
                "examd_hip_timing")
         return ms.value, n.value
+
+
+def _params(cutoff, epsilon, sigma, mass, device, block_threads,
+            cap_slack_pct, run_groups, graph_chunk):
+    p = Params()
+    p.cutoff, p.epsilon, p.sigma, p.mass = (float(cutoff), float(epsilon),
+                                            float(sigma), float(mass))
+    p.device = int(device)
+    p.block_threads = int(block_threads)
+    p.cap_slack_pct = int(cap_slack_pct)
+    p.run_groups = int(run_groups)
+    p.graph_chunk = int(graph_chunk)
+    return p
+
+
+class MidpointLJ(_Handle):
+    """One LJ system on one GPU (a handle of the C ABI)."""
+
+    def __init__(self, natoms, box, cutoff, epsilon=1.0, sigma=1.0, mass=1.0,
+                 tiles=(0, 0, 0), device=0, block_threads=0,
+                 cap_slack_pct=0, run_groups=0, graph_chunk=0):
+        p = _params(cutoff, epsilon, sigma, mass, device, block_threads,
+                    cap_slack_pct, run_groups, graph_chunk)
+        p.natoms = int(natoms)
+        p.box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
+        p.tiles = (C.c_int32 * 3)(*[int(t) for t in tiles])
+        self.params = p
+        self.box = np.array(np.reshape(box, 3), dtype=np.float64)
+        self.cutoff = float(cutoff)
+        self.nb, _ = domain(self.box, 0.5 * self.cutoff)
+        h = C.c_void_p()
+        _check(lib().examd_hip_create(C.byref(p), C.byref(h)),
+               "examd_hip_create")
+        self._h = h
+        self._info()
+
+    def forces(self, precision="fp64", allow=()):
+        """(forces (n, 3), Stats, status) at the current positions."""
+        f, s, st = self._forces(precision, allow)
+        return f, s[0], st
+
+    def run(self, nsteps, dt, precision="fp64", energy_every=0):
+        """Velocity-Verlet steps; Stats of every tallied step."""
+        return self._run(nsteps, dt, precision, energy_every)
+
+
+class MidpointLJBatch(_Handle):
+    """Independent periodic LJ systems of uneven sizes in one handle: every
+    kernel of a step covers all systems in one launch.  systems: sequence of
+    (natoms, box).  Positions and velocities are concatenated in system
+    order."""
+
+    def __init__(self, systems, cutoff, epsilon=1.0, sigma=1.0, mass=1.0,
+                 device=0, block_threads=0, cap_slack_pct=0, run_groups=0,
+                 graph_chunk=0):
+        p = _params(cutoff, epsilon, sigma, mass, device, block_threads,
+                    cap_slack_pct, run_groups, graph_chunk)
+        arr = (System * len(systems))()
+        for k, (n, box) in enumerate(systems):
+            arr[k].natoms = int(n)
+            arr[k].box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
+        self.params = p
+        self.sizes = np.array([int(n) for n, _ in systems], dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+        self.cutoff = float(cutoff)
+        h = C.c_void_p()
+        _check(lib().examd_hip_batch_create(C.byref(p), arr, len(systems),
+                                            C.byref(h)),
+               "examd_hip_batch_create")
+        self._h = h
+        self._info()
+
+    def forces(self, precision="fp64", allow=()):
+        """(forces (n, 3), [Stats per system], status)."""
+        return self._forces(precision, allow)
+
+    def run(self, nsteps, dt, precision="mixed", energy_every=0):
+        """Velocity-Verlet steps of every system; one list of per-system
+        Stats per tallied step."""
+        flat = self._run(nsteps, dt, precision, energy_every)
+        k = self.nsys
+        return [flat[i:i + k] for i in range(0, len(flat), k)]

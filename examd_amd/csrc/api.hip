@@ -2,8 +2,13 @@
 //
 // Host orchestration only: device buffers, the cell-block (tile) map that
 // replaces the METIS partition (SRC/exaMD_partitioning.f90:65-455), LDS
-// sizing, the per-step kernel sequence and the seeded generators.
+// sizing, the per-step kernel sequence, captured multi-step chunks and the
+// seeded generators.  A handle holds one system (examd_hip_create) or a
+// batch of independent systems (examd_hip_batch_create); the kernels see
+// both through one table of systems, the single system being a batch of
+// one.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -15,12 +20,23 @@
 
 using namespace examd;
 
-// Cell-block map and force-kernel launch shape of one precision.
+using Tri = std::array<int, 3>;
+
+// One system of a handle, host side.
+struct SysHost {
+  int64_t atom0 = 0, n = 0;       // atom range in the concatenated order
+  int64_t block0 = 0, nblocks = 0;
+  Grid g{};                       // nb, L, bl (T unused here)
+};
+
+// Cell-block maps and force-kernel launch shape of one precision.
 struct TileConfig {
-  int T[3] = {1, 1, 1};   // cell blocks per dimension
+  std::vector<Tri> T;     // cell blocks per dimension, per system
+  int edge = 0;           // tile edge the maps were derived from (0 = given)
   int threads = 256;      // workgroup size
   int groups = 1;         // run groups per region atom
   int lane_queue = 0;     // per-lane queue depth of deferred pairs (0 = off)
+  int workgroups = 0;     // persistent force grid (0 = one per tile)
   int cap = 0;            // LDS atom capacity
   int maxrb = 0;          // largest region in blocks
   bool fixed_tiles = false;
@@ -36,20 +52,31 @@ struct GraphKey {
   TileConfig cfg;
 };
 
+// Device copy of the system table of one precision.
+struct DevTables {
+  SysDesc *sys = nullptr;
+  int *tile_sys = nullptr;
+  int *tile_seg = nullptr;
+  int tile_cap = 0;
+  int ntiles = 0;
+};
+
 struct examd_handle {
   examd_params p{};
   int dev = 0;
   hipStream_t stream = nullptr;
-  Grid g{};
-  int64_t nblocks = 0;
-  int64_t n = 0;
+  int nsys = 1;
+  std::vector<SysHost> sys;
+  long long *atom_seg = nullptr;  // device, nsys + 1
+  DevTables tab[2];               // per precision
+  int64_t nblocks = 0;            // all systems
+  int64_t n = 0;                  // all systems
   DevState d{};
   StepLog *log = nullptr;
-  int64_t log_cap = 0;
-  TileConfig tc[2];         // cell-block map and launch shape per precision
+  int64_t log_cap = 0;            // StepLog entries
+  TileConfig tc[2];               // per precision
   size_t lds_limit = 0;
   int64_t part_cap = 0;
-  std::vector<double> host_pos;   // positions of the last set_state (AoS)
   // captured multi-step chunk
   int64_t graph_chunk = 0;        // steps per captured chunk (0 = eager)
   hipGraph_t graph = nullptr;
@@ -57,8 +84,7 @@ struct examd_handle {
   GraphKey graph_key{};
   size_t graph_events = 0;        // timing events recorded inside the graph
   bool state_set = false;
-  int forces_valid = -1;    // precision whose forces match the positions
-  int last_prec = 0;
+  int forces_valid = -1;          // precision whose forces match the positions
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -94,30 +120,101 @@ int dalloc(T **p, int64_t n) {
   return EXAMD_OK;
 }
 
+void drop_graph(examd_handle *h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+  h->graph_events = 0;
+}
+
+void free_tables(DevTables &t) {
+  if (t.sys) (void)hipFree(t.sys);
+  if (t.tile_sys) (void)hipFree(t.tile_sys);
+  if (t.tile_seg) (void)hipFree(t.tile_seg);
+  t = DevTables{};
+}
+
 void free_state(examd_handle *h) {
   DevState &d = h->d;
   void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
                   d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.rank,
-                  d.count, d.start, d.sorted, d.chunk, d.home, d.part, h->log};
+                  d.count, d.start, d.sorted, d.chunk, d.home, d.part, h->log,
+                  h->atom_seg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DevState{};
   h->log = nullptr;
+  h->atom_seg = nullptr;
   h->log_cap = 0;
   h->part_cap = 0;
+  free_tables(h->tab[0]);
+  free_tables(h->tab[1]);
 }
 
 int64_t tiles_of(const TileConfig &c) {
-  return (int64_t)c.T[0] * c.T[1] * c.T[2];
+  int64_t t = 0;
+  for (const Tri &T : c.T) t += (int64_t)T[0] * T[1] * T[2];
+  return t;
 }
 
-Grid grid_for(const examd_handle *h, int prec) {
-  Grid g = h->g;
-  for (int a = 0; a < 3; ++a) g.T[a] = h->tc[prec].T[a];
+Grid grid_of(const examd_handle *h, int prec, int k) {
+  Grid g = h->sys[k].g;
+  for (int a = 0; a < 3; ++a) g.T[a] = h->tc[prec].T[k][a];
   return g;
 }
 
-void drop_graph(examd_handle *h);
+// The kernels' view of the systems with the tile maps of one precision.
+SysTable table(const examd_handle *h, int prec) {
+  SysTable t;
+  t.sys = h->tab[prec].sys;
+  t.tile_sys = h->tab[prec].tile_sys;
+  t.tile_seg = h->tab[prec].tile_seg;
+  t.atom_seg = h->atom_seg;
+  t.nsys = h->nsys;
+  t.ntiles = h->tab[prec].ntiles;
+  return t;
+}
+
+// Upload the system table of one precision (after its tile maps changed).
+int upload_tables(examd_handle *h, int prec) {
+  drop_graph(h);   // a captured chunk holds the old tables
+  HIPCHK(hipStreamSynchronize(h->stream));
+  DevTables &t = h->tab[prec];
+  const int nsys = h->nsys;
+  std::vector<SysDesc> sd((size_t)nsys);
+  std::vector<int> seg((size_t)nsys + 1), tsys;
+  int tiles = 0;
+  for (int k = 0; k < nsys; ++k) {
+    sd[k].g = grid_of(h, prec, k);
+    sd[k].atom0 = (long long)h->sys[k].atom0;
+    sd[k].block0 = (int)h->sys[k].block0;
+    sd[k].tile0 = tiles;
+    seg[k] = tiles;
+    const Tri &T = h->tc[prec].T[k];
+    const int nt = T[0] * T[1] * T[2];
+    for (int i = 0; i < nt; ++i) tsys.push_back(k);
+    tiles += nt;
+  }
+  seg[nsys] = tiles;
+  if (!t.sys && dalloc(&t.sys, nsys)) return EXAMD_ERR_NOMEM;
+  if (!t.tile_seg && dalloc(&t.tile_seg, nsys + 1)) return EXAMD_ERR_NOMEM;
+  if (t.tile_cap < tiles) {
+    if (t.tile_sys) (void)hipFree(t.tile_sys);
+    t.tile_sys = nullptr;
+    t.tile_cap = 0;
+    if (dalloc(&t.tile_sys, tiles)) return EXAMD_ERR_NOMEM;
+    t.tile_cap = tiles;
+  }
+  HIPCHK(hipMemcpy(t.sys, sd.data(), nsys * sizeof(SysDesc),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(t.tile_seg, seg.data(), (nsys + 1) * sizeof(int),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(t.tile_sys, tsys.data(), tiles * sizeof(int),
+                   hipMemcpyHostToDevice));
+  t.ntiles = tiles;
+  return EXAMD_OK;
+}
 
 // Partial-tally slots for the current tile maps.
 int ensure_partials(examd_handle *h) {
@@ -132,7 +229,9 @@ int ensure_partials(examd_handle *h) {
   return EXAMD_OK;
 }
 
-int ensure_log(examd_handle *h, int64_t n) {
+// StepLog entries for `steps` + 1 steps of every system.
+int ensure_log(examd_handle *h, int64_t steps) {
+  const int64_t n = (steps + 1) * h->nsys;
   if (h->log_cap >= n) return EXAMD_OK;
   drop_graph(h);   // a captured chunk holds the old address
   if (h->log) (void)hipFree(h->log);
@@ -201,57 +300,87 @@ int64_t max_region_atoms(const Grid &g, const double *pos, int64_t n) {
   return best;
 }
 
+// Cell-block map of a system from a tile edge in blocks.
+Tri tiles_from_edge(const Grid &g, double edge) {
+  Tri T;
+  for (int a = 0; a < 3; ++a)
+    T[a] = std::max(1, std::min(g.nb[a], (int)std::lround(g.nb[a] / edge)));
+  return T;
+}
+
 // Pick the LDS capacity of a cell block for one precision from the largest
-// region of the given positions, with cap_slack_pct headroom for density
-// fluctuations during a run (at least 5%).  When the tiles were chosen
-// automatically and a region does not fit the device's LDS, refine the
-// cell-block map along its coarsest axis.
+// region of the given positions (over all systems), with cap_slack_pct
+// headroom for density fluctuations during a run (at least 5%).  When the
+// maps were chosen automatically and a region does not fit the device's
+// LDS, refine the map of the system with the fullest region along its
+// coarsest axis.  Uploads the system table on success.
 int choose_caps(examd_handle *h, int prec, const double *pos) {
   TileConfig &c = h->tc[prec];
   const bool fp64 = prec == EXAMD_PREC_FP64;
-  for (int iter = 0; iter < 256; ++iter) {
-    Grid g = grid_for(h, prec);
-    c.maxrb = max_region_blocks(g);
-    // region coordinates are packed 8 bits per axis on the device
+  const int nsys = h->nsys;
+  for (int iter = 0; iter < 4096; ++iter) {
     bool ok = true;
-    for (int a = 0; a < 3; ++a)
-      if ((g.nb[a] + g.T[a] - 1) / g.T[a] + 2 > 255) ok = false;
-    int64_t mra = ok ? max_region_atoms(g, pos, h->n) : 0;
+    int64_t mra = 0;
+    int worst = 0;
+    c.maxrb = 0;
+    double reg_atoms = 0.0, reg_tiles = 0.0;
+    for (int k = 0; k < nsys; ++k) {
+      const Grid g = grid_of(h, prec, k);
+      c.maxrb = std::max(c.maxrb, max_region_blocks(g));
+      // region coordinates are packed 8 bits per axis on the device
+      for (int a = 0; a < 3; ++a)
+        if ((g.nb[a] + g.T[a] - 1) / g.T[a] + 2 > 255) ok = false;
+      if (!ok) break;
+      const SysHost &S = h->sys[k];
+      const int64_t m = max_region_atoms(g, pos + 3 * S.atom0, S.n);
+      if (m > mra) {
+        mra = m;
+        worst = k;
+      }
+      const double vol = g.L[0] * g.L[1] * g.L[2];
+      double reg = (double)S.n / vol;
+      for (int a = 0; a < 3; ++a) reg *= ((double)g.nb[a] / g.T[a] + 2.0) * g.bl;
+      const double nt = (double)g.T[0] * g.T[1] * g.T[2];
+      reg_atoms += reg * nt;
+      reg_tiles += nt;
+    }
     const double slack =
         (h->p.cap_slack_pct > 0 ? h->p.cap_slack_pct : 15) / 100.0;
     int64_t want = ((int64_t)std::ceil((double)mra * (1.0 + slack)) + 32 + 15) / 16 * 16;
     int64_t need = (int64_t)std::ceil((double)mra * 1.05) + 16;
     int64_t cap = want;
-    while (cap > need &&
-           force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit)
-      cap -= 16;
-    if (force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit ||
-        cap > 65535)
-      ok = false;
+    if (ok) {
+      while (cap > need &&
+             force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit)
+        cap -= 16;
+      if (force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit ||
+          cap > 65535)
+        ok = false;
+    }
     if (ok) {
       c.cap = (int)cap;
       // about three work items per lane: run groups from the mean region
-      const double vol = g.L[0] * g.L[1] * g.L[2];
-      double reg = (double)h->n / vol;
-      for (int a = 0; a < 3; ++a) reg *= ((double)g.nb[a] / g.T[a] + 2.0) * g.bl;
+      const double reg = reg_tiles > 0.0 ? reg_atoms / reg_tiles : 1.0;
       int ng = c.fixed_groups > 0
                    ? c.fixed_groups
                    : (int)std::lround(3.0 * c.threads / std::max(reg, 1.0));
       c.groups = std::max(1, std::min(4, ng));
-      return EXAMD_OK;
+      return upload_tables(h, prec);
     }
     if (c.fixed_tiles) return EXAMD_ERR_TILE_OVERFLOW;
+    Tri &T = c.T[worst];
+    const Grid &g = h->sys[worst].g;
     int ax = -1;
     double best = 0.0;
     for (int a = 0; a < 3; ++a) {
-      double size = (double)g.nb[a] / c.T[a];
-      if (c.T[a] < g.nb[a] && size > best) {
+      double size = (double)g.nb[a] / T[a];
+      if (T[a] < g.nb[a] && size > best) {
         best = size;
         ax = a;
       }
     }
     if (ax < 0) return EXAMD_ERR_TILE_OVERFLOW;
-    c.T[ax] += 1;
+    T[ax] += 1;
   }
   return EXAMD_ERR_TILE_OVERFLOW;
 }
@@ -263,6 +392,7 @@ ForceConfig force_config(const examd_handle *h, int prec) {
   c.maxrb = h->tc[prec].maxrb;
   c.groups = h->tc[prec].groups;
   c.lane_queue = h->tc[prec].lane_queue;
+  c.workgroups = h->tc[prec].workgroups;
   return c;
 }
 
@@ -301,7 +431,8 @@ void record_event(examd_handle *h, hipEvent_t e) {
                                            hipStreamSetCaptureDependencies);
 }
 
-// Force launch, bracketed by events when timing is on.
+// Force launch (tallies of system k into log[k]), bracketed by events when
+// timing is on.
 void forces_timed(examd_handle *h, int prec, bool energy, bool homecount,
                   StepLog *log) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -310,7 +441,7 @@ void forces_timed(examd_handle *h, int prec, bool energy, bool homecount,
     e1 = next_event(h);
     if (e0) record_event(h, e0);
   }
-  launch_forces(h->d, grid_for(h, prec), lj(h), force_config(h, prec),
+  launch_forces(h->d, table(h, prec), lj(h), force_config(h, prec),
                 prec == EXAMD_PREC_MIXED, energy, homecount, log, h->stream);
   if (e1) record_event(h, e1);
 }
@@ -328,35 +459,65 @@ void collect_timing(examd_handle *h) {
   h->ev_used = 0;
 }
 
-// Bin + sort + force evaluation at the current positions into log[slot].
+// Bin + sort + force evaluation at the current positions into log[0..nsys).
 void evaluate(examd_handle *h, int prec, bool energy, bool homecount,
               StepLog *log) {
-  launch_step_bin(h->d, h->g, prec == EXAMD_PREC_MIXED, kBinOnly, 0.0, 0.0,
-                  nullptr, log, h->stream);
+  launch_step_bin(h->d, table(h, prec), prec == EXAMD_PREC_MIXED, kBinOnly,
+                  0.0, 0.0, nullptr, log, h->stream);
   launch_sort(h->d, h->nblocks, h->stream);
   forces_timed(h, prec, energy, homecount, log);
 }
 
+// Kinetic energy of the current velocities into log[0..nsys) (h = 0 leaves
+// them unchanged).
+void kinetic(examd_handle *h, int prec, StepLog *log) {
+  if (h->nsys == 1)
+    launch_kick(h->d, prec == EXAMD_PREC_MIXED, 0.0, log, h->stream);
+  else
+    launch_sys_ke(h->d, table(h, prec), log, h->stream);
+}
+
 // Enqueue `count` velocity-Verlet steps on the handle's stream, tallies of
-// chunk step c in log[c]: kick-drift-bin, sort, force; then
+// chunk step c in log[c * nsys + k]: kick-drift-bin, sort, force; then
 // (finish-kick-drift-bin, sort, force) x (count - 1); then the closing kick.
+// A batch takes its per-system kinetic energies between a separate closing
+// kick and the next kick-drift-bin (the same arithmetic as the fused form).
 // Chunks of any length compose into the same arithmetic as one long run.
 template <class IsEnergy>
 void enqueue_steps(examd_handle *h, int prec, int64_t count, double dt,
                    IsEnergy is_energy) {
   const bool mixed = prec == EXAMD_PREC_MIXED;
   const double hk = (0.5 * dt) / h->p.mass;
-  launch_fill_log(h->log + 1, (int)count, h->stream);
+  const int ns = h->nsys;
+  const SysTable t = table(h, prec);
+  launch_fill_log(h->log + ns, (int)(count * ns), h->stream);
   for (int64_t c = 1; c <= count; ++c) {
-    StepLog *ke_prev = (c > 1 && is_energy(c - 1)) ? h->log + (c - 1) : nullptr;
-    launch_step_bin(h->d, h->g, mixed, c == 1 ? kKickDriftBin : kFinishKickDriftBin,
-                    hk, dt, ke_prev, h->log + c, h->stream);
+    StepLog *Lc = h->log + c * ns;
+    StepLog *Lp = h->log + (c - 1) * ns;
+    if (c == 1) {
+      launch_step_bin(h->d, t, mixed, kKickDriftBin, hk, dt, nullptr, Lc,
+                      h->stream);
+    } else if (ns == 1 || !is_energy(c - 1)) {
+      launch_step_bin(h->d, t, mixed, kFinishKickDriftBin, hk, dt,
+                      is_energy(c - 1) ? Lp : nullptr, Lc, h->stream);
+    } else {
+      launch_kick(h->d, mixed, hk, nullptr, h->stream);
+      launch_sys_ke(h->d, t, Lp, h->stream);
+      launch_step_bin(h->d, t, mixed, kKickDriftBin, hk, dt, nullptr, Lc,
+                      h->stream);
+    }
     launch_sort(h->d, h->nblocks, h->stream);
-    forces_timed(h, prec, is_energy(c), false, h->log + c);
+    forces_timed(h, prec, is_energy(c), false, Lc);
   }
-  if (count > 0)
-    launch_kick(h->d, mixed, hk, is_energy(count) ? h->log + count : nullptr,
-                h->stream);
+  if (count > 0) {
+    StepLog *Ln = h->log + count * ns;
+    if (ns == 1) {
+      launch_kick(h->d, mixed, hk, is_energy(count) ? Ln : nullptr, h->stream);
+    } else {
+      launch_kick(h->d, mixed, hk, nullptr, h->stream);
+      if (is_energy(count)) launch_sys_ke(h->d, t, Ln, h->stream);
+    }
+  }
 }
 
 bool same_key(const GraphKey &a, const GraphKey &b) {
@@ -364,17 +525,9 @@ bool same_key(const GraphKey &a, const GraphKey &b) {
       a.dt != b.dt || a.timing != b.timing)
     return false;
   const TileConfig &x = a.cfg, &y = b.cfg;
-  return x.T[0] == y.T[0] && x.T[1] == y.T[1] && x.T[2] == y.T[2] &&
-         x.threads == y.threads && x.groups == y.groups &&
-         x.lane_queue == y.lane_queue && x.cap == y.cap && x.maxrb == y.maxrb;
-}
-
-void drop_graph(examd_handle *h) {
-  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  h->graph_exec = nullptr;
-  h->graph = nullptr;
-  h->graph_events = 0;
+  return x.T == y.T && x.threads == y.threads && x.groups == y.groups &&
+         x.lane_queue == y.lane_queue && x.workgroups == y.workgroups &&
+         x.cap == y.cap && x.maxrb == y.maxrb;
 }
 
 // Capture one chunk of `key.chunk` steps into an executable graph (reused
@@ -410,6 +563,68 @@ int ensure_graph(examd_handle *h, const GraphKey &key) {
   return EXAMD_OK;
 }
 
+void fill_stats(const examd_handle *h, const StepLog &L, int prec,
+                int64_t step, examd_stats *s) {
+  double pe = prec == EXAMD_PREC_MIXED
+                  ? (double)(long long)L.pe_fix * kFixInv
+                  : L.pe;
+  double rc = h->p.cutoff, sr2 = h->p.sigma * h->p.sigma / (rc * rc);
+  double sr6 = sr2 * sr2 * sr2;
+  double vrc = 4.0 * h->p.epsilon * (sr6 * sr6 - sr6);
+  s->step = step;
+  s->ke = 0.5 * h->p.mass * L.ke;
+  s->pe = pe;
+  s->pe_shifted = pe - (double)L.cut * vrc;
+  s->etot = s->ke + pe;
+  s->pairs_cand = (int64_t)L.cand;
+  s->pairs_cut = (int64_t)L.cut;
+}
+
+int download_positions(examd_handle *h, std::vector<double> &aos) {
+  const int64_t n = h->n;
+  aos.assign((size_t)std::max<int64_t>(3 * n, 1), 0.0);
+  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
+  const double *src[3] = {h->d.x, h->d.y, h->d.z};
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int a = 0; a < 3; ++a) {
+    HIPCHK(hipMemcpy(buf.data(), src[a], n * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) aos[3 * i + a] = buf[i];
+  }
+  return EXAMD_OK;
+}
+
+// Apply a launch shape to one precision: the cell-block map (explicit T of
+// a single system, or every system's map from a tile edge), threads,
+// groups, queue depth and persistent grid.
+int apply_config(examd_handle *h, int prec, const int *T, double edge,
+                 int threads, int groups, int lane_queue, int workgroups,
+                 const double *pos) {
+  TileConfig saved = h->tc[prec];
+  TileConfig &c = h->tc[prec];
+  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 8 ? 8 : 16);
+  if (T) {
+    for (int a = 0; a < 3; ++a)
+      c.T[0][a] = std::max(1, std::min(T[a], h->sys[0].g.nb[a]));
+    c.edge = 0;
+  } else {
+    for (int k = 0; k < h->nsys; ++k) c.T[k] = tiles_from_edge(h->sys[k].g, edge);
+    c.edge = (int)std::lround(edge);
+  }
+  c.threads = threads;
+  c.workgroups = std::max(0, workgroups);
+  c.fixed_tiles = true;
+  c.fixed_groups = groups;
+  int st = choose_caps(h, prec, pos);
+  if (st == EXAMD_OK) st = ensure_partials(h);
+  if (st != EXAMD_OK) {
+    h->tc[prec] = saved;
+    (void)upload_tables(h, prec);
+  }
+  if (h->forces_valid == prec) h->forces_valid = -1;
+  return st;
+}
+
 uint64_t splitmix64(uint64_t &s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -418,6 +633,150 @@ uint64_t splitmix64(uint64_t &s) {
 }
 double uniform01(uint64_t &s) {  // (0, 1)
   return ((double)(splitmix64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Maxwell-Boltzmann velocities at temperature T, zero net momentum,
+// rescaled so that sum v^2 = (3n - 3) T / m exactly.
+void mb_velocities(int64_t n, double temperature, double mass, uint64_t &s,
+                   double *vel) {
+  const double sd = std::sqrt(temperature / mass);
+  for (int64_t t = 0; t < 3 * n; t += 2) {
+    // Box-Muller pairs
+    double u1 = uniform01(s), u2 = uniform01(s);
+    double r = std::sqrt(-2.0 * std::log(u1));
+    vel[t] = sd * r * std::cos(2.0 * M_PI * u2);
+    if (t + 1 < 3 * n) vel[t + 1] = sd * r * std::sin(2.0 * M_PI * u2);
+  }
+  double mom[3] = {0, 0, 0};
+  for (int64_t t = 0; t < n; ++t)
+    for (int a = 0; a < 3; ++a) mom[a] += vel[3 * t + a];
+  for (int a = 0; a < 3; ++a) mom[a] /= (double)n;
+  double v2 = 0.0;
+  for (int64_t t = 0; t < n; ++t)
+    for (int a = 0; a < 3; ++a) {
+      vel[3 * t + a] -= mom[a];
+      v2 += vel[3 * t + a] * vel[3 * t + a];
+    }
+  double target = (3.0 * n - 3.0) * temperature / mass;  // sum v^2
+  double scale = (v2 > 0.0 && n > 1) ? std::sqrt(target / v2) : 0.0;
+  for (int64_t t = 0; t < 3 * n; ++t) vel[t] *= scale;
+}
+
+// Shared set-up of examd_hip_create and examd_hip_batch_create.
+int create_handle(const examd_params *prm, const examd_system *systems,
+                  int nsys, bool allow_tiles, examd_handle **out) {
+  if (!prm || !out || !systems || nsys <= 0) return EXAMD_ERR_ARG;
+  *out = nullptr;
+  if (!(prm->cutoff > 0.0) || !(prm->mass > 0.0) || !(prm->sigma > 0.0) ||
+      !(prm->epsilon >= 0.0))
+    return EXAMD_ERR_ARG;
+  std::vector<SysHost> sys((size_t)nsys);
+  int64_t atoms = 0, blocks = 0;
+  for (int k = 0; k < nsys; ++k) {
+    if (systems[k].natoms < 0) return EXAMD_ERR_ARG;
+    int32_t nbxyz[3];
+    int64_t nb;
+    int st = examd_hip_domain(systems[k].box, 0.5 * prm->cutoff, nbxyz, &nb);
+    if (st) return st;
+    for (int a = 0; a < 3; ++a)
+      if (nbxyz[a] < 5 || !(prm->cutoff < 0.5 * systems[k].box[a]))
+        return EXAMD_ERR_GRID;
+    SysHost &S = sys[k];
+    S.atom0 = atoms;
+    S.n = systems[k].natoms;
+    S.block0 = blocks;
+    S.nblocks = nb;
+    for (int a = 0; a < 3; ++a) {
+      S.g.nb[a] = nbxyz[a];
+      S.g.T[a] = 1;
+      S.g.L[a] = systems[k].box[a];
+    }
+    S.g.bl = 0.5 * prm->cutoff;
+    atoms += S.n;
+    blocks += nb;
+    if (atoms > (int64_t)2000000000 || blocks > (int64_t)1 << 30)
+      return EXAMD_ERR_ARG;
+  }
+  const bool fixed = prm->tiles[0] > 0 || prm->tiles[1] > 0 || prm->tiles[2] > 0;
+  if (fixed && !allow_tiles) return EXAMD_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return EXAMD_ERR_NO_DEVICE;
+  if (prm->device < 0 || prm->device >= ndev) return EXAMD_ERR_ARG;
+
+  examd_handle *h = new (std::nothrow) examd_handle();
+  if (!h) return EXAMD_ERR_NOMEM;
+  h->p = *prm;
+  h->dev = prm->device;
+  h->nsys = nsys;
+  h->sys = sys;
+  h->n = atoms;
+  h->nblocks = blocks;
+  const int thr = prm->block_threads;
+  for (int prec = 0; prec < 2; ++prec) {
+    TileConfig &c = h->tc[prec];
+    c.fixed_tiles = fixed;
+    c.fixed_groups = prm->run_groups;
+    c.threads = (thr == 256 || thr == 512 || thr == 1024) ? thr : 512;
+    c.T.resize((size_t)nsys);
+    for (int k = 0; k < nsys; ++k)
+      for (int a = 0; a < 3; ++a) {
+        const int nb = sys[k].g.nb[a];
+        int T = prm->tiles[a] > 0 ? prm->tiles[a] : (nb + 6) / 7;
+        c.T[k][a] = std::max(1, std::min(T, nb));
+      }
+  }
+  h->graph_chunk = prm->graph_chunk > 0 ? prm->graph_chunk : 0;
+
+  if (hipSetDevice(h->dev) != hipSuccess) {
+    delete h;
+    return EXAMD_ERR_HIP;
+  }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            h->dev) != hipSuccess || lds <= 0)
+    lds = 65536;
+  // keep room for the kernels' static LDS (reduction scratch)
+  h->lds_limit = (size_t)lds > 4096 ? (size_t)lds - 4096 : (size_t)lds;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    delete h;
+    return EXAMD_ERR_HIP;
+  }
+  DevState &d = h->d;
+  d.n = h->n;
+  const int64_t n = h->n;
+  const int64_t nchunk = (blocks + 2047) / 2048;
+  int rc = EXAMD_OK;
+  rc |= dalloc(&d.x, n) | dalloc(&d.y, n) | dalloc(&d.z, n);
+  rc |= dalloc(&d.vx, n) | dalloc(&d.vy, n) | dalloc(&d.vz, n);
+  rc |= dalloc(&d.fx, n) | dalloc(&d.fy, n) | dalloc(&d.fz, n);
+  rc |= dalloc(&d.qx, n) | dalloc(&d.qy, n) | dalloc(&d.qz, n);
+  rc |= dalloc(&d.cell, n) | dalloc(&d.rank, n) | dalloc(&d.sorted, n);
+  rc |= dalloc(&d.count, blocks) | dalloc(&d.start, blocks + 1);
+  rc |= dalloc(&d.chunk, nchunk);
+  rc |= dalloc(&h->atom_seg, nsys + 1);
+  if (rc == EXAMD_OK) rc = ensure_log(h, 1);
+  if (rc != EXAMD_OK) {
+    free_state(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return EXAMD_ERR_NOMEM;
+  }
+  std::vector<long long> seg((size_t)nsys + 1);
+  for (int k = 0; k < nsys; ++k) seg[k] = (long long)sys[k].atom0;
+  seg[nsys] = (long long)atoms;
+  hipError_t e = hipMemcpy(h->atom_seg, seg.data(), seg.size() * sizeof(long long),
+                           hipMemcpyHostToDevice);
+  launch_fill_i32(d.count, 0, blocks, h->stream);
+  if (e != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
+    free_state(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return EXAMD_ERR_HIP;
+  }
+  *out = h;
+  return EXAMD_OK;
 }
 
 }  // namespace
@@ -463,92 +822,29 @@ int examd_hip_domain(const double box[3], double block_length,
 
 int examd_hip_create(const examd_params *prm, examd_handle **out) {
   if (!prm || !out) return EXAMD_ERR_ARG;
-  *out = nullptr;
   if (prm->natoms < 0 || prm->natoms > (int64_t)2000000000) return EXAMD_ERR_ARG;
-  if (!(prm->cutoff > 0.0) || !(prm->mass > 0.0) || !(prm->sigma > 0.0) ||
-      !(prm->epsilon >= 0.0))
+  examd_system one;
+  one.natoms = prm->natoms;
+  for (int a = 0; a < 3; ++a) one.box[a] = prm->box[a];
+  return create_handle(prm, &one, 1, true, out);
+}
+
+int examd_hip_batch_create(const examd_params *common,
+                           const examd_system *systems, int32_t nsys,
+                           examd_handle **out) {
+  if (!common || !out) return EXAMD_ERR_ARG;
+  if (common->natoms != 0 || common->box[0] != 0.0 || common->box[1] != 0.0 ||
+      common->box[2] != 0.0)
     return EXAMD_ERR_ARG;
-  int32_t nbxyz[3];
-  int64_t nblocks;
-  int st = examd_hip_domain(prm->box, 0.5 * prm->cutoff, nbxyz, &nblocks);
-  if (st) return st;
-  for (int a = 0; a < 3; ++a)
-    if (nbxyz[a] < 5 || !(prm->cutoff < 0.5 * prm->box[a]))
-      return EXAMD_ERR_GRID;
-  if (nblocks > (int64_t)1 << 30) return EXAMD_ERR_ARG;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-    return EXAMD_ERR_NO_DEVICE;
-  if (prm->device < 0 || prm->device >= ndev) return EXAMD_ERR_ARG;
+  return create_handle(common, systems, nsys, false, out);
+}
 
-  examd_handle *h = new (std::nothrow) examd_handle();
-  if (!h) return EXAMD_ERR_NOMEM;
-  h->p = *prm;
-  h->dev = prm->device;
-  h->n = prm->natoms;
-  h->nblocks = nblocks;
-  Grid &g = h->g;
-  const bool fixed = prm->tiles[0] > 0 || prm->tiles[1] > 0 || prm->tiles[2] > 0;
-  const int thr = prm->block_threads;
-  for (int prec = 0; prec < 2; ++prec) {
-    TileConfig &c = h->tc[prec];
-    c.fixed_tiles = fixed;
-    c.fixed_groups = prm->run_groups;
-    c.threads = (thr == 256 || thr == 512 || thr == 1024) ? thr : 512;
-  }
-  for (int a = 0; a < 3; ++a) {
-    g.nb[a] = nbxyz[a];
-    g.L[a] = prm->box[a];
-    int T = prm->tiles[a] > 0 ? prm->tiles[a] : (nbxyz[a] + 6) / 7;
-    T = std::max(1, std::min(T, (int)nbxyz[a]));
-    g.T[a] = T;
-    h->tc[0].T[a] = h->tc[1].T[a] = T;
-  }
-  g.bl = 0.5 * prm->cutoff;
-  h->graph_chunk = prm->graph_chunk > 0 ? prm->graph_chunk : 0;
-
-  if (hipSetDevice(h->dev) != hipSuccess) {
-    delete h;
-    return EXAMD_ERR_HIP;
-  }
-  int lds = 0;
-  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                            h->dev) != hipSuccess || lds <= 0)
-    lds = 65536;
-  // keep room for the kernels' static LDS (reduction scratch)
-  h->lds_limit = (size_t)lds > 4096 ? (size_t)lds - 4096 : (size_t)lds;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) !=
-      hipSuccess) {
-    delete h;
-    return EXAMD_ERR_HIP;
-  }
-  DevState &d = h->d;
-  d.n = h->n;
-  int64_t n = h->n;
-  int64_t nchunk = (nblocks + 2047) / 2048;
-  int rc = EXAMD_OK;
-  rc |= dalloc(&d.x, n) | dalloc(&d.y, n) | dalloc(&d.z, n);
-  rc |= dalloc(&d.vx, n) | dalloc(&d.vy, n) | dalloc(&d.vz, n);
-  rc |= dalloc(&d.fx, n) | dalloc(&d.fy, n) | dalloc(&d.fz, n);
-  rc |= dalloc(&d.qx, n) | dalloc(&d.qy, n) | dalloc(&d.qz, n);
-  rc |= dalloc(&d.cell, n) | dalloc(&d.rank, n) | dalloc(&d.sorted, n);
-  rc |= dalloc(&d.count, nblocks) | dalloc(&d.start, nblocks + 1);
-  rc |= dalloc(&d.chunk, nchunk);
-  if (rc == EXAMD_OK) rc = ensure_log(h, 2);
-  if (rc != EXAMD_OK) {
-    free_state(h);
-    (void)hipStreamDestroy(h->stream);
-    delete h;
-    return EXAMD_ERR_NOMEM;
-  }
-  launch_fill_i32(d.count, 0, nblocks, h->stream);
-  if (hipStreamSynchronize(h->stream) != hipSuccess) {
-    free_state(h);
-    (void)hipStreamDestroy(h->stream);
-    delete h;
-    return EXAMD_ERR_HIP;
-  }
-  *out = h;
+int examd_hip_batch_info(examd_handle *h, int32_t *nsys, int64_t *natoms,
+                         int64_t *nblocks) {
+  if (!h) return EXAMD_ERR_ARG;
+  if (nsys) *nsys = h->nsys;
+  if (natoms) *natoms = h->n;
+  if (nblocks) *nblocks = h->nblocks;
   return EXAMD_OK;
 }
 
@@ -575,8 +871,8 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
     if (st) return st;
   }
   if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
-  h->host_pos.assign(pos, pos + 3 * n);
   drop_graph(h);
+  HIPCHK(hipStreamSynchronize(h->stream));
   std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
   double *dst[3] = {h->d.x, h->d.y, h->d.z};
   double *vdst[3] = {h->d.vx, h->d.vy, h->d.vz};
@@ -629,42 +925,29 @@ int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
   return EXAMD_OK;
 }
 
-static void fill_stats(const examd_handle *h, const StepLog &L, int prec,
-                       int64_t step, examd_stats *s) {
-  double pe = prec == EXAMD_PREC_MIXED
-                  ? (double)(long long)L.pe_fix * kFixInv
-                  : L.pe;
-  double rc = h->p.cutoff, sr2 = h->p.sigma * h->p.sigma / (rc * rc);
-  double sr6 = sr2 * sr2 * sr2;
-  double vrc = 4.0 * h->p.epsilon * (sr6 * sr6 - sr6);
-  s->step = step;
-  s->ke = 0.5 * h->p.mass * L.ke;
-  s->pe = pe;
-  s->pe_shifted = pe - (double)L.cut * vrc;
-  s->etot = s->ke + pe;
-  s->pairs_cand = (int64_t)L.cand;
-  s->pairs_cut = (int64_t)L.cut;
-}
-
 int examd_hip_forces(examd_handle *h, int precision, double *forces,
                      examd_stats *stats) {
   if (!h || (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
     return EXAMD_ERR_ARG;
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   HIPCHK(hipSetDevice(h->dev));
-  launch_fill_log(h->log, 1, h->stream);
+  const int ns = h->nsys;
+  launch_fill_log(h->log, ns, h->stream);
   evaluate(h, precision, true, false, h->log);
-  // kinetic energy of the given velocities (h = 0 leaves v unchanged)
-  launch_kick(h->d, precision == EXAMD_PREC_MIXED, 0.0, h->log, h->stream);
+  kinetic(h, precision, h->log);
   HIPCHK(hipGetLastError());
-  StepLog L;
-  HIPCHK(hipMemcpyAsync(&L, h->log, sizeof(L), hipMemcpyDeviceToHost,
-                        h->stream));
+  std::vector<StepLog> L((size_t)ns);
+  HIPCHK(hipMemcpyAsync(L.data(), h->log, ns * sizeof(StepLog),
+                        hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   if (h->timing) collect_timing(h);
   h->forces_valid = precision;
-  int st = flags_to_status(L.flags);
-  if (stats) fill_stats(h, L, precision, 0, stats);
+  unsigned long long fl = 0;
+  for (int k = 0; k < ns; ++k) {
+    fl |= L[k].flags;
+    if (stats) fill_stats(h, L[k], precision, 0, &stats[k]);
+  }
+  int st = flags_to_status(fl);
   if (st != EXAMD_OK && st != EXAMD_ERR_ZERO_DISTANCE) {
     h->forces_valid = -1;
     return st;
@@ -692,6 +975,7 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   if (n_out) *n_out = 0;
   HIPCHK(hipSetDevice(h->dev));
+  const int ns = h->nsys;
   auto is_energy = [&](int64_t s) {
     return s == nsteps || (energy_every > 0 && s % energy_every == 0);
   };
@@ -706,9 +990,10 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
   const int64_t nchunks = C > 0 ? nsteps / C : 0;
   const int64_t done_by_graph = nchunks * C;
   const int64_t rem = nsteps - done_by_graph;
-  if (ensure_log(h, std::max<int64_t>(C, rem) + 1)) return EXAMD_ERR_NOMEM;
-  // log[0]: flags of the evaluation at the starting positions, if one runs
-  launch_fill_log(h->log, 1, h->stream);
+  if (ensure_log(h, std::max<int64_t>(C, rem))) return EXAMD_ERR_NOMEM;
+  // log[0..ns): flags of the evaluation at the starting positions, if one
+  // runs
+  launch_fill_log(h->log, ns, h->stream);
   if (h->forces_valid != precision) {
     const bool t = h->timing;   // only the step launches are timed
     h->timing = false;
@@ -720,20 +1005,24 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
   std::vector<StepLog> L;
   // read back the tallies of steps base+1..base+count and check flags
   auto harvest = [&](int64_t base, int64_t count) -> int {
-    L.resize((size_t)count + 1);
-    HIPCHK(hipMemcpyAsync(L.data(), h->log, (count + 1) * sizeof(StepLog),
+    L.resize((size_t)((count + 1) * ns));
+    HIPCHK(hipMemcpyAsync(L.data(), h->log, (count + 1) * ns * sizeof(StepLog),
                           hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (h->timing) collect_timing(h);
-    fl |= L[0].flags;
+    for (int q = 0; q < ns; ++q) fl |= L[q].flags;
     for (int64_t c = 1; c <= count; ++c) {
-      fl |= L[c].flags;
       const int64_t s = base + c;
-      if (out && is_energy(s) && k < out_cap)
-        fill_stats(h, L[c], precision, s, &out[k++]);
+      for (int q = 0; q < ns; ++q) {
+        const StepLog &Lq = L[(size_t)(c * ns + q)];
+        fl |= Lq.flags;
+        if (out && is_energy(s) && k < out_cap)
+          fill_stats(h, Lq, precision, s, &out[k++]);
+      }
     }
     return EXAMD_OK;
   };
+  const unsigned long long fatal = kFlagOverflow | kFlagOutOfBox | kFlagNonFinite;
   if (nchunks > 0) {
     GraphKey key{precision, C, energy_every > 0 ? energy_every : 0, dt,
                  h->timing, h->tc[precision]};
@@ -744,10 +1033,10 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
       h->ev_used = h->graph_events;   // the events the replay recorded
       st = harvest(c * C, C);
       if (st) return st;
-      if (fl & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite)) break;
+      if (fl & fatal) break;
     }
   }
-  if (rem > 0 && !(fl & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite))) {
+  if (rem > 0 && !(fl & fatal)) {
     enqueue_steps(h, precision, rem, dt,
                   [&](int64_t c) { return is_energy(done_by_graph + c); });
     HIPCHK(hipGetLastError());
@@ -773,8 +1062,8 @@ int examd_hip_block_ids(examd_handle *h, int64_t *blocknum, int32_t *blockxyz,
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   HIPCHK(hipSetDevice(h->dev));
   launch_fill_log(h->log, 1, h->stream);
-  launch_step_bin(h->d, h->g, false, kBinOnly, 0.0, 0.0, nullptr, h->log,
-                  h->stream);
+  launch_step_bin(h->d, table(h, 0), false, kBinOnly, 0.0, 0.0, nullptr,
+                  h->log, h->stream);
   // clear the counts the binning pass left behind
   launch_fill_i32(h->d.count, 0, h->nblocks, h->stream);
   HIPCHK(hipGetLastError());
@@ -786,14 +1075,17 @@ int examd_hip_block_ids(examd_handle *h, int64_t *blocknum, int32_t *blockxyz,
                         h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->forces_valid = -1;  // force accumulators were cleared
-  const int64_t nx = h->g.nb[0], ny = h->g.nb[1];
-  for (int64_t i = 0; i < n; ++i) {
-    int64_t b = cell[i];
-    blocknum[i] = b + 1;
-    if (blockxyz) {
-      blockxyz[3 * i + 0] = (int32_t)(b % nx + 1);
-      blockxyz[3 * i + 1] = (int32_t)((b / nx) % ny + 1);
-      blockxyz[3 * i + 2] = (int32_t)(b / (nx * ny) + 1);
+  for (int k = 0; k < h->nsys; ++k) {
+    const SysHost &S = h->sys[k];
+    const int64_t nx = S.g.nb[0], ny = S.g.nb[1];
+    for (int64_t i = S.atom0; i < S.atom0 + S.n; ++i) {
+      const int64_t b = cell[i] - S.block0;
+      blocknum[i] = b + 1;
+      if (blockxyz) {
+        blockxyz[3 * i + 0] = (int32_t)(b % nx + 1);
+        blockxyz[3 * i + 1] = (int32_t)((b / nx) % ny + 1);
+        blockxyz[3 * i + 2] = (int32_t)(b / (nx * ny) + 1);
+      }
     }
   }
   return flags_to_status(L.flags & (kFlagOutOfBox | kFlagNonFinite));
@@ -807,69 +1099,35 @@ int examd_hip_home_pair_counts(examd_handle *h, int precision, int64_t *counts,
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   HIPCHK(hipSetDevice(h->dev));
   if (!h->d.home && dalloc(&h->d.home, h->nblocks)) return EXAMD_ERR_NOMEM;
+  const int ns = h->nsys;
   launch_fill_u64(h->d.home, h->nblocks, h->stream);
-  launch_fill_log(h->log, 1, h->stream);
+  launch_fill_log(h->log, ns, h->stream);
   evaluate(h, precision, false, true, h->log);
   HIPCHK(hipGetLastError());
-  StepLog L;
+  std::vector<StepLog> L((size_t)ns);
   HIPCHK(hipMemcpyAsync(counts, h->d.home, nblocks * sizeof(int64_t),
                         hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(&L, h->log, sizeof(L), hipMemcpyDeviceToHost,
-                        h->stream));
+  HIPCHK(hipMemcpyAsync(L.data(), h->log, ns * sizeof(StepLog),
+                        hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   if (h->timing) collect_timing(h);
   h->forces_valid = precision;
-  return flags_to_status(L.flags & ~(unsigned long long)kFlagZero);
+  unsigned long long fl = 0;
+  for (int k = 0; k < ns; ++k) fl |= L[k].flags;
+  return flags_to_status(fl & ~(unsigned long long)kFlagZero);
 }
-
-}  // extern "C"
-
-namespace {
-
-int download_positions(examd_handle *h, std::vector<double> &aos) {
-  const int64_t n = h->n;
-  aos.assign((size_t)std::max<int64_t>(3 * n, 1), 0.0);
-  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
-  const double *src[3] = {h->d.x, h->d.y, h->d.z};
-  HIPCHK(hipStreamSynchronize(h->stream));
-  for (int a = 0; a < 3; ++a) {
-    HIPCHK(hipMemcpy(buf.data(), src[a], n * sizeof(double),
-                     hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i) aos[3 * i + a] = buf[i];
-  }
-  return EXAMD_OK;
-}
-
-// Apply a launch shape to one precision (cell-block map, threads, groups).
-int apply_config(examd_handle *h, int prec, const int T[3], int threads,
-                 int groups, int lane_queue, const double *pos) {
-  TileConfig saved = h->tc[prec];
-  TileConfig &c = h->tc[prec];
-  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 8 ? 8 : 16);
-  for (int a = 0; a < 3; ++a) c.T[a] = std::max(1, std::min(T[a], h->g.nb[a]));
-  c.threads = threads;
-  c.fixed_tiles = true;
-  c.fixed_groups = groups;
-  int st = choose_caps(h, prec, pos);
-  if (st == EXAMD_OK) st = ensure_partials(h);
-  if (st != EXAMD_OK) h->tc[prec] = saved;
-  if (h->forces_valid == prec) h->forces_valid = -1;
-  return st;
-}
-
-}  // namespace
-
-extern "C" {
 
 int examd_hip_get_config(examd_handle *h, int precision, examd_tune *out) {
   if (!h || !out ||
       (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
     return EXAMD_ERR_ARG;
   const TileConfig &c = h->tc[precision];
-  for (int a = 0; a < 3; ++a) out->tiles[a] = c.T[a];
+  for (int a = 0; a < 3; ++a) out->tiles[a] = c.T[0][a];
   out->block_threads = c.threads;
   out->run_groups = c.groups;
   out->lane_queue = c.lane_queue;
+  out->tile_edge = c.edge;
+  out->workgroups = c.workgroups;
   out->force_ms = 0.0;
   return EXAMD_OK;
 }
@@ -884,14 +1142,22 @@ int examd_hip_set_config(examd_handle *h, int precision,
   if (thr != 256 && thr != 512 && thr != 1024) return EXAMD_ERR_ARG;
   if (cfg->run_groups < 0 || cfg->run_groups > 4) return EXAMD_ERR_ARG;
   if (cfg->lane_queue < 0 || cfg->lane_queue > 16) return EXAMD_ERR_ARG;
-  for (int a = 0; a < 3; ++a)
-    if (cfg->tiles[a] <= 0) return EXAMD_ERR_ARG;
+  if (cfg->workgroups < 0) return EXAMD_ERR_ARG;
+  const bool explicit_tiles =
+      cfg->tiles[0] > 0 && cfg->tiles[1] > 0 && cfg->tiles[2] > 0;
+  if (explicit_tiles) {
+    if (h->nsys != 1) return EXAMD_ERR_ARG;
+  } else if (cfg->tile_edge <= 0 || cfg->tiles[0] != 0 || cfg->tiles[1] != 0 ||
+             cfg->tiles[2] != 0) {
+    return EXAMD_ERR_ARG;
+  }
   HIPCHK(hipSetDevice(h->dev));
   std::vector<double> pos;
   int st = download_positions(h, pos);
   if (st) return st;
-  return apply_config(h, precision, cfg->tiles, thr, cfg->run_groups,
-                      cfg->lane_queue, pos.data());
+  return apply_config(h, precision, explicit_tiles ? cfg->tiles : nullptr,
+                      (double)cfg->tile_edge, thr, cfg->run_groups,
+                      cfg->lane_queue, cfg->workgroups, pos.data());
 }
 
 int examd_hip_autotune(examd_handle *h, int precision, int trials,
@@ -904,62 +1170,107 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
   std::vector<double> pos;
   int st = download_positions(h, pos);
   if (st) return st;
-  // candidate cell-block sizes (home blocks per side) and launch shapes
-  static const double sizes[] = {3.5, 4.0, 4.5, 5.0, 5.5, 6.0, 6.7, 7.5, 8.5};
+  // candidate cell-block sizes (home blocks per side) and launch shapes; a
+  // single system takes fractional edges as explicit maps, a batch integer
+  // edges for every system
+  static const double sizes1[] = {3.5, 4.0, 4.5, 5.0, 5.5, 6.0, 6.7, 7.5, 8.5};
+  static const double sizesN[] = {3.0, 4.0, 5.0, 6.0, 7.0, 8.0};
+  const bool single = h->nsys == 1;
+  const double *sizes = single ? sizes1 : sizesN;
+  const int nsizes = single ? 9 : 6;
   static const int threads[] = {256, 512, 1024};
   const bool save_timing = h->timing;
   const double save_ms = h->force_ms;
   const int64_t save_n = h->force_launches;
   const TileConfig start = h->tc[precision];
+  const int ns = h->nsys;
   TileConfig bestc = start;
   double best_ms = 1e300;
-  for (double sz : sizes)
+  // time `trials` evaluations of the current shape (1e300 if it overflowed)
+  auto time_shape = [&](double *ms) -> int {
+    launch_fill_log(h->log, ns, h->stream);
+    evaluate(h, precision, false, false, h->log);  // warm-up
+    h->timing = true;
+    h->ev_used = 0;
+    h->force_ms = 0.0;
+    h->force_launches = 0;
+    for (int t = 0; t < trials; ++t) {
+      launch_fill_log(h->log, ns, h->stream);
+      evaluate(h, precision, false, false, h->log);
+    }
+    hipError_t e = hipStreamSynchronize(h->stream);
+    collect_timing(h);
+    std::vector<StepLog> L((size_t)ns);
+    if (e == hipSuccess)
+      e = hipMemcpy(L.data(), h->log, ns * sizeof(StepLog), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return EXAMD_ERR_HIP;
+    unsigned long long fl = 0;
+    for (int k = 0; k < ns; ++k) fl |= L[k].flags;
+    *ms = (fl & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite))
+              ? 1e300
+              : h->force_ms / std::max<int64_t>(h->force_launches, 1);
+    return EXAMD_OK;
+  };
+  auto consider = [&](const int *T, double edge, int thr, int grp, int lq,
+                      int wgs) -> int {
+    if (apply_config(h, precision, T, edge, thr, grp, lq, wgs, pos.data()) !=
+        EXAMD_OK)
+      return EXAMD_OK;
+    double ms;
+    int e = time_shape(&ms);
+    if (e) return e;
+    if (ms < best_ms) {
+      best_ms = ms;
+      bestc = h->tc[precision];
+    }
+    return EXAMD_OK;
+  };
+  for (int si = 0; si < nsizes; ++si)
     for (int thr : threads)
       for (int lq = 0; lq <= 16; lq += 8)
-      for (int grp = 0; grp <= (thr == 1024 ? 3 : 1); ++grp) {
-        if (grp == 1) continue;  // 0 = automatic choice covers one group
-        int T[3];
-        for (int a = 0; a < 3; ++a)
-          T[a] = std::max(1, (int)std::lround(h->g.nb[a] / sz));
-        if (apply_config(h, precision, T, thr, grp, lq, pos.data()) != EXAMD_OK)
-          continue;
-        launch_fill_log(h->log, 1, h->stream);
-        evaluate(h, precision, false, false, h->log);  // warm-up
-        h->timing = true;
-        h->ev_used = 0;
-        h->force_ms = 0.0;
-        h->force_launches = 0;
-        for (int t = 0; t < trials; ++t) {
-          launch_fill_log(h->log, 1, h->stream);
-          evaluate(h, precision, false, false, h->log);
+        for (int grp = 0; grp <= (thr == 1024 ? 3 : 1); ++grp) {
+          if (grp == 1) continue;  // 0 = automatic choice covers one group
+          int T[3];
+          if (single) {
+            for (int a = 0; a < 3; ++a)
+              T[a] = std::max(1, (int)std::lround(h->sys[0].g.nb[a] / sizes[si]));
+          }
+          st = consider(single ? T : nullptr, sizes[si], thr, grp, lq, 0);
+          if (st) {
+            h->timing = save_timing;
+            return st;
+          }
         }
-        hipError_t e = hipStreamSynchronize(h->stream);
-        collect_timing(h);
-        StepLog L;
-        if (e == hipSuccess)
-          e = hipMemcpy(&L, h->log, sizeof(L), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-          h->timing = save_timing;
-          return EXAMD_ERR_HIP;
-        }
-        if (L.flags & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite))
-          continue;
-        const double ms = h->force_ms / std::max<int64_t>(h->force_launches, 1);
-        if (ms < best_ms) {
-          best_ms = ms;
-          bestc = h->tc[precision];
-        }
+  // persistent grids for the best shape: a few workgroups per CU
+  if (best_ms < 1e300) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              h->dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    const TileConfig b = bestc;
+    for (int per = 1; per <= 4; per *= 2) {
+      const int wgs = cus * per;
+      if (wgs >= (int)tiles_of(b)) break;
+      st = consider(b.edge == 0 ? b.T[0].data() : nullptr, (double)b.edge,
+                    b.threads, b.fixed_groups, b.lane_queue, wgs);
+      if (st) {
+        h->timing = save_timing;
+        return st;
       }
+    }
+  }
   h->timing = save_timing;
   h->force_ms = save_ms;
   h->force_launches = save_n;
   h->ev_used = 0;
   if (best_ms >= 1e300) {
     h->tc[precision] = start;
+    (void)upload_tables(h, precision);
     return EXAMD_ERR_TILE_OVERFLOW;
   }
-  st = apply_config(h, precision, bestc.T, bestc.threads, bestc.fixed_groups,
-                    bestc.lane_queue, pos.data());
+  st = apply_config(h, precision, bestc.edge == 0 ? bestc.T[0].data() : nullptr,
+                    (double)bestc.edge, bestc.threads, bestc.fixed_groups,
+                    bestc.lane_queue, bestc.workgroups, pos.data());
   h->forces_valid = -1;
   if (st) return st;
   if (best) {
@@ -980,7 +1291,7 @@ int examd_hip_phase_profile(examd_handle *h, int precision,
   unsigned long long *buf = nullptr;
   if (dalloc(&buf, G * kProfSlots)) return EXAMD_ERR_NOMEM;
   h->d.prof = buf;
-  launch_fill_log(h->log, 1, h->stream);
+  launch_fill_log(h->log, h->nsys, h->stream);
   evaluate(h, precision, false, false, h->log);
   h->d.prof = nullptr;
   std::vector<unsigned long long> t((size_t)(G * kProfSlots));
@@ -1045,30 +1356,7 @@ int examd_hip_generate_fcc(const int32_t nc[3], double rho, double temperature,
             pos[3 * i + a] = x;
           }
         }
-  const int64_t n = i;
-  if (vel) {
-    const double sd = std::sqrt(temperature / mass);
-    for (int64_t t = 0; t < 3 * n; t += 2) {
-      // Box-Muller pairs
-      double u1 = uniform01(s), u2 = uniform01(s);
-      double r = std::sqrt(-2.0 * std::log(u1));
-      vel[t] = sd * r * std::cos(2.0 * M_PI * u2);
-      if (t + 1 < 3 * n) vel[t + 1] = sd * r * std::sin(2.0 * M_PI * u2);
-    }
-    double mom[3] = {0, 0, 0};
-    for (int64_t t = 0; t < n; ++t)
-      for (int a = 0; a < 3; ++a) mom[a] += vel[3 * t + a];
-    for (int a = 0; a < 3; ++a) mom[a] /= (double)n;
-    double v2 = 0.0;
-    for (int64_t t = 0; t < n; ++t)
-      for (int a = 0; a < 3; ++a) {
-        vel[3 * t + a] -= mom[a];
-        v2 += vel[3 * t + a] * vel[3 * t + a];
-      }
-    double target = (3.0 * n - 3.0) * temperature / mass;  // sum v^2
-    double scale = (v2 > 0.0 && n > 1) ? std::sqrt(target / v2) : 0.0;
-    for (int64_t t = 0; t < 3 * n; ++t) vel[t] *= scale;
-  }
+  if (vel) mb_velocities(i, temperature, mass, s, vel);
   return EXAMD_OK;
 }
 

@@ -128,6 +128,16 @@ struct Grid {
   double bl;        // block length rc/2
 };
 
+// One independent periodic system of a launch.  Systems are concatenated:
+// atoms [atom0, atom0 + n), global blocks [block0, block0 + nb^3), tiles
+// [tile0, tile0 + T^3) of the tile map of one precision.
+struct SysDesc {
+  Grid g;
+  long long atom0;
+  int block0;
+  int tile0;
+};
+
 // Home-block range [lo, hi) of tile t along one axis: nb blocks shared by T
 // tiles as evenly as possible.
 __host__ __device__ inline void tile_range(int t, int nb, int T, int &lo,

@@ -52,10 +52,10 @@ __device__ __forceinline__ void add_partial(Partial &a, const Partial &b) {
   a.cut += b.cut;
 }
 
-// Workgroup sum of one Partial per thread, stored at out[blockIdx.x] (all
+// Workgroup sum of one Partial per thread, stored at out[slot] (all
 // threads must call it; the order of the sums is fixed).
 template <int NT>
-__device__ void write_partial(Partial v, Partial *out) {
+__device__ void write_partial(Partial v, Partial *out, int slot) {
   __shared__ Partial s_p[NT / 64];
   v.ke = wave_sum(v.ke);
   v.pe = wave_sum(v.pe);
@@ -67,21 +67,27 @@ __device__ void write_partial(Partial v, Partial *out) {
   if (threadIdx.x == 0) {
     Partial t = s_p[0];
     for (int w = 1; w < NT / 64; ++w) add_partial(t, s_p[w]);
-    out[blockIdx.x] = t;
+    out[slot] = t;
   }
 }
 
 enum : unsigned { kSumKE = 1u, kSumForce = 2u };
 
-// Sum n partials in index order into one step's log.
+// Sum partials in index order into a step's log: one workgroup per
+// segment k, partials [seg[k], seg[k+1]) into log[k] (seg == nullptr: one
+// segment [0, n)).
 constexpr int kReduceThreads = 256;
 __global__ __launch_bounds__(kReduceThreads) void k_reduce_partials(
-    const Partial *p, int n, StepLog *log, unsigned what) {
+    const Partial *p, const int *seg, int n, StepLog *log, unsigned what) {
+  const int k = blockIdx.x;
+  const int i0 = seg ? seg[k] : 0, i1 = seg ? seg[k + 1] : n;
   Partial acc{0.0, 0.0, 0ull, 0ull, 0ull};
-  for (int i = threadIdx.x; i < n; i += kReduceThreads) add_partial(acc, p[i]);
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += kReduceThreads)
+    add_partial(acc, p[i]);
   __shared__ Partial s_tot;
-  write_partial<kReduceThreads>(acc, &s_tot);  // one workgroup: index 0
+  write_partial<kReduceThreads>(acc, &s_tot, 0);
   __syncthreads();
+  log += k;
   if (threadIdx.x == 0) {
     if (what & kSumKE) log->ke += s_tot.ke;
     if (what & kSumForce) {
@@ -111,13 +117,27 @@ __global__ void k_fill_log(StepLog *p, int n) {
 // One thread per atom (input order).  mode kFinishKickDriftBin closes the
 // previous step (v += h F, kinetic energy) and opens the next one; all modes
 // zero the force accumulators and assign the block.
+// System of atom i: the last k with atom_seg[k] <= i.
+__device__ __forceinline__ int system_of(const SysTable &t, int64_t i) {
+  int lo = 0, hi = t.nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.atom_seg[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <bool MIXED, int MODE>
 __global__ __launch_bounds__(kBinThreads) void k_step_bin(
-    DevState d, Grid g, double h, double dt, int want_ke, StepLog *flag_log) {
+    DevState d, SysTable t, double h, double dt, int want_ke,
+    StepLog *flag_log) {
   const int64_t i = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
   double ke = 0.0;
   unsigned flags = 0;
   if (i < d.n) {
+    const SysDesc &sd = t.sys[t.nsys > 1 ? system_of(t, i) : 0];
+    const Grid g = sd.g;
     double X = d.x[i], Y = d.y[i], Z = d.z[i];
     if constexpr (MODE != kBinOnly) {
       double Fx, Fy, Fz;
@@ -175,14 +195,14 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
                       (long long)(bz - 1) * ((long long)g.nb[0] * g.nb[1]);
       long long nblk = (long long)g.nb[0] * g.nb[1] * g.nb[2];
       if (num < 1 || num > nblk) flags |= kFlagOutOfBox;
-      else cell = (int)(num - 1);
+      else cell = sd.block0 + (int)(num - 1);
     }
     d.cell[i] = cell;
     d.rank[i] = atomicAdd(&d.count[cell], 1);
   }
   if constexpr (MODE == kFinishKickDriftBin) {
     if (want_ke) write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull},
-                                            d.part);
+                                            d.part, blockIdx.x);
   }
   flags = wave_or(flags);
   if ((threadIdx.x & 63) == 0 && flags)
@@ -214,7 +234,26 @@ __global__ __launch_bounds__(kBinThreads) void k_kick(DevState d, double h,
     ke = Vx * Vx + Vy * Vy + Vz * Vz;
   }
   if (want_ke)
-    write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull}, d.part);
+    write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull}, d.part,
+                               blockIdx.x);
+}
+
+// Sum of v^2 over the atoms of system blockIdx.x into log[blockIdx.x].ke,
+// in a fixed order.
+__global__ __launch_bounds__(kBinThreads) void k_sys_ke(DevState d,
+                                                        SysTable t,
+                                                        StepLog *log) {
+  const int k = blockIdx.x;
+  const long long a0 = t.atom_seg[k], a1 = t.atom_seg[k + 1];
+  double ke = 0.0;
+  for (long long i = a0 + threadIdx.x; i < a1; i += kBinThreads) {
+    const double Vx = d.vx[i], Vy = d.vy[i], Vz = d.vz[i];
+    ke += Vx * Vx + Vy * Vy + Vz * Vz;
+  }
+  __shared__ Partial s_tot;
+  write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull}, &s_tot, 0);
+  __syncthreads();
+  if (threadIdx.x == 0) log[k].ke += s_tot.ke;
 }
 
 // ------------------------------------------------------------- scan ----
@@ -331,7 +370,7 @@ __global__ void k_block_order(DevState d, int64_t nblocks) {
 // ----------------------------------------------------------- forces ----
 struct ForceArgs {
   DevState d;
-  Grid g;
+  SysTable t;
   StepLog *log;
   int cap, maxrb, groups, lane_queue;
   double rc2, eps24, eps4, sig2, d2zero, d2fix;
@@ -509,17 +548,21 @@ __device__ __forceinline__ int bc_s(int bc, int axis) {
   return ((bc >> (24 + 2 * axis)) & 3) - 1;
 }
 
+// One cell block (tile) of one system, evaluated by the whole workgroup.
 template <bool FP64, bool ENERGY, bool HOMECNT, int NT>
-__global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
+                                           unsigned char *smem) {
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x;
-  const Grid &g = A.g;
-  const int G = g.T[0] * g.T[1] * g.T[2];
-  const int tile = xcd_remap(blockIdx.x, G);
-  const int tI = tile % g.T[0];
-  const int tJ = (tile / g.T[0]) % g.T[1];
-  const int tK = tile / (g.T[0] * g.T[1]);
+  const int sys = A.t.nsys > 1 ? A.t.tile_sys[tile] : 0;
+  const SysDesc &sd = A.t.sys[sys];
+  const Grid g = sd.g;
+  const int block0 = sd.block0;
+  StepLog *const log = A.log + sys;
+  const int lt = tile - sd.tile0;
+  const int tI = lt % g.T[0];
+  const int tJ = (lt / g.T[0]) % g.T[1];
+  const int tK = lt / (g.T[0] * g.T[1]);
   int h0x, h1x, h0y, h1y, h0z, h1z;
   tile_range(tI, g.nb[0], g.T[0], h0x, h1x);
   tile_range(tJ, g.nb[1], g.T[1], h0y, h1y);
@@ -549,7 +592,7 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   long long *s_qy = s_qx + cap4;
   long long *s_qz = s_qy + cap4;
   unsigned long long *prof =
-      A.d.prof ? A.d.prof + (size_t)blockIdx.x * kProfSlots : nullptr;
+      A.d.prof ? A.d.prof + (size_t)tile * kProfSlots : nullptr;
   if (prof && tid == 0) prof[0] = clock64();
 
   // 1. region blocks: home blocks of the tile plus a one-block ghost shell,
@@ -563,7 +606,7 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
     gy -= gy >= nby ? nby : 0;
     gz += gz < 0 ? nbz : 0;
     gz -= gz >= nbz ? nbz : 0;
-    int gb = gx + nbx * (gy + nby * gz);
+    int gb = block0 + gx + nbx * (gy + nby * gz);
     int st = A.d.start[gb];
     s_off[rb] = A.d.start[gb + 1] - st;
     s_gst[rb] = st;
@@ -590,8 +633,8 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   if (prof && tid == 0) prof[1] = clock64();
   if (R > A.cap) {
     if (tid == 0) {
-      atomicOr(&A.log->flags, (unsigned long long)kFlagOverflow);
-      A.d.part[blockIdx.x] = Partial{0.0, 0.0, 0ull, 0ull, 0ull};
+      atomicOr(&log->flags, (unsigned long long)kFlagOverflow);
+      A.d.part[tile] = Partial{0.0, 0.0, 0ull, 0ull, 0ull};
       if (prof) prof[2] = prof[3] = prof[4] = prof[1];
     }
     return;
@@ -767,7 +810,7 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
         const int hx = h0x - 1 + ix - ((int)(o & 3) - 1);
         const int hy = h0y - 1 + iy - ((int)((o >> 2) & 3) - 1);
         const int hz = h0z - 1 + iz - ((int)((o >> 4) & 3) - 1);
-        atomicAdd(&A.d.home[hx + nbx * (hy + nby * hz)], 1ull);
+        atomicAdd(&A.d.home[block0 + hx + nbx * (hy + nby * hz)], 1ull);
       }
     };
     // evaluate a queued pair (a, j) from positions
@@ -936,14 +979,29 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
 
   flags = wave_or(flags);
   if ((tid & 63) == 0 && flags)
-    atomicOr(&A.log->flags, (unsigned long long)flags);
+    atomicOr(&log->flags, (unsigned long long)flags);
   Partial mine{0.0, 0.0, 0ull, cand, cut};
   if constexpr (ENERGY) {
     if constexpr (FP64) mine.pe = pe64;
     else mine.pe_fix = (unsigned long long)pefix;
   }
-  write_partial<NT>(mine, A.d.part);
+  write_partial<NT>(mine, A.d.part, tile);
   if (prof && tid == 0) prof[4] = clock64();
+}
+
+// Force kernel: workgroup b evaluates tiles b, b + gridDim, ... (one tile
+// per workgroup when the grid covers every tile; a persistent grid loops).
+// Tiles are taken in the XCD-aware order, so the tiles of one XCD's
+// workgroups are contiguous in space (the grid is a multiple of 8 when it
+// loops).
+template <bool FP64, bool ENERGY, bool HOMECNT, int NT>
+__global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ntiles = A.t.ntiles;
+  for (int wg = blockIdx.x; wg < ntiles; wg += gridDim.x) {
+    if (wg != (int)blockIdx.x) __syncthreads();  // LDS of the last tile
+    force_tile<FP64, ENERGY, HOMECNT, NT>(A, xcd_remap(wg, ntiles), smem);
+  }
 }
 
 __global__ void k_forces_to_fp64(DevState d, double *out, bool mixed) {
@@ -1000,14 +1058,14 @@ int64_t partial_slots(int64_t n, int64_t tiles) {
   return b > tiles ? b : tiles;
 }
 
-void launch_step_bin(const DevState &d, const Grid &g, bool mixed, int mode,
-                     double h, double dt, StepLog *ke_log, StepLog *flag_log,
-                     hipStream_t s) {
+void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
+                     int mode, double h, double dt, StepLog *ke_log,
+                     StepLog *flag_log, hipStream_t s) {
   unsigned nb = nblk(d.n, kBinThreads);
   if (nb == 0) return;
   const int want = (mode == kFinishKickDriftBin && ke_log) ? 1 : 0;
 #define EXAMD_BIN(M, MD)                                               \
-  k_step_bin<M, MD><<<nb, kBinThreads, 0, s>>>(d, g, h, dt, want, flag_log)
+  k_step_bin<M, MD><<<nb, kBinThreads, 0, s>>>(d, t, h, dt, want, flag_log)
   if (mixed) {
     if (mode == kBinOnly) EXAMD_BIN(true, kBinOnly);
     else if (mode == kKickDriftBin) EXAMD_BIN(true, kKickDriftBin);
@@ -1019,8 +1077,8 @@ void launch_step_bin(const DevState &d, const Grid &g, bool mixed, int mode,
   }
 #undef EXAMD_BIN
   if (want)
-    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, (int)nb, ke_log,
-                                                    kSumKE);
+    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, nullptr, (int)nb,
+                                                    ke_log, kSumKE);
 }
 
 void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
@@ -1031,8 +1089,13 @@ void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
   if (mixed) k_kick<true><<<nb, kBinThreads, 0, s>>>(d, h, want);
   else k_kick<false><<<nb, kBinThreads, 0, s>>>(d, h, want);
   if (want)
-    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, (int)nb, ke_log,
-                                                    kSumKE);
+    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, nullptr, (int)nb,
+                                                    ke_log, kSumKE);
+}
+
+void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
+                   hipStream_t s) {
+  if (t.nsys > 0) k_sys_ke<<<t.nsys, kBinThreads, 0, s>>>(d, t, log);
 }
 
 void launch_sort(const DevState &d, int64_t nblocks, hipStream_t s) {
@@ -1045,12 +1108,13 @@ void launch_sort(const DevState &d, int64_t nblocks, hipStream_t s) {
   k_block_order<<<nblk(nblocks, 256), 256, 0, s>>>(d, nblocks);
 }
 
-void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
+void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                    const ForceConfig &c, bool mixed, bool energy,
                    bool homecount, StepLog *log, hipStream_t s) {
+  if (t.ntiles <= 0) return;
   ForceArgs A;
   A.d = d;
-  A.g = g;
+  A.t = t;
   A.log = log;
   A.cap = c.cap;
   A.maxrb = c.maxrb;
@@ -1075,7 +1139,10 @@ void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
   A.d2fixf = (float)A.d2fix;
   const bool fp64 = !mixed;
   const size_t lds = force_lds_bytes(fp64, c.cap, c.maxrb, c.threads, c.lane_queue);
-  const int G = g.T[0] * g.T[1] * g.T[2];
+  // persistent grids are multiples of 8 (one tile order per XCD)
+  int G = t.ntiles;
+  if (c.workgroups > 0 && c.workgroups < G) G = (c.workgroups + 7) / 8 * 8;
+  if (G > t.ntiles) G = t.ntiles;
 #define EXAMD_F(F64, E, H) launch_forces_t<F64, E, H>(A, c.threads, lds, G, s)
   if (homecount) {
     if (fp64) EXAMD_F(true, false, true);
@@ -1088,7 +1155,8 @@ void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
     else EXAMD_F(false, false, false);
   }
 #undef EXAMD_F
-  k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, G, log, kSumForce);
+  k_reduce_partials<<<t.nsys, kReduceThreads, 0, s>>>(
+      d.part, t.nsys > 1 ? t.tile_seg : nullptr, t.ntiles, log, kSumForce);
 }
 
 void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,

@@ -30,6 +30,17 @@ constexpr int kProfSlots = 5;   // timestamps per cell block
 // Partial slots needed by a launch sequence over n atoms and G tiles.
 int64_t partial_slots(int64_t n, int64_t tiles);
 
+// The systems of a launch (device tables).  A single system is the case
+// nsys = 1, where tile_sys is not read.
+struct SysTable {
+  const SysDesc *sys = nullptr;        // nsys entries, tile map of one precision
+  const int *tile_sys = nullptr;       // system of each tile (ntiles)
+  const int *tile_seg = nullptr;       // first tile of each system (nsys + 1)
+  const long long *atom_seg = nullptr; // first atom of each system (nsys + 1)
+  int nsys = 1;
+  int ntiles = 0;
+};
+
 struct LJParams {
   double eps, sig, rc, mass;
 };
@@ -41,6 +52,8 @@ struct ForceConfig {
   int maxrb = 0;      // largest region (home + shell) in blocks
   int groups = 1;     // run groups per atom (work items = groups x atoms)
   int lane_queue = 0; // depth of per-lane queues of deferred pairs (0, 8, 16)
+  int workgroups = 0; // persistent grid: workgroups that loop over the tiles
+                      // (0 = one workgroup per tile)
 };
 
 size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads,
@@ -53,13 +66,19 @@ void launch_fill_i32(int *p, int v, int64_t n, hipStream_t s);
 void launch_fill_log(StepLog *p, int n, hipStream_t s);
 void launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
 
-void launch_step_bin(const DevState &d, const Grid &g, bool mixed, int mode,
-                     double h, double dt, StepLog *ke_log, StepLog *flag_log,
-                     hipStream_t s);
+// ke_log (single system only): kinetic energy of the finished step.
+void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
+                     int mode, double h, double dt, StepLog *ke_log,
+                     StepLog *flag_log, hipStream_t s);
+// ke_log (single system only): kinetic energy after the kick.
 void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
                  hipStream_t s);
+// Kinetic energy of every system into log[sys].ke (sum of v^2).
+void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
+                   hipStream_t s);
 void launch_sort(const DevState &d, int64_t nblocks, hipStream_t s);
-void launch_forces(const DevState &d, const Grid &g, const LJParams &p,
+// Tallies of system k go to log[k].
+void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                    const ForceConfig &c, bool mixed, bool energy,
                    bool homecount, StepLog *log, hipStream_t s);
 void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,

@@ -40,8 +40,15 @@ Module exaMD_hip_binding
      Integer(c_int32_t) :: block_threads
      Integer(c_int32_t) :: run_groups
      Integer(c_int32_t) :: lane_queue
+     Integer(c_int32_t) :: tile_edge
+     Integer(c_int32_t) :: workgroups
      Real(c_double)     :: force_ms
   End Type examd_tune
+
+  Type, Bind(C) :: examd_system
+     Integer(c_int64_t) :: natoms
+     Real(c_double)     :: box(3)
+  End Type examd_system
 
   Type, Bind(C) :: examd_stats
      Integer(c_int64_t) :: step
@@ -181,6 +188,23 @@ Module exaMD_hip_binding
        Integer(c_int), Value :: precision
        Type(examd_tune), Intent(in) :: cfg
      End Function examd_hip_set_config
+
+     Integer(c_int) Function examd_hip_batch_create(common, systems, nsys, &
+          handle) Bind(C, name="examd_hip_batch_create")
+       Import :: c_int, c_ptr, c_int32_t, examd_params, examd_system
+       Type(examd_params), Intent(in) :: common
+       Type(examd_system), Intent(in) :: systems(*)
+       Integer(c_int32_t), Value :: nsys
+       Type(c_ptr), Intent(out) :: handle
+     End Function examd_hip_batch_create
+
+     Integer(c_int) Function examd_hip_batch_info(handle, nsys, natoms, &
+          nblocks) Bind(C, name="examd_hip_batch_info")
+       Import :: c_int, c_ptr, c_int32_t, c_int64_t
+       Type(c_ptr), Value :: handle
+       Integer(c_int32_t), Intent(out) :: nsys
+       Integer(c_int64_t), Intent(out) :: natoms, nblocks
+     End Function examd_hip_batch_info
 
      Integer(c_int) Function examd_hip_set_graph_chunk(handle, chunk) &
           Bind(C, name="examd_hip_set_graph_chunk")

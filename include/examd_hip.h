@@ -97,6 +97,34 @@ int examd_hip_domain(const double box[3], double block_length,
  * SRC/exaMD_module.f90:276). */
 int examd_hip_create(const examd_params *params, examd_handle **out);
 
+/* One system of a batch. */
+typedef struct examd_system {
+    int64_t natoms;
+    double box[3];
+} examd_system;
+
+/* Batching form (no reference counterpart: the reference runs one system
+ * per MPI job): nsys independent periodic LJ systems of uneven sizes in one
+ * handle.  Every kernel of a step covers all systems in one launch (the
+ * force kernel walks one list of cell blocks across the systems), and
+ * examd_hip_run captures multi-step chunks of the whole batch in one
+ * hipGraph.  common supplies cutoff, epsilon, sigma, mass, device,
+ * block_threads, cap_slack_pct, run_groups and graph_chunk for every system;
+ * common->natoms, box and tiles must be 0.  The other entry points take the
+ * batch's atoms concatenated in system order (n = total atoms), return one
+ * examd_stats per system (forces: stats[nsys]; run: nsys entries per
+ * reported step, system order), and number blocks per system
+ * (block_ids: 1-based within the atom's system; home_pair_counts: the
+ * systems' blocks concatenated). */
+int examd_hip_batch_create(const examd_params *common,
+                           const examd_system *systems, int32_t nsys,
+                           examd_handle **out);
+
+/* Systems, total atoms and total blocks of a handle (1 system for
+ * examd_hip_create). */
+int examd_hip_batch_info(examd_handle *h, int32_t *nsys, int64_t *natoms,
+                         int64_t *nblocks);
+
 /* Release everything (SRC/exaMD_utils.f90:90 exaMD_Finalize). */
 int examd_hip_destroy(examd_handle *h);
 
@@ -153,11 +181,18 @@ int examd_hip_home_pair_counts(examd_handle *h, int precision,
  * SRC/exaMD_partitioning.f90:65-455 becomes this map of cell blocks to
  * workgroups), the workgroup size and the run groups per atom. */
 typedef struct examd_tune {
-    int32_t tiles[3];
+    int32_t tiles[3];     /* cell blocks per dimension (single system; of
+                             system 0 in a batch) */
     int32_t block_threads;
     int32_t run_groups;
     int32_t lane_queue;   /* depth of per-lane queues of deferred in-cutoff
                              pairs: 0 (evaluate at once), 8 or 16 */
+    int32_t tile_edge;    /* set_config: when tiles are 0, every system's map
+                             is T = max(1, round(nblocks_xyz / tile_edge));
+                             get_config: the edge the map came from (0 when
+                             tiles were given) */
+    int32_t workgroups;   /* persistent force grid: workgroups that loop over
+                             the cell blocks (0 = one workgroup per block) */
     double force_ms;      /* mean force-kernel time measured by autotune */
 } examd_tune;
 

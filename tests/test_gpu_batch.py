@@ -1,0 +1,175 @@
+"""Batching form on the GPU: independent periodic systems of uneven sizes in
+one handle, every kernel of a step covering all systems in one launch.
+
+Bars: each system's FP64 forces within 1e-10 relative of the oracle run on
+that system alone, pair counts and cell assignment bit-exact; the mixed path
+bit-identical to the same system run in its own handle (fixed-point sums are
+order-independent and the cell-block maps are the same), over single
+evaluations and over MD runs replayed from captured graphs.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+# (FCC cells per side, density, perturbation, seed): 500 to 4000 atoms,
+# 6 to 13 blocks per side (one cell block per axis for the smallest)
+SYSTEMS = [(5, 0.90, 0.10, 1), (7, 0.60, 0.05, 2), (6, 0.75, 0.20, 3),
+           (10, 0.8442, 0.05, 4), (5, 0.65, 0.10, 5)]
+
+
+@pytest.fixture(scope="module")
+def E():
+    import examd_amd
+    examd_amd.lib()
+    return examd_amd
+
+
+def _systems(E, temperature=0.0):
+    out = []
+    for nc, rho, amp, seed in SYSTEMS:
+        pos, vel, box = E.generate_fcc(nc, rho, temperature, perturb=amp,
+                                       seed=seed)
+        out.append((pos, vel, box))
+    return out
+
+
+def _batch(E, sy, **kw):
+    b = E.MidpointLJBatch([(len(p), box) for p, _, box in sy], 2.5, **kw)
+    b.set_state(np.vstack([p for p, _, _ in sy]),
+                np.vstack([v for _, v, _ in sy]))
+    return b
+
+
+def _rel(f, ref):
+    return np.abs(f - ref).max() / np.sqrt((ref ** 2).sum(1).mean())
+
+
+def test_batch_fp64_forces_match_oracle(E, oracle):
+    print("[batch fp64 vs oracle]", flush=True)
+    sy = _systems(E)
+    with _batch(E, sy) as b:
+        f, stats, _ = b.forces("fp64")
+        off = b.offsets
+    for k, (pos, _, box) in enumerate(sy):
+        ref, t, _ = oracle.forces(pos, box, 2.5)
+        assert _rel(f[off[k]:off[k + 1]], ref) < FP64_TOL, k
+        assert (stats[k].pairs_cand, stats[k].pairs_cut) == (t.cand, t.cut), k
+        assert abs(stats[k].pe - t.pe) < FP64_TOL * abs(t.pe), k
+
+
+def test_single_cell_block_per_axis_matches_oracle(E, oracle):
+    """A system whose map has one cell block per axis: the region's ghost
+    shell holds periodic images of the block's own atoms."""
+    print("[one cell block]", flush=True)
+    pos, _, box = E.generate_fcc(5, 0.9, 0.0, perturb=0.1, seed=9)
+    ref, t, hc = oracle.forces(pos, box, 2.5, home_counts=True)
+    with E.MidpointLJ(len(pos), box, 2.5, tiles=(1, 1, 1)) as m:
+        m.set_state(pos)
+        f, s, _ = m.forces("fp64")
+        got = m.home_pair_counts("fp64")
+    assert _rel(f, ref) < FP64_TOL
+    assert (s.pairs_cand, s.pairs_cut) == (t.cand, t.cut)
+    np.testing.assert_array_equal(got, hc)
+
+
+def test_batch_mixed_equals_standalone(E):
+    print("[batch mixed vs standalone]", flush=True)
+    sy = _systems(E)
+    with _batch(E, sy) as b:
+        fb, sb, _ = b.forces("mixed")
+        off = b.offsets
+    for k, (pos, vel, box) in enumerate(sy):
+        with E.MidpointLJ(len(pos), box, 2.5) as m:
+            m.set_state(pos, vel)
+            f1, s1, _ = m.forces("mixed")
+        assert np.array_equal(fb[off[k]:off[k + 1]], f1), k
+        assert (sb[k].pe, sb[k].pairs_cand, sb[k].pairs_cut) == \
+            (s1.pe, s1.pairs_cand, s1.pairs_cut), k
+        assert abs(sb[k].ke - s1.ke) <= 1e-12 * s1.ke, k
+
+
+def test_batch_run_equals_standalone(E):
+    """45 mixed MD steps, tallies every 10 (two replays of a captured
+    20-step chunk of the whole batch, then 5 eager steps), then 10 eager
+    steps, against each system run on its own with the same chunking."""
+    print("[batch run vs standalone]", flush=True)
+    sy = _systems(E, temperature=1.0)
+    with _batch(E, sy, graph_chunk=20) as b:
+        rows = b.run(45, 0.005, "mixed", energy_every=10)
+        rows += b.run(10, 0.005, "mixed", energy_every=10)
+        pb, vb = b.get_state()
+        off = b.offsets
+    assert [r[0].step for r in rows] == [10, 20, 30, 40, 45, 10]
+    for k, (pos, vel, box) in enumerate(sy):
+        with E.MidpointLJ(len(pos), box, 2.5, graph_chunk=20) as m:
+            m.set_state(pos, vel)
+            st = m.run(45, 0.005, "mixed", energy_every=10)
+            st += m.run(10, 0.005, "mixed", energy_every=10)
+            p1, v1 = m.get_state()
+        assert np.array_equal(pb[off[k]:off[k + 1]], p1), k
+        assert np.array_equal(vb[off[k]:off[k + 1]], v1), k
+        for r, s in zip(rows, st):
+            assert (r[k].step, r[k].pe, r[k].pairs_cut) == \
+                (s.step, s.pe, s.pairs_cut), k
+            assert abs(r[k].ke - s.ke) <= 1e-12 * s.ke, k
+
+
+def test_batch_block_ids_and_ownership(E, oracle):
+    print("[batch ids + ownership]", flush=True)
+    sy = _systems(E)
+    with _batch(E, sy) as b:
+        num, xyz = b.block_ids()
+        hc = b.home_pair_counts("fp64")
+        off = b.offsets
+    boff = 0
+    for k, (pos, _, box) in enumerate(sy):
+        d = oracle.domain_create(box, 1.25)
+        want = np.array([oracle.identify_block(d, p)[2] for p in pos])
+        np.testing.assert_array_equal(num[off[k]:off[k + 1]], want)
+        _, _, ref_hc = oracle.forces(pos, box, 2.5, home_counts=True)
+        np.testing.assert_array_equal(hc[boff:boff + d.nblocks], ref_hc)
+        boff += d.nblocks
+    assert boff == len(hc)
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_persistent_grid_matches(E, batch):
+    """A persistent force grid (workgroups looping over the cell blocks)
+    gives the same mixed forces bit for bit and the same FP64 forces to
+    rounding."""
+    print(f"[persistent grid] batch={batch}", flush=True)
+    if batch:
+        sy = _systems(E)
+        h = _batch(E, sy)
+        cfg = {"tile_edge": 5}
+    else:
+        pos, vel, box = E.generate_fcc(12, 0.8442, 0.0, perturb=0.1, seed=8)
+        h = E.MidpointLJ(len(pos), box, 2.5)
+        h.set_state(pos)
+        cfg = {"tiles": (4, 4, 3)}
+    with h:
+        out = {}
+        for wgs in (0, 16, 24):
+            for prec in ("mixed", "fp64"):
+                h.set_config(prec, block_threads=256, workgroups=wgs, **cfg)
+                assert h.get_config(prec)["workgroups"] == wgs
+                out[wgs, prec] = h.forces(prec)[0]
+    for wgs in (16, 24):
+        assert np.array_equal(out[wgs, "mixed"], out[0, "mixed"])
+        assert _rel(out[wgs, "fp64"], out[0, "fp64"]) < 1e-13
+
+
+def test_batch_autotune(E, oracle):
+    print("[batch autotune]", flush=True)
+    sy = _systems(E)
+    with _batch(E, sy) as b:
+        best = b.autotune("mixed", trials=1)
+        assert best["tile_edge"] > 0 and best["force_ms"] > 0
+        f, _, _ = b.forces("mixed")
+        off = b.offsets
+    pos, _, box = sy[3]
+    ref, _, _ = oracle.forces(pos, box, 2.5)
+    assert _rel(f[off[3]:off[4]], ref) < 2e-5
