@@ -74,13 +74,15 @@ def read_traffic():
 
 
 def parse_shape(text):
-    """'Tx,Ty,Tz,threads[,groups[,lane_queue]]' -> dict, or None."""
+    """'Tx,Ty,Tz,threads[,groups[,lane_queue[,workgroups]]]' -> dict, or
+    None."""
     if not text:
         return None
     v = [int(x) for x in text.split(",")]
     return {"tiles": tuple(v[:3]), "block_threads": v[3],
             "run_groups": v[4] if len(v) > 4 else 0,
-            "lane_queue": v[5] if len(v) > 5 else 0}
+            "lane_queue": v[5] if len(v) > 5 else 0,
+            "workgroups": v[6] if len(v) > 6 else 0}
 
 
 def force_error(m):
@@ -109,8 +111,7 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
         if shape is None:
             tune = m.autotune(prec, trials=3)
         else:
-            m.set_config(prec, shape["tiles"], shape["block_threads"],
-                         shape["run_groups"], shape["lane_queue"])
+            m.set_config(prec, **shape)
             tune = m.get_config(prec)
         if warmup:
             m.run(warmup, DT, prec, energy_every=0)
@@ -125,6 +126,118 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
         raise RuntimeError(f"{prec}: {launches} timed force launches for "
                            f"{steps} steps")
     return t1 - t0, stats, force_ms / max(launches, 1), launches, tune, err
+
+
+def batch_systems(E, nsys=256, seed=5):
+    """Batching workload: nsys periodic LJ systems, atom counts log-uniform
+    in [500, 20000], densities uniform in [0.60, 0.90], temperatures uniform
+    in [0.70, 1.50] (one master seed), each a randomly depleted FCC lattice
+    at its density with Maxwell-Boltzmann velocities."""
+    rng = np.random.default_rng(seed)
+    n = np.rint(np.exp(rng.uniform(np.log(500), np.log(20000), nsys)))
+    rho = rng.uniform(0.60, 0.90, nsys)
+    temp = rng.uniform(0.70, 1.50, nsys)
+    out = []
+    for k in range(nsys):
+        pos, vel, box = E.generate_vacancy_fcc(int(n[k]), rho[k], temp[k],
+                                               perturb=0.02, seed=seed * 1000 + k)
+        out.append((pos, vel, box))
+    return out
+
+
+def batch_bench(E, steps, warmup, loop_steps, nsys=256):
+    """Aggregate atom-steps/s of the batch in one handle (every kernel of a
+    step covers all systems; 100-step hipGraph chunks; autotuned tile edge
+    and launch shape) against a per-system launch loop: each system in its
+    own handle with the same shape and its own 100-step graphs, run in turn
+    for loop_steps steps."""
+    sy = batch_systems(E, nsys)
+    total = int(sum(len(p) for p, _, _ in sy))
+    systems = [(len(p), box) for p, _, box in sy]
+    pos = np.vstack([p for p, _, _ in sy])
+    vel = np.vstack([v for _, v, _ in sy])
+    with E.MidpointLJBatch(systems, RC, graph_chunk=100) as b:
+        b.set_state(pos, vel)
+        tune = b.autotune("mixed", trials=2)
+        if warmup:
+            b.run(warmup, DT, "mixed")
+        b.timing(enable=True, reset=True)
+        t0 = time.perf_counter()
+        rows = b.run(steps, DT, "mixed", energy_every=100)
+        wall = time.perf_counter() - t0
+        force_ms, launches = b.timing(enable=False, reset=True)
+    e0 = np.array([s.ke + s.pe_shifted for s in rows[0]])
+    e1 = np.array([s.ke + s.pe_shifted for s in rows[-1]])
+    natoms = np.array([len(p) for p, _, _ in sy], dtype=float)
+    handles = []
+    try:
+        for (p, v, box) in sy:
+            m = E.MidpointLJ(len(p), box, RC, graph_chunk=100)
+            handles.append(m)
+            m.set_state(p, v)
+            m.set_config("mixed", block_threads=tune["block_threads"],
+                         run_groups=tune["run_groups"],
+                         lane_queue=tune["lane_queue"],
+                         tile_edge=tune["tile_edge"])
+            m.run(warmup if warmup else 1, DT, "mixed")
+        t0 = time.perf_counter()
+        for m in handles:
+            m.run(loop_steps, DT, "mixed", energy_every=100)
+        loop_wall = time.perf_counter() - t0
+    finally:
+        for m in handles:
+            m.close()
+    rate = total * steps / wall
+    loop_rate = total * loop_steps / loop_wall
+    return {
+        "workload": f"{nsys} periodic LJ systems, atoms log-uniform in "
+                    "[500, 20000], rho U[0.60, 0.90], T* U[0.70, 1.50], "
+                    "rc 2.5, mixed precision, dt 0.005, 100-step hipGraph "
+                    "chunks of the whole batch",
+        "systems": nsys, "atoms": total, "steps": steps,
+        "atom_steps_per_s": rate, "ms_per_step": 1e3 * wall / steps,
+        "force_ms": force_ms / max(launches, 1),
+        "per_system_loop_atom_steps_per_s": loop_rate,
+        "per_system_loop_steps": loop_steps,
+        "speedup_vs_loop": rate / loop_rate,
+        "max_energy_change_per_atom": float(np.max(np.abs(e1 - e0) / natoms)),
+        "launch": tune,
+    }
+
+
+def slab_bench(E, steps, warmup):
+    """Non-uniform density: 4,000,000-atom liquid-vapour slab (box 3:1:1,
+    central third of x at density 0.80, the rest at 0.05, uniform with a
+    0.8 sigma minimum separation, T* 0.85), rc 3.0, mixed precision,
+    autotuned cell-block map."""
+    rc = 3.0
+    vol = 4.0e6 / (0.80 / 3 + 0.05 * 2 / 3)
+    ly = (vol / 3.0) ** (1.0 / 3.0)
+    box = np.array([3 * ly, ly, ly])
+    t0 = time.perf_counter()
+    pos, vel = E.generate_slab(box, ly, 2 * ly, 0.80, 0.05, min_sep=0.8,
+                               temperature=0.85, seed=4)
+    gen_s = time.perf_counter() - t0
+    n = len(pos)
+    with E.MidpointLJ(n, box, rc, graph_chunk=100) as m:
+        m.set_state(pos, vel)
+        tune = m.autotune("mixed", trials=2)
+        if warmup:
+            m.run(warmup, DT, "mixed")
+        m.timing(enable=True, reset=True)
+        t0 = time.perf_counter()
+        st = m.run(steps, DT, "mixed", energy_every=100)
+        wall = time.perf_counter() - t0
+        force_ms, launches = m.timing(enable=False, reset=True)
+    return {
+        "workload": "liquid-vapour slab, box 3:1:1, central third at rho 0.80, "
+                    "vapour 0.05, 0.8 sigma minimum separation, T* 0.85, "
+                    "rc 3.0, mixed precision, dt 0.005",
+        "atoms": n, "box": box.tolist(), "steps": steps,
+        "atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "force_ms": force_ms / max(launches, 1),
+        "pairs_cut": st[-1].pairs_cut, "generate_s": gen_s, "launch": tune,
+    }
 
 
 def cpu_baseline(pos, vel, box, nsteps):
@@ -151,8 +264,7 @@ def pmc_probe(E, shape):
         if shape is None:
             print("tune", m.autotune("mixed", trials=2), flush=True)
         else:
-            m.set_config("mixed", shape["tiles"], shape["block_threads"],
-                         shape["run_groups"], shape["lane_queue"])
+            m.set_config("mixed", **shape)
         for _ in range(3):
             m.forces("mixed")
     hip = ctypes.CDLL("libamdhip64.so")
@@ -178,16 +290,19 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--loop-steps", type=int, default=200)
+    ap.add_argument("--slab-steps", type=int, default=2000)
     ap.add_argument("--graph-chunk", type=int, default=100,
                     help="MD steps per captured hipGraph chunk (0 = eager)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short counter-collection program instead")
     ap.add_argument("--mixed-shape", default=None,
-                    help="Tx,Ty,Tz,threads,groups,lane_queue instead of "
-                         "autotuning")
+                    help="Tx,Ty,Tz,threads,groups,lane_queue,workgroups "
+                         "instead of autotuning")
     ap.add_argument("--fp64-shape", default=None,
-                    help="Tx,Ty,Tz,threads,groups,lane_queue instead of "
-                         "autotuning")
+                    help="Tx,Ty,Tz,threads,groups,lane_queue,workgroups "
+                         "instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the "
@@ -223,6 +338,13 @@ def main():
     achieved = w / (force_ms_mx * 1e-3) / 1e12
     w64 = algorithmic_flops(cand, cut, False) + 3.0 * cut / energy_every
     achieved64 = w64 / (force_ms_64 * 1e-3) / 1e12
+
+    batch = None
+    if args.batch_steps > 0:
+        batch = batch_bench(E, args.batch_steps, args.warmup, args.loop_steps)
+    slab = None
+    if args.slab_steps > 0:
+        slab = slab_bench(E, args.slab_steps, args.warmup)
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -275,6 +397,8 @@ def main():
             "launch_mixed": tune_mx,
             "launch_fp64": tune_64,
             "graph_chunk_steps": args.graph_chunk,
+            "batch": batch,
+            "slab": slab,
             "parallelism": "1 GPU",
         },
         "roofline": {

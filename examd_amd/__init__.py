@@ -44,6 +44,7 @@ EXPORTS = (
     "examd_hip_phase_profile", "examd_hip_autotune", "examd_hip_get_config",
     "examd_hip_set_config", "examd_hip_set_graph_chunk",
     "examd_hip_batch_create", "examd_hip_batch_info",
+    "examd_hip_generate_vacancy_fcc", "examd_hip_generate_slab",
 )
 
 
@@ -127,6 +128,12 @@ def lib():
         L.examd_hip_generate_fcc.argtypes = [ip32, C.c_double, C.c_double,
                                              C.c_double, C.c_double,
                                              C.c_uint64, dp, dp, dp]
+        L.examd_hip_generate_vacancy_fcc.argtypes = [
+            C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+            C.c_uint64, dp, dp, dp]
+        L.examd_hip_generate_slab.argtypes = [
+            dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+            C.c_double, C.c_double, C.c_uint64, dp, dp, C.c_int64, ip64]
         L.examd_hip_strerror.argtypes = [C.c_int]
         L.examd_hip_strerror.restype = C.c_char_p
         for name in EXPORTS:
@@ -173,6 +180,39 @@ def generate_fcc(nc, rho, temperature=0.0, mass=1.0, perturb=0.0, seed=0):
                                         _dp(vel), _dp(box)),
            "examd_hip_generate_fcc")
     return pos, vel, box
+
+
+def generate_vacancy_fcc(n, rho, temperature=0.0, mass=1.0, perturb=0.0,
+                         seed=0):
+    """n atoms at density rho on a randomly depleted FCC lattice in a cubic
+    box, Maxwell-Boltzmann velocities: (pos, vel, box)."""
+    pos = np.zeros((int(n), 3))
+    vel = np.zeros((int(n), 3))
+    box = np.zeros(3)
+    _check(lib().examd_hip_generate_vacancy_fcc(
+        int(n), float(rho), float(temperature), float(mass), float(perturb),
+        int(seed) & (2**64 - 1), _dp(pos), _dp(vel), _dp(box)),
+        "examd_hip_generate_vacancy_fcc")
+    return pos, vel, box
+
+
+def generate_slab(box, slab_lo, slab_hi, rho_slab, rho_vapour, min_sep=0.8,
+                  temperature=0.0, mass=1.0, seed=0):
+    """Liquid-vapour slab (random sequential addition with a minimum
+    separation), Maxwell-Boltzmann velocities: (pos, vel)."""
+    b = np.ascontiguousarray(box, dtype=np.float64).reshape(3)
+    n = C.c_int64()
+    args = (_dp(b), float(slab_lo), float(slab_hi), float(rho_slab),
+            float(rho_vapour), float(min_sep), float(temperature), float(mass),
+            int(seed) & (2**64 - 1))
+    _check(lib().examd_hip_generate_slab(*args, None, None, 0, C.byref(n)),
+           "examd_hip_generate_slab")
+    pos = np.zeros((n.value, 3))
+    vel = np.zeros((n.value, 3))
+    _check(lib().examd_hip_generate_slab(*args, _dp(pos), _dp(vel), n.value,
+                                         C.byref(n)),
+           "examd_hip_generate_slab")
+    return pos, vel
 
 
 class _Handle:

@@ -1360,4 +1360,146 @@ int examd_hip_generate_fcc(const int32_t nc[3], double rho, double temperature,
   return EXAMD_OK;
 }
 
+int examd_hip_generate_vacancy_fcc(int64_t n, double rho, double temperature,
+                                   double mass, double perturb, uint64_t seed,
+                                   double *pos, double *vel, double box[3]) {
+  if (n <= 0 || n > (int64_t)2000000000 || !pos || !box || !(rho > 0.0) ||
+      !(mass > 0.0) || !(temperature >= 0.0) || !(perturb >= 0.0))
+    return EXAMD_ERR_ARG;
+  const double L = std::cbrt((double)n / rho);
+  int64_t nc = 1;
+  while (4 * nc * nc * nc < n) ++nc;
+  const int64_t m = 4 * nc * nc * nc;
+  const double a = L / (double)nc;
+  for (int d = 0; d < 3; ++d) box[d] = L;
+  uint64_t s = seed;
+  // partial Fisher-Yates: the first n entries of a seeded shuffle of sites
+  std::vector<int64_t> site((size_t)m);
+  for (int64_t i = 0; i < m; ++i) site[i] = i;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t j = i + (int64_t)(uniform01(s) * (double)(m - i));
+    std::swap(site[i], site[std::min(j, m - 1)]);
+  }
+  std::sort(site.begin(), site.begin() + n);   // lattice order
+  static const double basis[4][3] = {
+      {0.0, 0.0, 0.0}, {0.5, 0.5, 0.0}, {0.5, 0.0, 0.5}, {0.0, 0.5, 0.5}};
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t q = site[i];
+    const int b = (int)(q % 4);
+    const int64_t c = q / 4;
+    const int64_t cc[3] = {c % nc, (c / nc) % nc, c / (nc * nc)};
+    for (int d = 0; d < 3; ++d) {
+      double x = ((double)cc[d] + basis[b][d]) * a;
+      if (perturb > 0.0) x += perturb * (2.0 * uniform01(s) - 1.0);
+      x = x - std::floor(x / L) * L;
+      if (x >= L) x -= L;
+      if (x < 0.0) x = 0.0;
+      pos[3 * i + d] = x;
+    }
+  }
+  if (vel) mb_velocities(n, temperature, mass, s, vel);
+  return EXAMD_OK;
+}
+
+int examd_hip_generate_slab(const double box[3], double slab_lo,
+                            double slab_hi, double rho_slab,
+                            double rho_vapour, double min_sep,
+                            double temperature, double mass, uint64_t seed,
+                            double *pos, double *vel, int64_t cap,
+                            int64_t *n) {
+  if (!box || !n || !(rho_slab >= 0.0) || !(rho_vapour >= 0.0) ||
+      !(min_sep >= 0.0) || !(mass > 0.0) || !(temperature >= 0.0))
+    return EXAMD_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (!(box[d] > 0.0)) return EXAMD_ERR_ARG;
+  if (!(slab_lo >= 0.0 && slab_lo < slab_hi && slab_hi <= box[0]))
+    return EXAMD_ERR_ARG;
+  const double area = box[1] * box[2];
+  const double vs = (slab_hi - slab_lo) * area;
+  const double vv = box[0] * area - vs;
+  const int64_t ns = (int64_t)std::llround(rho_slab * vs);
+  const int64_t nv = (int64_t)std::llround(rho_vapour * vv);
+  *n = ns + nv;
+  if (*n > (int64_t)2000000000) return EXAMD_ERR_ARG;
+  if (!pos) return EXAMD_OK;
+  if (cap < *n) return EXAMD_ERR_ARG;
+  // cell list of accepted atoms, cells at least min_sep wide
+  const double cw = std::max(min_sep, 1e-3);
+  int nc[3];
+  for (int d = 0; d < 3; ++d)
+    nc[d] = std::max(1, std::min(1 << 20, (int)(box[d] / cw)));
+  const int64_t ncell = (int64_t)nc[0] * nc[1] * nc[2];
+  if (ncell > (int64_t)1 << 31) return EXAMD_ERR_ARG;
+  std::vector<int64_t> head((size_t)ncell, -1), next((size_t)*n, -1);
+  auto cell_of = [&](const double *x, int *c) {
+    for (int d = 0; d < 3; ++d) {
+      int v = (int)(x[d] / box[d] * nc[d]);
+      c[d] = v < 0 ? 0 : (v >= nc[d] ? nc[d] - 1 : v);
+    }
+  };
+  const double ms2 = min_sep * min_sep;
+  auto fits = [&](const double *x) {
+    int c[3];
+    cell_of(x, c);
+    const int r = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3 ? 1 : 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (r == 0 && (dx || dy || dz)) continue;
+          const int cx = (c[0] + dx + nc[0]) % nc[0];
+          const int cy = (c[1] + dy + nc[1]) % nc[1];
+          const int cz = (c[2] + dz + nc[2]) % nc[2];
+          for (int64_t j = head[(size_t)cx + (size_t)nc[0] * (cy + (size_t)nc[1] * cz)];
+               j >= 0; j = next[(size_t)j]) {
+            double d2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+              double u = pos[3 * j + d] - x[d];
+              u -= box[d] * std::nearbyint(u / box[d]);
+              d2 += u * u;
+            }
+            if (d2 < ms2) return false;
+          }
+        }
+    return true;
+  };
+  if (nc[0] < 3 || nc[1] < 3 || nc[2] < 3) {
+    // tiny boxes: check every accepted atom
+    for (int d = 0; d < 3; ++d) nc[d] = 1;
+  }
+  uint64_t s = seed;
+  const double wv = box[0] - (slab_hi - slab_lo);
+  for (int64_t i = 0; i < *n; ++i) {
+    const bool slab = i < ns;
+    bool placed = false;
+    for (int attempt = 0; attempt < 10000 && !placed; ++attempt) {
+      double x[3];
+      if (slab) {
+        x[0] = slab_lo + uniform01(s) * (slab_hi - slab_lo);
+      } else {
+        // vapour: the complement of the slab along x, periodic
+        double u = uniform01(s) * wv;
+        x[0] = slab_hi + u;
+        if (x[0] >= box[0]) x[0] -= box[0];
+      }
+      x[1] = uniform01(s) * box[1];
+      x[2] = uniform01(s) * box[2];
+      for (int d = 0; d < 3; ++d) {
+        if (x[d] >= box[d]) x[d] = std::nextafter(box[d], 0.0);
+        if (x[d] < 0.0) x[d] = 0.0;
+      }
+      if (min_sep > 0.0 && !fits(x)) continue;
+      for (int d = 0; d < 3; ++d) pos[3 * i + d] = x[d];
+      int c[3];
+      cell_of(x, c);
+      const size_t ci = (size_t)c[0] + (size_t)nc[0] * (c[1] + (size_t)nc[1] * c[2]);
+      next[(size_t)i] = head[ci];
+      head[ci] = i;
+      placed = true;
+    }
+    if (!placed) return EXAMD_ERR_ARG;
+  }
+  if (vel) mb_velocities(*n, temperature, mass, s, vel);
+  return EXAMD_OK;
+}
+
 }  // extern "C"

@@ -213,6 +213,29 @@ Module exaMD_hip_binding
        Integer(c_int64_t), Value :: chunk
      End Function examd_hip_set_graph_chunk
 
+     Integer(c_int) Function examd_hip_generate_vacancy_fcc(n, rho, &
+          temperature, mass, perturb, seed, pos, vel, box) &
+          Bind(C, name="examd_hip_generate_vacancy_fcc")
+       Import :: c_int, c_int64_t, c_double
+       Integer(c_int64_t), Value :: n
+       Real(c_double), Value :: rho, temperature, mass, perturb
+       Integer(c_int64_t), Value :: seed
+       Real(c_double), Intent(out) :: pos(*), vel(*), box(3)
+     End Function examd_hip_generate_vacancy_fcc
+
+     Integer(c_int) Function examd_hip_generate_slab(box, slab_lo, slab_hi, &
+          rho_slab, rho_vapour, min_sep, temperature, mass, seed, pos, vel, &
+          cap, n) Bind(C, name="examd_hip_generate_slab")
+       Import :: c_int, c_int64_t, c_double, c_ptr
+       Real(c_double), Intent(in) :: box(3)
+       Real(c_double), Value :: slab_lo, slab_hi, rho_slab, rho_vapour, &
+            min_sep, temperature, mass
+       Integer(c_int64_t), Value :: seed
+       Type(c_ptr), Value :: pos, vel
+       Integer(c_int64_t), Value :: cap
+       Integer(c_int64_t), Intent(out) :: n
+     End Function examd_hip_generate_slab
+
      Type(c_ptr) Function examd_hip_strerror(status) &
           Bind(C, name="examd_hip_strerror")
        Import :: c_int, c_ptr

@@ -233,6 +233,30 @@ int examd_hip_generate_fcc(const int32_t nc[3], double rho, double temperature,
                            double mass, double perturb, uint64_t seed,
                            double *pos, double *vel, double box[3]);
 
+/* n atoms at number density rho in a cubic periodic box (box[0..2] = L =
+ * (n/rho)^(1/3)): n sites drawn without replacement (seeded) from the
+ * smallest FCC lattice of nc^3 cells with at least n sites, perturbed by
+ * uniform(-p, p); velocities as examd_hip_generate_fcc.  The nearest sites
+ * are L/(nc sqrt 2) apart, so no pair starts closer than that minus
+ * 2 sqrt(3) p. */
+int examd_hip_generate_vacancy_fcc(int64_t n, double rho, double temperature,
+                                   double mass, double perturb, uint64_t seed,
+                                   double *pos, double *vel, double box[3]);
+
+/* Liquid-vapour slab in a periodic box: atoms uniform (seeded) in the slab
+ * slab_lo <= x < slab_hi at density rho_slab and in the rest of the box at
+ * rho_vapour, no pair closer than min_sep (random sequential addition with
+ * up to 10^4 attempts per atom), slab atoms first; velocities as
+ * examd_hip_generate_fcc.  The counts are round(rho * volume) per region;
+ * *n receives the total.  With pos == NULL only *n is computed; otherwise
+ * cap must be at least *n. */
+int examd_hip_generate_slab(const double box[3], double slab_lo,
+                            double slab_hi, double rho_slab,
+                            double rho_vapour, double min_sep,
+                            double temperature, double mass, uint64_t seed,
+                            double *pos, double *vel, int64_t cap,
+                            int64_t *n);
+
 /* Human-readable text of a status code. */
 const char *examd_hip_strerror(int status);
 

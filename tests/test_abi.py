@@ -67,6 +67,55 @@ def test_generate_fcc_invariants(E):
     assert 0 < np.abs(d).max() <= 0.05
 
 
+def _min_image_min_dist(pos, box):
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= np.round(d / box) * box
+    r2 = (d ** 2).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    return np.sqrt(r2.min())
+
+
+@pytest.mark.parametrize("n,rho", [(500, 0.9), (501, 0.9), (777, 0.6)])
+def test_generate_vacancy_fcc_invariants(E, n, rho):
+    pos, vel, box = E.generate_vacancy_fcc(n, rho, 1.1, perturb=0.02, seed=5)
+    assert pos.shape == (n, 3)
+    L = (n / rho) ** (1 / 3)
+    assert np.allclose(box, L, rtol=1e-14)
+    assert np.all(pos >= 0) and np.all(pos < box)
+    nc = 1
+    while 4 * nc ** 3 < n:
+        nc += 1
+    assert _min_image_min_dist(pos, box) >= L / nc / np.sqrt(2) - 0.04 * np.sqrt(3)
+    assert np.abs(vel.sum(0)).max() < 1e-12
+    assert abs(0.5 * (vel ** 2).sum() - 1.5 * (n - 1) * 1.1) < 1e-9
+    p2, v2, _ = E.generate_vacancy_fcc(n, rho, 1.1, perturb=0.02, seed=5)
+    assert np.array_equal(p2, pos) and np.array_equal(v2, vel)
+    p3, _, _ = E.generate_vacancy_fcc(n, rho, 1.1, perturb=0.02, seed=6)
+    assert not np.array_equal(p3, pos)
+
+
+def test_generate_slab_invariants(E):
+    box = np.array([24.0, 8.0, 8.0])
+    lo, hi = 8.0, 16.0
+    pos, vel = E.generate_slab(box, lo, hi, 0.8, 0.05, min_sep=0.8,
+                               temperature=0.85, seed=11)
+    ns = round(0.8 * 8 * 64)
+    nv = round(0.05 * 16 * 64)
+    assert len(pos) == ns + nv
+    assert np.all(pos >= 0) and np.all(pos < box)
+    assert np.all((pos[:ns, 0] >= lo) & (pos[:ns, 0] < hi))
+    assert np.all((pos[ns:, 0] < lo) | (pos[ns:, 0] >= hi))
+    assert _min_image_min_dist(pos, box) >= 0.8
+    n = len(pos)
+    assert np.abs(vel.sum(0)).max() < 1e-12
+    assert abs(0.5 * (vel ** 2).sum() - 1.5 * (n - 1) * 0.85) < 1e-9
+    p2, _ = E.generate_slab(box, lo, hi, 0.8, 0.05, min_sep=0.8,
+                            temperature=0.85, seed=11)
+    assert np.array_equal(p2, pos)
+    with pytest.raises(E.ExamdError):   # cannot pack 2.0 at 1.0 separation
+        E.generate_slab(box, lo, hi, 2.0, 0.0, min_sep=1.0, seed=1)
+
+
 def test_create_without_device_reports_status(E):
     """On a host without a GPU the ABI returns a status, never aborts."""
     try:
