@@ -35,7 +35,7 @@ struct TileConfig {
   int edge = 0;           // tile edge the maps were derived from (0 = given)
   int threads = 256;      // workgroup size
   int groups = 1;         // run groups per region atom
-  int lane_queue = 0;     // per-lane queue depth of deferred pairs (0 = off)
+  int lane_queue = 0;     // per-lane queue entries of deferred pairs (0 = off)
   int workgroups = 0;     // persistent force grid (0 = one per tile)
   int cap = 0;            // LDS atom capacity
   int maxrb = 0;          // largest region in blocks
@@ -602,7 +602,7 @@ int apply_config(examd_handle *h, int prec, const int *T, double edge,
                  const double *pos) {
   TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
-  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 8 ? 8 : 16);
+  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 4 ? 4 : 8);
   if (T) {
     for (int a = 0; a < 3; ++a)
       c.T[0][a] = std::max(1, std::min(T[a], h->sys[0].g.nb[a]));
@@ -1227,7 +1227,7 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
   };
   for (int si = 0; si < nsizes; ++si)
     for (int thr : threads)
-      for (int lq = 0; lq <= 16; lq += 8)
+      for (int lq = 0; lq <= 8; lq += 4)
         for (int grp = 0; grp <= (thr == 1024 ? 3 : 1); ++grp) {
           if (grp == 1) continue;  // 0 = automatic choice covers one group
           int T[3];

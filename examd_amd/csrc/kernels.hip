@@ -385,7 +385,6 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
 // arrays hold cap + 4 entries: the pair loop reads four candidates at a
 // time and may look up to three entries past the last atom (masked off).
 constexpr int kWorkBins = 64;   // owned J blocks of an atom: 0..63
-constexpr int kMaxLaneQueue = 16;  // deepest per-lane queue of deferred pairs
 struct LdsLayout {
   size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_perm, off_hist,
       off_q, total;
@@ -417,7 +416,7 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   o += (kWorkBins + 1) * 4;
   o = align_up(o, 16);
   L.off_q = o;
-  o += (size_t)nwaves * 64 * qdepth * 2;
+  o += (size_t)nwaves * 64 * qdepth * 4;
   L.total = align_up(o, 16);
   return L;
 }
@@ -460,17 +459,35 @@ struct RunAxisMasks {
           m[r][a][o + 1] = (unsigned char)axis_mask(r, a, o);
   }
 };
-__constant__ RunAxisMasks kRunMasks;
+// The same masks packed per (run, axis): byte o + 1 holds the mask for
+// o1 = o, so one 32-bit word per axis serves every o1.
+struct RunAxisPacked {
+  unsigned p[kNumRuns][3];
+  constexpr RunAxisPacked() : p() {
+    const RunAxisMasks M;
+    for (int r = 0; r < kNumRuns; ++r)
+      for (int a = 0; a < 3; ++a)
+        p[r][a] = (unsigned)M.m[r][a][0] | ((unsigned)M.m[r][a][1] << 8) |
+                  ((unsigned)M.m[r][a][2] << 16);
+  }
+};
+__constant__ RunAxisPacked kRunPacked;
 
-__device__ __forceinline__ unsigned axis_sel(const unsigned char *m,
-                                             unsigned ok) {
-  return ((ok & 1u) ? m[0] : 0u) | ((ok & 2u) ? m[1] : 0u) |
-         ((ok & 4u) ? m[2] : 0u);
+// Byte selector of an atom's own_bits: 0xff in byte k when bit k is set.
+__device__ __forceinline__ unsigned own_spread(unsigned ok) {
+  return ((ok & 1u) ? 0xffu : 0u) | ((ok & 2u) ? 0xff00u : 0u) |
+         ((ok & 4u) ? 0xff0000u : 0u);
 }
-__device__ __forceinline__ unsigned run_mask_rt(int r, unsigned ox,
-                                                unsigned oy, unsigned oz) {
-  return axis_sel(kRunMasks.m[r][0], ox) & axis_sel(kRunMasks.m[r][1], oy) &
-         axis_sel(kRunMasks.m[r][2], oz);
+__device__ __forceinline__ unsigned axis_or(unsigned v) {
+  return (v | (v >> 8) | (v >> 16)) & 0x1fu;
+}
+// Ownership mask of run r (bit k <-> dx = k - 2) for an I block whose
+// per-axis own_bits have the byte selectors sx, sy, sz.  r is uniform over
+// the wave, so the table words are scalar loads.
+__device__ __forceinline__ unsigned run_mask_rt(int r, unsigned sx,
+                                                unsigned sy, unsigned sz) {
+  return axis_or(kRunPacked.p[r][0] & sx) & axis_or(kRunPacked.p[r][1] & sy) &
+         axis_or(kRunPacked.p[r][2] & sz);
 }
 
 // Owned J blocks of an atom as a function of its per-axis own_bits
@@ -634,7 +651,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   if (R > A.cap) {
     if (tid == 0) {
       atomicOr(&log->flags, (unsigned long long)kFlagOverflow);
-      A.d.part[tile] = Partial{0.0, 0.0, 0ull, 0ull, 0ull};
+      if (ENERGY) A.d.part[tile] = Partial{0.0, 0.0, 0ull, 0ull, 0ull};
       if (prof) prof[2] = prof[3] = prof[4] = prof[1];
     }
     return;
@@ -723,24 +740,30 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
 
   // 3. pairs.  Work item w = (run group g, atom perm[i]): the lane lists the
   //    J ranges atom a owns in the group's runs (home block I - o1 inside
-  //    the tile) and tests them four candidates at a time.  In-cutoff pairs
-  //    are either evaluated at once or, with lane queues, deferred into a
-  //    per-lane LDS queue that the wave empties together, one pair per lane
-  //    per step.  The force on a accumulates in registers, the force on j
-  //    with LDS atomics.
+  //    the tile) and tests them four candidates at a time.  Each run group
+  //    takes whole waves, so the group and run indices are wave-uniform and
+  //    the run tables are scalar loads.  In-cutoff pairs are either
+  //    evaluated at once or, with lane queues, deferred into a per-lane LDS
+  //    queue (one entry per four candidates with a hit: j << 4 | hit bits)
+  //    that the wave empties together, one pair per lane per step.  The
+  //    force on a accumulates in registers, the force on j with LDS
+  //    atomics.
   unsigned long long cand = 0, cut = 0;
   unsigned flags = 0;
   double pe64 = 0.0;
   long long pefix = 0;
   const int NG = A.groups;
-  const int W = NG * RW;
-  const int QD = A.lane_queue;   // queue depth, 0 = evaluate at once
+  const int RWp = (RW + 63) & ~63;   // atoms per group, padded to whole waves
+  const int W = NG * RWp;
+  const int QD = A.lane_queue;   // queue entries per lane, 0 = evaluate at once
   const bool LQ = QD > 0;
-  unsigned short *my_q = reinterpret_cast<unsigned short *>(smem + LL.off_q) +
-                         (size_t)(tid >> 6) * 64 * QD + (tid & 63);
-  for (int w = tid; w < W; w += NT) {
-    const int grp = w / RW;
-    const int a = (int)s_perm[w - grp * RW];
+  unsigned *my_q = reinterpret_cast<unsigned *>(smem + LL.off_q) +
+                   (size_t)(tid >> 6) * 64 * QD + (tid & 63);
+  for (int wb = tid & ~63; wb < W; wb += NT) {
+    const int grp = __builtin_amdgcn_readfirstlane(wb / RWp);
+    const int ia = wb - grp * RWp + (tid & 63);
+    const bool active = ia < RW;
+    const int a = active ? (int)s_perm[ia] : 0;
     int bc;
     double xa = 0.0, ya = 0.0, za = 0.0;
     float xf = 0.f, yf = 0.f, zf = 0.f;
@@ -757,8 +780,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       bc = __float_as_int(p.w);
     }
     const int ix = bc_x(bc), iy = bc_y(bc), iz = bc_z(bc);
-    const unsigned okx = own_bits(ix, tx), oky = own_bits(iy, ty),
+    // an idle lane (group padding) owns nothing
+    const unsigned okx = active ? own_bits(ix, tx) : 0u, oky = own_bits(iy, ty),
                    okz = own_bits(iz, tz);
+    const unsigned spx = own_spread(okx), spy = own_spread(oky),
+                   spz = own_spread(okz);
     const int sax = bc_s(bc, 0), say = bc_s(bc, 1), saz = bc_s(bc, 2);
     double fax = 0.0, fay = 0.0, faz = 0.0;
     long long qax = 0, qay = 0, qaz = 0;
@@ -836,16 +862,31 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       home_count(j);
       ++cut;
     };
-    // empty the queues of the active lanes together, one pair per step
+    // empty the queues of the wave's lanes together, one pair per lane per
+    // step: a lane takes its next entry when the hit bits of the current
+    // one are used up
     auto drain = [&]() {
-      for (int e = 0; __ballot(e < qn); ++e)
-        if (e < qn) eval_queued((int)my_q[e * 64]);
+      int e = 0, jb = 0;
+      unsigned m = 0;
+      while (__ballot(m != 0u || e < qn)) {
+        if (m == 0u && e < qn) {
+          const unsigned v = my_q[e * 64];
+          jb = (int)(v >> 4);
+          m = v & 15u;
+          ++e;
+        }
+        if (m) {
+          const int k = __builtin_ctz(m);
+          m &= m - 1u;
+          eval_queued(jb + k);
+        }
+      }
       qn = 0;
     };
 
     const int r0 = kGroupStart[NG][grp], r1 = kGroupStart[NG][grp + 1];
     for (int r = r0; r < r1; ++r) {
-      unsigned mask = run_mask_rt(r, okx, oky, okz);
+      unsigned mask = run_mask_rt(r, spx, spy, spz);
       const int jy = iy + kRunDY[r], jz = iz + kRunDZ[r];
       const int row0 = RX * (jy + RY * jz) + ix - 2;
       double bty = 0.0, btz = 0.0;
@@ -890,10 +931,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
 #pragma unroll
             for (int k = 0; k < 4; ++k)
               dd2[k] = ddx[k] * ddx[k] + ddy[k] * ddy[k] + ddz[k] * ddz[k];
-            hits = ((j < j1 && dd2[0] <= A.rc2) ? 1u : 0u) |
-                   ((j + 1 < j1 && dd2[1] <= A.rc2) ? 2u : 0u) |
-                   ((j + 2 < j1 && dd2[2] <= A.rc2) ? 4u : 0u) |
-                   ((j + 3 < j1 && dd2[3] <= A.rc2) ? 8u : 0u);
+            hits = ((dd2[0] <= A.rc2) ? 1u : 0u) | ((dd2[1] <= A.rc2) ? 2u : 0u) |
+                   ((dd2[2] <= A.rc2) ? 4u : 0u) | ((dd2[3] <= A.rc2) ? 8u : 0u);
           } else {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -904,21 +943,26 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
               fd2[k] = __builtin_fmaf(fdx[k], fdx[k],
                                       __builtin_fmaf(fdy[k], fdy[k], fdz[k] * fdz[k]));
             }
-            hits = ((j < j1 && fd2[0] <= A.rc2f) ? 1u : 0u) |
-                   ((j + 1 < j1 && fd2[1] <= A.rc2f) ? 2u : 0u) |
-                   ((j + 2 < j1 && fd2[2] <= A.rc2f) ? 4u : 0u) |
-                   ((j + 3 < j1 && fd2[3] <= A.rc2f) ? 8u : 0u);
+            hits = ((fd2[0] <= A.rc2f) ? 1u : 0u) | ((fd2[1] <= A.rc2f) ? 2u : 0u) |
+                   ((fd2[2] <= A.rc2f) ? 4u : 0u) | ((fd2[3] <= A.rc2f) ? 8u : 0u);
+          }
+          // candidates past the range end are masked off
+          {
+            const int lim = j1 - j;
+            hits &= lim >= 4 ? 15u : ((1u << lim) - 1u);
           }
           if (LQ) {
-            // defer: write the four candidates, keep the hits
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              my_q[qn * 64] = (unsigned short)(j + k);
-              qn += (hits >> k) & 1u;
-            }
-            if (qn > QD - 4) {   // rare: this lane's queue is full
-              for (int e = 0; e < qn; ++e) eval_queued((int)my_q[e * 64]);
-              qn = 0;
+            // defer the hits of these four candidates as one entry
+            if (hits) {
+              my_q[qn * 64] = ((unsigned)j << 4) | hits;
+              if (++qn == QD) {   // rare: this lane's queue is full
+                for (int e = 0; e < qn; ++e) {
+                  const unsigned v = my_q[e * 64];
+                  for (unsigned m = v & 15u; m; m &= m - 1u)
+                    eval_queued((int)(v >> 4) + __builtin_ctz(m));
+                }
+                qn = 0;
+              }
             }
           } else {
             while (hits) {
@@ -947,14 +991,16 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       if (LQ && __ballot(qn >= QD / 2)) drain();
     }
     if (LQ) drain();
-    if constexpr (FP64) {
-      lds_add(&s_fx[a], fax);
-      lds_add(&s_fy[a], fay);
-      lds_add(&s_fz[a], faz);
-    } else {
-      lds_add(&s_qx[a], qax);
-      lds_add(&s_qy[a], qay);
-      lds_add(&s_qz[a], qaz);
+    if (active) {
+      if constexpr (FP64) {
+        lds_add(&s_fx[a], fax);
+        lds_add(&s_fy[a], fay);
+        lds_add(&s_fz[a], faz);
+      } else {
+        lds_add(&s_qx[a], qax);
+        lds_add(&s_qy[a], qay);
+        lds_add(&s_qz[a], qaz);
+      }
     }
   }
   __syncthreads();
@@ -980,12 +1026,13 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   flags = wave_or(flags);
   if ((tid & 63) == 0 && flags)
     atomicOr(&log->flags, (unsigned long long)flags);
-  Partial mine{0.0, 0.0, 0ull, cand, cut};
+  // tallies (pair counts, energy) only on tally launches
   if constexpr (ENERGY) {
+    Partial mine{0.0, 0.0, 0ull, cand, cut};
     if constexpr (FP64) mine.pe = pe64;
     else mine.pe_fix = (unsigned long long)pefix;
+    write_partial<NT>(mine, A.d.part, tile);
   }
-  write_partial<NT>(mine, A.d.part, tile);
   if (prof && tid == 0) prof[4] = clock64();
 }
 
@@ -1155,8 +1202,9 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
     else EXAMD_F(false, false, false);
   }
 #undef EXAMD_F
-  k_reduce_partials<<<t.nsys, kReduceThreads, 0, s>>>(
-      d.part, t.nsys > 1 ? t.tile_seg : nullptr, t.ntiles, log, kSumForce);
+  if (energy && !homecount)
+    k_reduce_partials<<<t.nsys, kReduceThreads, 0, s>>>(
+        d.part, t.nsys > 1 ? t.tile_seg : nullptr, t.ntiles, log, kSumForce);
 }
 
 void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,

@@ -51,7 +51,8 @@ struct ForceConfig {
   int cap = 0;        // LDS atom capacity per cell block
   int maxrb = 0;      // largest region (home + shell) in blocks
   int groups = 1;     // run groups per atom (work items = groups x atoms)
-  int lane_queue = 0; // depth of per-lane queues of deferred pairs (0, 8, 16)
+  int lane_queue = 0; // entries per lane queue of deferred pairs (0, 4, 8);
+                      // an entry holds the hits of four candidates
   int workgroups = 0; // persistent grid: workgroups that loop over the tiles
                       // (0 = one workgroup per tile)
 };

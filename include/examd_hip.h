@@ -185,8 +185,10 @@ typedef struct examd_tune {
                              system 0 in a batch) */
     int32_t block_threads;
     int32_t run_groups;
-    int32_t lane_queue;   /* depth of per-lane queues of deferred in-cutoff
-                             pairs: 0 (evaluate at once), 8 or 16 */
+    int32_t lane_queue;   /* entries of the per-lane queues of deferred
+                             in-cutoff pairs (an entry holds the hits among
+                             four candidates): 0 (evaluate at once), 4 or 8;
+                             other values round up to 4 or down to 8 */
     int32_t tile_edge;    /* set_config: when tiles are 0, every system's map
                              is T = max(1, round(nblocks_xyz / tile_edge));
                              get_config: the edge the map came from (0 when
