@@ -120,6 +120,14 @@ __device__ __forceinline__ long long fix32(float f) {
   return __double_as_longlong(t) - 0x4338000000000000LL;
 }
 
+// fix32 of a value already multiplied by 2^32 in FP32 (exact for |f| < 2^19:
+// scaling by a power of two commutes with FP32 rounding), so the FP64 step
+// is one add of the magic constant: fix32_scaled(f * 2^32) == fix32(f).
+__device__ __forceinline__ long long fix32_scaled(float fs) {
+  double t = (double)fs + kFixMagic;
+  return __double_as_longlong(t) - 0x4338000000000000LL;
+}
+
 // ---------------------------------------------------------------- grid ----
 struct Grid {
   int nb[3];        // nblocks_xyz

@@ -375,6 +375,7 @@ struct ForceArgs {
   int cap, maxrb, groups, lane_queue;
   double rc2, eps24, eps4, sig2, d2zero, d2fix;
   float rc2f, eps24f, eps4f, sig2f, d2zerof, d2fixf;
+  float eps24fs, eps4fs;   // eps24f, eps4f times 2^32
 };
 
 __host__ __device__ inline size_t align_up(size_t v, size_t a) {
@@ -382,16 +383,18 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
 }
 
 // LDS image of one cell block: region-block offsets, then atoms.  The atom
-// arrays hold cap + 4 entries: the pair loop reads four candidates at a
-// time and may look up to three entries past the last atom (masked off).
+// arrays hold cap + kChunkPad entries: the pair loop reads up to eight
+// candidates at a time and may look up to seven entries past the last atom
+// (masked off).
 constexpr int kWorkBins = 64;   // owned J blocks of an atom: 0..63
+constexpr int kChunkPad = 8;    // atom entries past the last one a chunk may read
 struct LdsLayout {
   size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_perm, off_hist,
       off_q, total;
 };
 __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
                                                 int nwaves, int qdepth) {
-  cap += 4;
+  cap += kChunkPad;
   LdsLayout L;
   size_t o = (size_t)(maxrb + 1) * 4;  // s_off
   L.off_gst = o;
@@ -597,7 +600,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   int *s_bc = reinterpret_cast<int *>(smem + LL.off_bc);
   // FP64 image: SoA doubles + packed coords; mixed image: float4 with the
   // packed coords in w
-  const int cap4 = A.cap + 4;
+  const int cap4 = A.cap + kChunkPad;
   double *s_x = reinterpret_cast<double *>(smem + LL.off_pos);
   double *s_y = s_x + cap4;
   double *s_z = s_y + cap4;
@@ -752,6 +755,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   unsigned flags = 0;
   double pe64 = 0.0;
   long long pefix = 0;
+  // candidates tested per step: eight (FP32) or four (FP64, whose
+  // differences take twice the registers)
+  constexpr int kChunk = FP64 ? 4 : 8;
+  constexpr unsigned kChunkMask = (1u << kChunk) - 1u;
   const int NG = A.groups;
   const int RWp = (RW + 63) & ~63;   // atoms per group, padded to whole waves
   const int W = NG * RWp;
@@ -813,17 +820,18 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       const float r2i = __builtin_amdgcn_rcpf(d2);
       const float s2 = A.sig2f * r2i;
       const float s6 = s2 * s2 * s2;
-      const float fs = A.eps24f * s6 * __builtin_fmaf(2.0f, s6, -1.0f) * r2i;
-      const long long q1 = fix32(-fs * dx);
-      const long long q2 = fix32(-fs * dy);
-      const long long q3 = fix32(-fs * dz);
+      // force scalar pre-multiplied by 2^32 (the fixed-point scale)
+      const float fs = A.eps24fs * s6 * __builtin_fmaf(2.0f, s6, -1.0f) * r2i;
+      const long long q1 = fix32_scaled(-fs * dx);
+      const long long q2 = fix32_scaled(-fs * dy);
+      const long long q3 = fix32_scaled(-fs * dz);
       qax += q1;
       qay += q2;
       qaz += q3;
       lds_sub(&s_qx[j], q1);
       lds_sub(&s_qy[j], q2);
       lds_sub(&s_qz[j], q3);
-      if constexpr (ENERGY) pefix += fix32(A.eps4f * s6 * (s6 - 1.0f));
+      if constexpr (ENERGY) pefix += fix32_scaled(A.eps4fs * s6 * (s6 - 1.0f));
       if (d2 < A.d2zerof) flags |= kFlagZero;
       if (d2 < A.d2fixf) flags |= kFlagFixedRange;
     };
@@ -871,8 +879,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       while (__ballot(m != 0u || e < qn)) {
         if (m == 0u && e < qn) {
           const unsigned v = my_q[e * 64];
-          jb = (int)(v >> 4);
-          m = v & 15u;
+          jb = (int)(v >> kChunk);
+          m = v & kChunkMask;
           ++e;
         }
         if (m) {
@@ -882,6 +890,19 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         }
       }
       qn = 0;
+    };
+    // defer the hits among the chunk at j as one queue entry; a full queue
+    // (rare) is emptied by its own lane
+    auto push = [&](int j, unsigned hits) {
+      my_q[qn * 64] = ((unsigned)j << kChunk) | hits;
+      if (++qn == QD) {
+        for (int e = 0; e < qn; ++e) {
+          const unsigned v = my_q[e * 64];
+          for (unsigned m = v & kChunkMask; m; m &= m - 1u)
+            eval_queued((int)(v >> kChunk) + __builtin_ctz(m));
+        }
+        qn = 0;
+      }
     };
 
     const int r0 = kGroupStart[NG][grp], r1 = kGroupStart[NG][grp + 1];
@@ -906,11 +927,35 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         const int j0 = (r == 0 && lo == 2) ? a + 1 : s_off[row0 + lo];
         const int j1 = s_off[row0 + hi + 1];
         cand += (unsigned long long)(j1 > j0 ? j1 - j0 : 0);
-        for (int j = j0; j < j1; j += 4) {
-          unsigned hits;
-          double ddx[4], ddy[4], ddz[4], dd2[4];
-          float fdx[4], fdy[4], fdz[4], fd2[4];
-          if constexpr (FP64) {
+        if constexpr (!FP64) {
+          // mixed: eight candidates per step; hits are evaluated (at once
+          // or from the queue) from the LDS positions
+          for (int j = j0; j < j1; j += kChunk) {
+            unsigned hits = 0u;
+#pragma unroll
+            for (int k = 0; k < kChunk; ++k) {
+              const float4 p = s_p4[j + k];
+              const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
+              const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+              hits |= (d2 <= A.rc2f) ? (1u << k) : 0u;
+            }
+            // candidates past the range end are masked off
+            const int lim = j1 - j;
+            hits &= lim >= kChunk ? kChunkMask : ((1u << lim) - 1u);
+            if (LQ) {
+              if (hits) push(j, hits);
+            } else {
+              while (hits) {
+                const int k = __builtin_ctz(hits);
+                hits &= hits - 1u;
+                eval_queued(j + k);
+              }
+            }
+          }
+        } else {
+          for (int j = j0; j < j1; j += kChunk) {
+            unsigned hits;
+            double ddx[4], ddy[4], ddz[4], dd2[4];
             double btx0 = 0.0, btx1 = 0.0, btx2 = 0.0, btx3 = 0.0;
             if (wraps) {
               auto btx_of = [&](int jj) {
@@ -933,56 +978,23 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
               dd2[k] = ddx[k] * ddx[k] + ddy[k] * ddy[k] + ddz[k] * ddz[k];
             hits = ((dd2[0] <= A.rc2) ? 1u : 0u) | ((dd2[1] <= A.rc2) ? 2u : 0u) |
                    ((dd2[2] <= A.rc2) ? 4u : 0u) | ((dd2[3] <= A.rc2) ? 8u : 0u);
-          } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              const float4 p = s_p4[j + k];
-              fdx[k] = p.x - xf;
-              fdy[k] = p.y - yf;
-              fdz[k] = p.z - zf;
-              fd2[k] = __builtin_fmaf(fdx[k], fdx[k],
-                                      __builtin_fmaf(fdy[k], fdy[k], fdz[k] * fdz[k]));
-            }
-            hits = ((fd2[0] <= A.rc2f) ? 1u : 0u) | ((fd2[1] <= A.rc2f) ? 2u : 0u) |
-                   ((fd2[2] <= A.rc2f) ? 4u : 0u) | ((fd2[3] <= A.rc2f) ? 8u : 0u);
-          }
-          // candidates past the range end are masked off
-          {
+            // candidates past the range end are masked off
             const int lim = j1 - j;
-            hits &= lim >= 4 ? 15u : ((1u << lim) - 1u);
-          }
-          if (LQ) {
-            // defer the hits of these four candidates as one entry
-            if (hits) {
-              my_q[qn * 64] = ((unsigned)j << 4) | hits;
-              if (++qn == QD) {   // rare: this lane's queue is full
-                for (int e = 0; e < qn; ++e) {
-                  const unsigned v = my_q[e * 64];
-                  for (unsigned m = v & 15u; m; m &= m - 1u)
-                    eval_queued((int)(v >> 4) + __builtin_ctz(m));
-                }
-                qn = 0;
-              }
-            }
-          } else {
-            while (hits) {
-              const int k = __builtin_ctz(hits);
-              hits &= hits - 1;
-              if constexpr (FP64) {
+            hits &= lim >= kChunk ? kChunkMask : ((1u << lim) - 1u);
+            if (LQ) {
+              if (hits) push(j, hits);
+            } else {
+              while (hits) {
+                const int k = __builtin_ctz(hits);
+                hits &= hits - 1;
                 const double dx = k == 0 ? ddx[0] : k == 1 ? ddx[1] : k == 2 ? ddx[2] : ddx[3];
                 const double dy = k == 0 ? ddy[0] : k == 1 ? ddy[1] : k == 2 ? ddy[2] : ddy[3];
                 const double dz = k == 0 ? ddz[0] : k == 1 ? ddz[1] : k == 2 ? ddz[2] : ddz[3];
                 const double d2 = k == 0 ? dd2[0] : k == 1 ? dd2[1] : k == 2 ? dd2[2] : dd2[3];
                 pair64(j + k, dx, dy, dz, d2);
-              } else {
-                const float dx = k == 0 ? fdx[0] : k == 1 ? fdx[1] : k == 2 ? fdx[2] : fdx[3];
-                const float dy = k == 0 ? fdy[0] : k == 1 ? fdy[1] : k == 2 ? fdy[2] : fdy[3];
-                const float dz = k == 0 ? fdz[0] : k == 1 ? fdz[1] : k == 2 ? fdz[2] : fdz[3];
-                const float d2 = k == 0 ? fd2[0] : k == 1 ? fd2[1] : k == 2 ? fd2[2] : fd2[3];
-                pair32(j + k, dx, dy, dz, d2);
+                home_count(j + k);
+                ++cut;
               }
-              home_count(j + k);
-              ++cut;
             }
           }
         }
@@ -1181,6 +1193,8 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
   A.rc2f = (float)A.rc2;
   A.eps24f = (float)A.eps24;
   A.eps4f = (float)A.eps4;
+  A.eps24fs = A.eps24f * 4294967296.0f;
+  A.eps4fs = A.eps4f * 4294967296.0f;
   A.sig2f = (float)A.sig2;
   A.d2zerof = (float)A.d2zero;
   A.d2fixf = (float)A.d2fix;
