@@ -74,15 +74,16 @@ def read_traffic():
 
 
 def parse_shape(text):
-    """'Tx,Ty,Tz,threads[,groups[,lane_queue[,workgroups]]]' -> dict, or
-    None."""
+    """'Tx,Ty,Tz,threads[,groups[,lane_queue[,workgroups[,balance]]]]' ->
+    dict, or None."""
     if not text:
         return None
     v = [int(x) for x in text.split(",")]
     return {"tiles": tuple(v[:3]), "block_threads": v[3],
             "run_groups": v[4] if len(v) > 4 else 0,
             "lane_queue": v[5] if len(v) > 5 else 0,
-            "workgroups": v[6] if len(v) > 6 else 0}
+            "workgroups": v[6] if len(v) > 6 else 0,
+            "balance": v[7] if len(v) > 7 else 0}
 
 
 def force_error(m):
@@ -178,7 +179,8 @@ def batch_bench(E, steps, warmup, loop_steps, nsys=256):
             m.set_config("mixed", block_threads=tune["block_threads"],
                          run_groups=tune["run_groups"],
                          lane_queue=tune["lane_queue"],
-                         tile_edge=tune["tile_edge"])
+                         tile_edge=tune["tile_edge"],
+                         balance=tune["balance"])
             m.run(warmup if warmup else 1, DT, "mixed")
         t0 = time.perf_counter()
         for m in handles:
@@ -298,11 +300,11 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run the short counter-collection program instead")
     ap.add_argument("--mixed-shape", default=None,
-                    help="Tx,Ty,Tz,threads,groups,lane_queue,workgroups "
-                         "instead of autotuning")
+                    help="Tx,Ty,Tz,threads,groups,lane_queue,workgroups,"
+                         "balance instead of autotuning")
     ap.add_argument("--fp64-shape", default=None,
-                    help="Tx,Ty,Tz,threads,groups,lane_queue,workgroups "
-                         "instead of autotuning")
+                    help="Tx,Ty,Tz,threads,groups,lane_queue,workgroups,"
+                         "balance instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the "

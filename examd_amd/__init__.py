@@ -68,13 +68,14 @@ class Tune(C.Structure):
     _fields_ = [("tiles", C.c_int32 * 3), ("block_threads", C.c_int32),
                 ("run_groups", C.c_int32), ("lane_queue", C.c_int32),
                 ("tile_edge", C.c_int32), ("workgroups", C.c_int32),
+                ("balance", C.c_int32), ("reserved", C.c_int32),
                 ("force_ms", C.c_double)]
 
     def as_dict(self):
         return {"tiles": tuple(self.tiles), "block_threads": self.block_threads,
                 "run_groups": self.run_groups, "lane_queue": self.lane_queue,
                 "tile_edge": self.tile_edge, "workgroups": self.workgroups,
-                "force_ms": self.force_ms}
+                "balance": self.balance, "force_ms": self.force_ms}
 
 
 class System(C.Structure):
@@ -313,7 +314,8 @@ class _Handle:
         return t.as_dict()
 
     def set_config(self, precision, tiles=(0, 0, 0), block_threads=512,
-                   run_groups=0, lane_queue=0, tile_edge=0, workgroups=0):
+                   run_groups=0, lane_queue=0, tile_edge=0, workgroups=0,
+                   balance=0):
         t = Tune()
         t.tiles = (C.c_int32 * 3)(*[int(v) for v in tiles])
         t.block_threads = int(block_threads)
@@ -321,6 +323,7 @@ class _Handle:
         t.lane_queue = int(lane_queue)
         t.tile_edge = int(tile_edge)
         t.workgroups = int(workgroups)
+        t.balance = int(balance)
         _check(lib().examd_hip_set_config(self._h, _PREC[precision],
                                           C.byref(t)), "examd_hip_set_config")
 

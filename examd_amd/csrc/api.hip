@@ -21,6 +21,7 @@
 using namespace examd;
 
 using Tri = std::array<int, 3>;
+using Cuts = std::array<std::vector<int>, 3>;   // T + 1 boundaries per axis
 
 // One system of a handle, host side.
 struct SysHost {
@@ -37,6 +38,8 @@ struct TileConfig {
   int groups = 1;         // run groups per region atom
   int lane_queue = 0;     // per-lane queue entries of deferred pairs (0 = off)
   int workgroups = 0;     // persistent force grid (0 = one per tile)
+  bool balance = false;   // boundaries at equal atom counts along each axis
+  std::vector<Cuts> cuts; // balanced boundaries per system (when balance)
   int cap = 0;            // LDS atom capacity
   int maxrb = 0;          // largest region in blocks
   bool fixed_tiles = false;
@@ -57,7 +60,9 @@ struct DevTables {
   SysDesc *sys = nullptr;
   int *tile_sys = nullptr;
   int *tile_seg = nullptr;
+  int *cuts = nullptr;
   int tile_cap = 0;
+  int cut_cap = 0;
   int ntiles = 0;
 };
 
@@ -132,6 +137,7 @@ void free_tables(DevTables &t) {
   if (t.sys) (void)hipFree(t.sys);
   if (t.tile_sys) (void)hipFree(t.tile_sys);
   if (t.tile_seg) (void)hipFree(t.tile_seg);
+  if (t.cuts) (void)hipFree(t.cuts);
   t = DevTables{};
 }
 
@@ -171,6 +177,7 @@ SysTable table(const examd_handle *h, int prec) {
   t.tile_sys = h->tab[prec].tile_sys;
   t.tile_seg = h->tab[prec].tile_seg;
   t.atom_seg = h->atom_seg;
+  t.cuts = h->tab[prec].cuts;
   t.nsys = h->nsys;
   t.ntiles = h->tab[prec].ntiles;
   return t;
@@ -183,13 +190,21 @@ int upload_tables(examd_handle *h, int prec) {
   DevTables &t = h->tab[prec];
   const int nsys = h->nsys;
   std::vector<SysDesc> sd((size_t)nsys);
-  std::vector<int> seg((size_t)nsys + 1), tsys;
+  std::vector<int> seg((size_t)nsys + 1), tsys, cuts;
   int tiles = 0;
+  const TileConfig &c = h->tc[prec];
   for (int k = 0; k < nsys; ++k) {
     sd[k].g = grid_of(h, prec, k);
     sd[k].atom0 = (long long)h->sys[k].atom0;
     sd[k].block0 = (int)h->sys[k].block0;
     sd[k].tile0 = tiles;
+    for (int a = 0; a < 3; ++a) {
+      sd[k].cut[a] = -1;
+      if (c.balance && (size_t)k < c.cuts.size() && !c.cuts[k][a].empty()) {
+        sd[k].cut[a] = (int)cuts.size();
+        cuts.insert(cuts.end(), c.cuts[k][a].begin(), c.cuts[k][a].end());
+      }
+    }
     seg[k] = tiles;
     const Tri &T = h->tc[prec].T[k];
     const int nt = T[0] * T[1] * T[2];
@@ -212,6 +227,17 @@ int upload_tables(examd_handle *h, int prec) {
                    hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(t.tile_sys, tsys.data(), tiles * sizeof(int),
                    hipMemcpyHostToDevice));
+  if (!cuts.empty()) {
+    if (t.cut_cap < (int)cuts.size()) {
+      if (t.cuts) (void)hipFree(t.cuts);
+      t.cuts = nullptr;
+      t.cut_cap = 0;
+      if (dalloc(&t.cuts, (int64_t)cuts.size())) return EXAMD_ERR_NOMEM;
+      t.cut_cap = (int)cuts.size();
+    }
+    HIPCHK(hipMemcpy(t.cuts, cuts.data(), cuts.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
   t.ntiles = tiles;
   return EXAMD_OK;
 }
@@ -242,44 +268,99 @@ int ensure_log(examd_handle *h, int64_t steps) {
   return EXAMD_OK;
 }
 
+// Boundaries of the even split of nb blocks into T tiles.
+std::vector<int> even_cuts(int nb, int T) {
+  std::vector<int> b((size_t)T + 1);
+  for (int t = 0; t < T; ++t) {
+    int lo, hi;
+    tile_range(t, nb, T, lo, hi);
+    b[t] = lo;
+    b[t + 1] = hi;
+  }
+  return b;
+}
+
+// Cell-block boundaries of system k's map in one precision.
+Cuts cuts_of(const examd_handle *h, int prec, int k, const Grid &g) {
+  const TileConfig &c = h->tc[prec];
+  Cuts b;
+  for (int a = 0; a < 3; ++a) {
+    if (c.balance && (size_t)k < c.cuts.size() && !c.cuts[k][a].empty())
+      b[a] = c.cuts[k][a];
+    else
+      b[a] = even_cuts(g.nb[a], g.T[a]);
+  }
+  return b;
+}
+
+// Block coordinate of a position along one axis (exaMD_identify_block).
+inline int host_block(double x, double bl, int nb) {
+  double q = x / bl + 1.0;
+  int v = (int)q;
+  if (v > nb) v = nb;
+  if (v < 1) v = 1;
+  return v - 1;
+}
+
+// Balanced map: along each axis, boundaries at equal shares of the atoms
+// counted per block slab (the weights METIS gets from atom counts,
+// SRC/exaMD_partitioning.f90:199-203, restated for a rectilinear map), each
+// tile at least one block wide.
+Cuts balanced_cuts(const Grid &g, const double *pos, int64_t n) {
+  Cuts b;
+  for (int a = 0; a < 3; ++a) {
+    const int nb = g.nb[a], T = g.T[a];
+    std::vector<double> w((size_t)nb, 0.0);
+    for (int64_t i = 0; i < n; ++i) w[host_block(pos[3 * i + a], g.bl, nb)] += 1.0;
+    // a small floor so empty slabs still spread over tiles
+    const double floor_w = std::max(1e-3, 0.02 * (double)n / nb);
+    double total = 0.0;
+    for (double &x : w) total += (x += floor_w);
+    std::vector<int> &c = b[a];
+    c.assign((size_t)T + 1, 0);
+    c[T] = nb;
+    double acc = 0.0;
+    int blk = 0;
+    for (int t = 1; t < T; ++t) {
+      const double target = total * t / T;
+      while (blk < nb && acc + w[blk] * 0.5 < target) acc += w[blk++];
+      int v = std::max(blk, c[t - 1] + 1);
+      v = std::min(v, nb - (T - t));
+      while (blk < v) acc += w[blk++];
+      c[t] = v;
+    }
+  }
+  return b;
+}
+
 // Largest region (tile plus one ghost block per side) in blocks.
-int max_region_blocks(const Grid &g) {
+int max_region_blocks(const Cuts &b) {
   int m[3];
   for (int a = 0; a < 3; ++a) {
     int mx = 0;
-    for (int t = 0; t < g.T[a]; ++t) {
-      int lo, hi;
-      tile_range(t, g.nb[a], g.T[a], lo, hi);
-      mx = std::max(mx, hi - lo + 2);
-    }
+    for (size_t t = 0; t + 1 < b[a].size(); ++t)
+      mx = std::max(mx, b[a][t + 1] - b[a][t] + 2);
     m[a] = mx;
   }
   return m[0] * m[1] * m[2];
 }
 
 // Largest number of atoms any tile's region holds for host positions.
-int64_t max_region_atoms(const Grid &g, const double *pos, int64_t n) {
+int64_t max_region_atoms(const Grid &g, const Cuts &cb, const double *pos,
+                         int64_t n) {
   const int nx = g.nb[0], ny = g.nb[1], nz = g.nb[2];
   std::vector<int> cnt((size_t)nx * ny * nz, 0);
   for (int64_t i = 0; i < n; ++i) {
     int b[3];
-    for (int a = 0; a < 3; ++a) {
-      double q = pos[3 * i + a] / g.bl + 1.0;
-      int v = (int)q;
-      if (v > g.nb[a]) v = g.nb[a];
-      if (v < 1) v = 1;
-      b[a] = v - 1;
-    }
+    for (int a = 0; a < 3; ++a) b[a] = host_block(pos[3 * i + a], g.bl, g.nb[a]);
     cnt[(size_t)b[0] + (size_t)nx * (b[1] + (size_t)ny * b[2])]++;
   }
   // per-axis tile boundaries
   std::vector<int> lo[3], hi[3];
   for (int a = 0; a < 3; ++a)
     for (int t = 0; t < g.T[a]; ++t) {
-      int l, h2;
-      tile_range(t, g.nb[a], g.T[a], l, h2);
-      lo[a].push_back(l);
-      hi[a].push_back(h2);
+      lo[a].push_back(cb[a][t]);
+      hi[a].push_back(cb[a][t + 1]);
     }
   int64_t best = 0;
   for (int tz = 0; tz < g.T[2]; ++tz)
@@ -324,15 +405,19 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
     int worst = 0;
     c.maxrb = 0;
     double reg_atoms = 0.0, reg_tiles = 0.0;
+    if (c.balance) c.cuts.resize((size_t)nsys);
     for (int k = 0; k < nsys; ++k) {
       const Grid g = grid_of(h, prec, k);
-      c.maxrb = std::max(c.maxrb, max_region_blocks(g));
+      const SysHost &S = h->sys[k];
+      if (c.balance) c.cuts[k] = balanced_cuts(g, pos + 3 * S.atom0, S.n);
+      const Cuts cb = cuts_of(h, prec, k, g);
+      c.maxrb = std::max(c.maxrb, max_region_blocks(cb));
       // region coordinates are packed 8 bits per axis on the device
       for (int a = 0; a < 3; ++a)
-        if ((g.nb[a] + g.T[a] - 1) / g.T[a] + 2 > 255) ok = false;
+        for (int t = 0; t < g.T[a]; ++t)
+          if (cb[a][t + 1] - cb[a][t] + 2 > 255) ok = false;
       if (!ok) break;
-      const SysHost &S = h->sys[k];
-      const int64_t m = max_region_atoms(g, pos + 3 * S.atom0, S.n);
+      const int64_t m = max_region_atoms(g, cb, pos + 3 * S.atom0, S.n);
       if (m > mra) {
         mra = m;
         worst = k;
@@ -527,7 +612,8 @@ bool same_key(const GraphKey &a, const GraphKey &b) {
   const TileConfig &x = a.cfg, &y = b.cfg;
   return x.T == y.T && x.threads == y.threads && x.groups == y.groups &&
          x.lane_queue == y.lane_queue && x.workgroups == y.workgroups &&
-         x.cap == y.cap && x.maxrb == y.maxrb;
+         x.balance == y.balance && x.cuts == y.cuts && x.cap == y.cap &&
+         x.maxrb == y.maxrb;
 }
 
 // Capture one chunk of `key.chunk` steps into an executable graph (reused
@@ -599,7 +685,7 @@ int download_positions(examd_handle *h, std::vector<double> &aos) {
 // groups, queue depth and persistent grid.
 int apply_config(examd_handle *h, int prec, const int *T, double edge,
                  int threads, int groups, int lane_queue, int workgroups,
-                 const double *pos) {
+                 bool balance, const double *pos) {
   TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
   c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 4 ? 4 : 8);
@@ -613,6 +699,8 @@ int apply_config(examd_handle *h, int prec, const int *T, double edge,
   }
   c.threads = threads;
   c.workgroups = std::max(0, workgroups);
+  c.balance = balance;
+  c.cuts.clear();
   c.fixed_tiles = true;
   c.fixed_groups = groups;
   int st = choose_caps(h, prec, pos);
@@ -1128,6 +1216,8 @@ int examd_hip_get_config(examd_handle *h, int precision, examd_tune *out) {
   out->lane_queue = c.lane_queue;
   out->tile_edge = c.edge;
   out->workgroups = c.workgroups;
+  out->balance = c.balance ? 1 : 0;
+  out->reserved = 0;
   out->force_ms = 0.0;
   return EXAMD_OK;
 }
@@ -1142,7 +1232,8 @@ int examd_hip_set_config(examd_handle *h, int precision,
   if (thr != 256 && thr != 512 && thr != 1024) return EXAMD_ERR_ARG;
   if (cfg->run_groups < 0 || cfg->run_groups > 4) return EXAMD_ERR_ARG;
   if (cfg->lane_queue < 0 || cfg->lane_queue > 16) return EXAMD_ERR_ARG;
-  if (cfg->workgroups < 0) return EXAMD_ERR_ARG;
+  if (cfg->workgroups < 0 || cfg->balance < 0 || cfg->balance > 1)
+    return EXAMD_ERR_ARG;
   const bool explicit_tiles =
       cfg->tiles[0] > 0 && cfg->tiles[1] > 0 && cfg->tiles[2] > 0;
   if (explicit_tiles) {
@@ -1157,7 +1248,8 @@ int examd_hip_set_config(examd_handle *h, int precision,
   if (st) return st;
   return apply_config(h, precision, explicit_tiles ? cfg->tiles : nullptr,
                       (double)cfg->tile_edge, thr, cfg->run_groups,
-                      cfg->lane_queue, cfg->workgroups, pos.data());
+                      cfg->lane_queue, cfg->workgroups, cfg->balance != 0,
+                      pos.data());
 }
 
 int examd_hip_autotune(examd_handle *h, int precision, int trials,
@@ -1212,9 +1304,9 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
     return EXAMD_OK;
   };
   auto consider = [&](const int *T, double edge, int thr, int grp, int lq,
-                      int wgs) -> int {
-    if (apply_config(h, precision, T, edge, thr, grp, lq, wgs, pos.data()) !=
-        EXAMD_OK)
+                      int wgs, bool bal) -> int {
+    if (apply_config(h, precision, T, edge, thr, grp, lq, wgs, bal,
+                     pos.data()) != EXAMD_OK)
       return EXAMD_OK;
     double ms;
     int e = time_shape(&ms);
@@ -1235,24 +1327,32 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
             for (int a = 0; a < 3; ++a)
               T[a] = std::max(1, (int)std::lround(h->sys[0].g.nb[a] / sizes[si]));
           }
-          st = consider(single ? T : nullptr, sizes[si], thr, grp, lq, 0);
+          st = consider(single ? T : nullptr, sizes[si], thr, grp, lq, 0, false);
           if (st) {
             h->timing = save_timing;
             return st;
           }
         }
-  // persistent grids for the best shape: a few workgroups per CU
+  // for the best shape: balanced boundaries, then persistent grids of a
+  // few workgroups per CU
   if (best_ms < 1e300) {
+    TileConfig b = bestc;
+    st = consider(b.edge == 0 ? b.T[0].data() : nullptr, (double)b.edge,
+                  b.threads, b.fixed_groups, b.lane_queue, 0, true);
+    if (st) {
+      h->timing = save_timing;
+      return st;
+    }
+    b = bestc;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                               h->dev) != hipSuccess || cus <= 0)
       cus = 256;
-    const TileConfig b = bestc;
     for (int per = 1; per <= 4; per *= 2) {
       const int wgs = cus * per;
       if (wgs >= (int)tiles_of(b)) break;
       st = consider(b.edge == 0 ? b.T[0].data() : nullptr, (double)b.edge,
-                    b.threads, b.fixed_groups, b.lane_queue, wgs);
+                    b.threads, b.fixed_groups, b.lane_queue, wgs, b.balance);
       if (st) {
         h->timing = save_timing;
         return st;
@@ -1270,7 +1370,8 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
   }
   st = apply_config(h, precision, bestc.edge == 0 ? bestc.T[0].data() : nullptr,
                     (double)bestc.edge, bestc.threads, bestc.fixed_groups,
-                    bestc.lane_queue, bestc.workgroups, pos.data());
+                    bestc.lane_queue, bestc.workgroups, bestc.balance,
+                    pos.data());
   h->forces_valid = -1;
   if (st) return st;
   if (best) {

@@ -144,6 +144,8 @@ struct SysDesc {
   long long atom0;
   int block0;
   int tile0;
+  int cut[3];   // per axis: offset of the T + 1 cell-block boundaries in the
+                // launch's boundary array, or -1 for the even split
 };
 
 // Home-block range [lo, hi) of tile t along one axis: nb blocks shared by T

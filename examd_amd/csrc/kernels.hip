@@ -584,9 +584,20 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   const int tJ = (lt / g.T[0]) % g.T[1];
   const int tK = lt / (g.T[0] * g.T[1]);
   int h0x, h1x, h0y, h1y, h0z, h1z;
-  tile_range(tI, g.nb[0], g.T[0], h0x, h1x);
-  tile_range(tJ, g.nb[1], g.T[1], h0y, h1y);
-  tile_range(tK, g.nb[2], g.T[2], h0z, h1z);
+  // home-block range of the tile per axis: the even split, or the
+  // boundaries of a balanced map
+  auto bounds = [&](int axis, int t, int &lo, int &hi) {
+    const int c = sd.cut[axis];
+    if (c < 0) {
+      tile_range(t, g.nb[axis], g.T[axis], lo, hi);
+    } else {
+      lo = A.t.cuts[c + t];
+      hi = A.t.cuts[c + t + 1];
+    }
+  };
+  bounds(0, tI, h0x, h1x);
+  bounds(1, tJ, h0y, h1y);
+  bounds(2, tK, h0z, h1z);
   const int tx = h1x - h0x, ty = h1y - h0y, tz = h1z - h0z;
   const int RX = tx + 2, RY = ty + 2, RZ = tz + 2;
   const int RXY = RX * RY, NRB = RXY * RZ;

@@ -37,6 +37,7 @@ struct SysTable {
   const int *tile_sys = nullptr;       // system of each tile (ntiles)
   const int *tile_seg = nullptr;       // first tile of each system (nsys + 1)
   const long long *atom_seg = nullptr; // first atom of each system (nsys + 1)
+  const int *cuts = nullptr;           // cell-block boundaries of balanced maps
   int nsys = 1;
   int ntiles = 0;
 };

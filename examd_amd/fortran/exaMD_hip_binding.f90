@@ -42,6 +42,8 @@ Module exaMD_hip_binding
      Integer(c_int32_t) :: lane_queue
      Integer(c_int32_t) :: tile_edge
      Integer(c_int32_t) :: workgroups
+     Integer(c_int32_t) :: balance
+     Integer(c_int32_t) :: reserved
      Real(c_double)     :: force_ms
   End Type examd_tune
 

@@ -195,6 +195,11 @@ typedef struct examd_tune {
                              tiles were given) */
     int32_t workgroups;   /* persistent force grid: workgroups that loop over
                              the cell blocks (0 = one workgroup per block) */
+    int32_t balance;      /* 1: cell-block boundaries along each axis at equal
+                             shares of the atoms (a load-balanced map for
+                             uneven density, set from the current positions);
+                             0: even split of the blocks */
+    int32_t reserved;
     double force_ms;      /* mean force-kernel time measured by autotune */
 } examd_tune;
 

@@ -173,3 +173,55 @@ def test_batch_autotune(E, oracle):
     pos, _, box = sy[3]
     ref, _, _ = oracle.forces(pos, box, 2.5)
     assert _rel(f[off[3]:off[4]], ref) < 2e-5
+
+
+def _small_slab(E, seed=3):
+    box = np.array([24.0, 12.0, 12.0])
+    pos, vel = E.generate_slab(box, 8.0, 16.0, 0.8, 0.05, min_sep=0.85,
+                               temperature=0.85, seed=seed)
+    return pos, vel, box
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_balanced_map_matches_oracle(E, oracle, prec):
+    """A map whose cell-block boundaries sit at equal atom shares (a slab:
+    dense middle, vapour outside) computes every pair exactly once: FP64
+    forces, pair counts and per-home-block ownership against the oracle,
+    the mixed path within its FP32 bars."""
+    print(f"[balanced map] {prec}", flush=True)
+    pos, vel, box = _small_slab(E)
+    ref, t, hc = oracle.forces(pos, box, 2.5, home_counts=True)
+    with E.MidpointLJ(len(pos), box, 2.5) as m:
+        m.set_state(pos, vel)
+        m.set_config(prec, (6, 2, 2), 256, balance=1)
+        assert m.get_config(prec)["balance"] == 1
+        f, s, _ = m.forces(prec)
+        got = m.home_pair_counts(prec)
+        m.set_config(prec, (6, 2, 2), 256, balance=0)
+        f0, _, _ = m.forces(prec)
+    if prec == "fp64":
+        assert _rel(f, ref) < FP64_TOL
+        assert (s.pairs_cand, s.pairs_cut) == (t.cand, t.cut)
+        np.testing.assert_array_equal(got, hc)
+    else:
+        assert _rel(f, ref) < 2e-5
+        assert s.pairs_cand == t.cand and abs(s.pairs_cut - t.cut) <= 2
+        assert np.abs(got - hc).sum() <= 4
+        assert _rel(f, f0) < 2e-5
+
+
+def test_balanced_batch_equals_standalone(E):
+    """Balanced maps in a batch give each system the map it gets alone:
+    mixed forces bit-identical."""
+    print("[balanced batch]", flush=True)
+    sy = [_small_slab(E, seed=s) for s in (4, 5)] + _systems(E)[:2]
+    with _batch(E, sy) as b:
+        b.set_config("mixed", block_threads=256, tile_edge=4, balance=1)
+        fb, _, _ = b.forces("mixed")
+        off = b.offsets
+    for k, (pos, vel, box) in enumerate(sy):
+        with E.MidpointLJ(len(pos), box, 2.5) as m:
+            m.set_state(pos, vel)
+            m.set_config("mixed", block_threads=256, tile_edge=4, balance=1)
+            f1, _, _ = m.forces("mixed")
+        assert np.array_equal(fb[off[k]:off[k + 1]], f1), k
