@@ -242,6 +242,37 @@ def slab_bench(E, steps, warmup):
     }
 
 
+def config1(E, steps=100):
+    """The reference's CPU-runnable case: 32,000 atoms (20^3 FCC, rho 0.8442,
+    T* 0.72), rc 2.5, 4x4x4 cell blocks, FP64, 100 velocity-Verlet steps;
+    the GPU FP64 path against the oracle on the same seeded input
+    (atom-steps/s of both and the largest position difference after the
+    run)."""
+    from oracle import oracle as O
+    pos, vel, box = E.generate_fcc(20, RHO, TEMP, seed=32000)
+    with E.MidpointLJ(len(pos), box, RC, tiles=(4, 4, 4),
+                      graph_chunk=steps) as m:
+        m.set_state(pos, vel)
+        m.run(steps, DT, "fp64")        # parity run (captures the chunk)
+        gpos, gvel = m.get_state()
+        t0 = time.perf_counter()        # timed: the same chunk replayed
+        m.run(steps, DT, "fp64")
+        gpu_s = time.perf_counter() - t0
+    f0, _, _ = O.forces(pos, box, RC)
+    t0 = time.perf_counter()
+    opos, ovel = O.run_vv(pos, vel, box, RC, DT, steps, forces0=f0)[:2]
+    cpu_s = time.perf_counter() - t0
+    d = gpos - opos
+    d -= np.round(d / box) * box
+    n = len(pos)
+    return {"workload": "32,000 atoms, 20^3 FCC, rho 0.8442, T* 0.72, rc 2.5, "
+                        "4x4x4 cell blocks, FP64, dt 0.005",
+            "steps": steps, "gpu_fp64_atom_steps_per_s": n * steps / gpu_s,
+            "cpu_oracle_atom_steps_per_s": n * steps / cpu_s,
+            "max_position_diff_vs_oracle": float(np.abs(d).max()),
+            "max_velocity_diff_vs_oracle": float(np.abs(gvel - ovel).max())}
+
+
 def cpu_baseline(pos, vel, box, nsteps):
     """The oracle (a 1-core C restatement of the reference path, FP64) on a
     bounded sample of the same workload: nsteps VV steps of the full system,
@@ -293,6 +324,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--config1-steps", type=int, default=100)
     ap.add_argument("--loop-steps", type=int, default=200)
     ap.add_argument("--slab-steps", type=int, default=2000)
     ap.add_argument("--graph-chunk", type=int, default=100,
@@ -348,6 +380,10 @@ def main():
     if args.slab_steps > 0:
         slab = slab_bench(E, args.slab_steps, args.warmup)
 
+    c1 = None
+    if not args.no_cpu and args.config1_steps > 0:
+        c1 = config1(E, args.config1_steps)
+
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         v = cpu_baseline(pos, vel, box, args.cpu_steps)
@@ -401,6 +437,7 @@ def main():
             "graph_chunk_steps": args.graph_chunk,
             "batch": batch,
             "slab": slab,
+            "config1": c1,
             "parallelism": "1 GPU",
         },
         "roofline": {

@@ -216,6 +216,30 @@ def generate_slab(box, slab_lo, slab_hi, rho_slab, rho_vapour, min_sep=0.8,
     return pos, vel
 
 
+def write_checkpoint(path, pos, vel, box, step=0):
+    """MD checkpoint in the Fortran layer's format (exaMD_write_checkpoint):
+    'n step', the box, then x y z vx vy vz per atom with 17 significant
+    digits, so that a restart continues bit for bit."""
+    pos = np.asarray(pos, dtype=np.float64)
+    vel = np.asarray(vel, dtype=np.float64)
+    with open(path, "w") as fh:
+        fh.write(f"{len(pos)} {int(step)}\n")
+        fh.write("".join(f"{v:25.16E}" for v in np.reshape(box, 3)) + "\n")
+        for p, v in zip(pos, vel):
+            fh.write("".join(f"{x:25.16E}" for x in (*p, *v)) + "\n")
+
+
+def read_checkpoint(path):
+    """(pos, vel, box, step) from a checkpoint file."""
+    with open(path) as fh:
+        n, step = (int(x) for x in fh.readline().split())
+        box = np.array([float(x) for x in fh.readline().split()])
+        data = np.loadtxt(fh, dtype=np.float64, ndmin=2)
+    if data.shape != (n, 6):
+        raise ValueError(f"{path}: expected {n} atoms with 6 values")
+    return data[:, :3].copy(), data[:, 3:].copy(), box, step
+
+
 class _Handle:
     """Common methods of a handle of the C ABI (one system or a batch)."""
 

@@ -225,6 +225,44 @@ Contains
     velocities = v
     globalbox = box
   End Subroutine exaMD_generate_fcc
+  !! Checkpoint of an MD state (no reference counterpart: the reference
+  !! file holds positions only): line 1 the atom count and the step, line 2
+  !! the box, then one line per atom with x y z vx vy vz, written with 17
+  !! significant digits so that a restart continues bit for bit.
+  Subroutine exaMD_write_checkpoint(filename, pos, vel, natms, step)
+    Character(len=*), Intent(in) :: filename
+    Real(kind=PRC), Intent(in) :: pos(:,:), vel(:,:)
+    Integer, Intent(in) :: natms
+    Integer(8), Intent(in) :: step
+    Integer :: i
+    Open(51, file=filename, status='replace')
+    Write(51, '(I0,1X,I0)') natms, step
+    Write(51, '(3ES25.16E3)') globalbox
+    Do i = 1, natms
+       Write(51, '(6ES25.16E3)') pos(1:3, i), vel(1:3, i)
+    End Do
+    Close(51)
+  End Subroutine exaMD_write_checkpoint
+
+  !! Reads a checkpoint written by exaMD_write_checkpoint; sets globalbox.
+  Subroutine exaMD_read_checkpoint(filename, pos, vel, natms, step)
+    Character(len=*), Intent(in) :: filename
+    Real(kind=PRC), Allocatable, Intent(inout) :: pos(:,:), vel(:,:)
+    Integer, Intent(out) :: natms
+    Integer(8), Intent(out) :: step
+    Integer :: i
+    Open(51, file=filename, status='old')
+    Read(51, *) natms, step
+    Read(51, *) globalbox
+    If (Allocated(pos)) Deallocate(pos)
+    If (Allocated(vel)) Deallocate(vel)
+    Allocate(pos(3, natms), vel(3, natms))
+    Do i = 1, natms
+       Read(51, *) pos(1:3, i), vel(1:3, i)
+    End Do
+    Close(51)
+  End Subroutine exaMD_read_checkpoint
+
 End Module exaMD_input_formats
 
 

@@ -13,6 +13,8 @@
 !!   -dt h           time step (default 0.005)
 !!   -precision p    fp64 | mixed (default fp64)
 !!   -seed s         generator seed (default 1)
+!!   -restart file   start from a checkpoint (positions, velocities, box, step)
+!!   -checkpoint file write the state after the MD steps as a checkpoint
 
 Program examd_md
 
@@ -37,7 +39,8 @@ Program examd_md
   Real(kind=PRC), Allocatable :: particle_positions(:,:), velocities(:,:)
   Real(c_double), Allocatable :: vflat(:,:), pos_out(:,:), vel_out(:,:)
   Type(examd_stats), Allocatable :: stats(:)
-  Character(len=32) :: arg
+  Character(len=256) :: arg, restart_file, checkpoint_file
+  Integer(8) :: step0
   Logical :: inputcase, overlapcase, writeout
   Integer(8) :: c0, c1, crate
   Real(c_double) :: force_ms, e0, drift
@@ -54,6 +57,9 @@ Program examd_md
   inputcase = .False.
   overlapcase = .False.
   writeout = .False.
+  restart_file = ''
+  checkpoint_file = ''
+  step0 = 0
 
   argc = command_argument_count()
   ia = 1
@@ -98,13 +104,22 @@ Program examd_md
         ia = ia + 1
         Call get_command_argument(ia, arg)
         If (Trim(arg) == 'mixed') exaMD_precision = EXAMD_PREC_MIXED
+     Case ('-restart')
+        ia = ia + 1
+        Call get_command_argument(ia, restart_file)
+     Case ('-checkpoint')
+        ia = ia + 1
+        Call get_command_argument(ia, checkpoint_file)
      End Select
      ia = ia + 1
   End Do
   block_length = half * cutoff
 
   ! positions (and velocities) before the domain: the FCC box follows N, rho
-  If (inputcase) Then
+  If (Len_Trim(restart_file) > 0) Then
+     Call exaMD_read_checkpoint(Trim(restart_file), particle_positions, &
+          velocities, natms_tot, step0)
+  Else If (inputcase) Then
      Call exaMD_read_params(natms_tot)
   Else
      nc = Nint((Real(natms_tot, PRC) / 4.0_PRC)**(1.0_PRC / 3.0_PRC))
@@ -113,10 +128,12 @@ Program examd_md
         Call exaMD_abort(103)
      End If
   End If
-  Allocate(particle_positions(3, natms_tot), velocities(3, natms_tot))
-  velocities = zero
-  If (.Not. inputcase) Call exaMD_generate_fcc(nc, rho, temp, seed, &
-       particle_positions, velocities)
+  If (Len_Trim(restart_file) == 0) Then
+     Allocate(particle_positions(3, natms_tot), velocities(3, natms_tot))
+     velocities = zero
+     If (.Not. inputcase) Call exaMD_generate_fcc(nc, rho, temp, seed, &
+          particle_positions, velocities)
+  End If
 
   Call exaMD_domain_create(block_length, nblocks, nblocks_xyz)
 
@@ -134,7 +151,7 @@ Program examd_md
   Write (*, *) '-----------------------------------------------------------------------'
 
   Allocate(ntot(nblocks), first_particle(nblocks), next_particle(natms_tot))
-  If (inputcase) Then
+  If (inputcase .And. Len_Trim(restart_file) == 0) Then
      Call exaMD_read_particles(particle_positions, natms_tot, ntot, &
           first_particle, next_particle, nblocks, nblocks_xyz, block_length)
   Else
@@ -218,6 +235,23 @@ Program examd_md
      Write (*, '(A,ES12.4)') ' atom-steps/s:', Real(natms_tot, 8) * &
           Real(nsteps, 8) / (Real(c1 - c0, 8) / Real(crate, 8))
      Write (*, '(A,F10.4)') ' Force kernel ms/step:', force_ms / Real(nsteps, 8)
+     If (Len_Trim(checkpoint_file) > 0) Then
+        ! the run returns atoms in the grid's order; put them back in input
+        ! order for the checkpoint
+        s = 0
+        Do b = 1, nblocks
+           j = first_particle(b)
+           Do i = 1, ntot(b)
+              s = s + 1
+              particle_positions(:, j) = pos_out(:, s)
+              velocities(:, j) = vel_out(:, s)
+              j = next_particle(j)
+           End Do
+        End Do
+        Call exaMD_write_checkpoint(Trim(checkpoint_file), particle_positions, &
+             velocities, natms_tot, step0 + nsteps)
+        Write (*, '(A,A)') ' Checkpoint written: ', Trim(checkpoint_file)
+     End If
   End If
 
   Deallocate(kktype, ltpvdw, prmvdw)

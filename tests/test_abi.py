@@ -134,3 +134,14 @@ def test_bad_arguments_rejected(E):
     with pytest.raises(E.ExamdError) as ei:
         E.domain([10.0, -1.0, 10.0], 1.25)
     assert ei.value.status == E.ERR_ARG
+
+
+def test_checkpoint_round_trip_is_exact(E, tmp_path):
+    pos, vel, box = E.generate_fcc(3, 0.8442, 1.3, perturb=0.1, seed=4)
+    pos[0, 0] = np.nextafter(box[0], 0)     # extreme values survive
+    vel[1, 2] = -1e-300
+    E.write_checkpoint(tmp_path / "s.chk", pos, vel, box, step=1234)
+    p2, v2, b2, st = E.read_checkpoint(tmp_path / "s.chk")
+    assert st == 1234
+    assert np.array_equal(p2, pos) and np.array_equal(v2, vel)
+    assert np.array_equal(b2, box)

@@ -67,3 +67,27 @@ def test_driver_force_check_and_md(tmp_path, oracle):
                             out).group(1))
     assert abs(drift) < 1e-3
     assert os.path.exists(tmp_path / "exaMD_particles.dat")
+
+
+@pytest.mark.gpu
+def test_driver_checkpoint_restart_is_exact(tmp_path):
+    """Mixed precision: 40 steps in one run and 20 + 20 steps through a
+    checkpoint (positions and velocities written with 17 digits, the box and
+    the step) end in the same checkpoint, bit for bit."""
+    drv = _driver()
+    common = ["-atoms", "4000", "-precision", "mixed", "-temp", "1.0"]
+
+    def run(*extra):
+        r = subprocess.run([drv, *common, *extra], cwd=tmp_path,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+        return r.stdout
+
+    run("-steps", "40", "-checkpoint", "a.chk")
+    run("-steps", "20", "-checkpoint", "b1.chk")
+    out = run("-restart", "b1.chk", "-steps", "20", "-checkpoint", "b2.chk")
+    assert "Checkpoint written" in out
+    a = open(tmp_path / "a.chk").read().split("\n")
+    b = open(tmp_path / "b2.chk").read().split("\n")
+    assert a[0].split() == ["4000", "40"] and b[0].split() == ["4000", "40"]
+    assert a[1:] == b[1:]

@@ -285,3 +285,25 @@ def test_graph_chunk_timing_counts_launches(E):
         ms, launches = m.timing(enable=False, reset=True)
     assert launches == 60
     assert ms > 0.0
+
+
+def test_checkpoint_restart_continues_exactly(E, tmp_path):
+    """Mixed precision: a run continued from a checkpoint in a new handle
+    matches the uninterrupted run bit for bit."""
+    pos, vel, box = E.generate_fcc(10, RHO, 1.0, seed=31)
+    with E.MidpointLJ(len(pos), box, 2.5) as m:
+        m.set_state(pos, vel)
+        m.run(40, 0.005, "mixed")
+        p_ref, v_ref = m.get_state()
+    with E.MidpointLJ(len(pos), box, 2.5) as m:
+        m.set_state(pos, vel)
+        m.run(20, 0.005, "mixed")
+        p20, v20 = m.get_state()
+    E.write_checkpoint(tmp_path / "c.chk", p20, v20, box, step=20)
+    p, v, b, step = E.read_checkpoint(tmp_path / "c.chk")
+    assert step == 20
+    with E.MidpointLJ(len(p), b, 2.5) as m:
+        m.set_state(p, v)
+        m.run(20, 0.005, "mixed")
+        p40, v40 = m.get_state()
+    assert np.array_equal(p40, p_ref) and np.array_equal(v40, v_ref)
