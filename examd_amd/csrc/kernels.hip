@@ -23,14 +23,6 @@ namespace {
 
 constexpr int kBinThreads = 256;
 
-template <int I, int N, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-  if constexpr (I < N) {
-    f(std::integral_constant<int, I>{});
-    static_for<I + 1, N>(f);
-  }
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -429,27 +421,6 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
 __device__ __forceinline__ unsigned own_bits(int i, int t) {
   return ((i + 1 >= 1 && i + 1 <= t) ? 1u : 0u) | ((i >= 1 && i <= t) ? 2u : 0u) |
          ((i - 1 >= 1 && i - 1 <= t) ? 4u : 0u);
-}
-
-// Ownership mask of run R (bit k <-> dx = k - 2) for an I block with
-// per-axis own_bits.  The per-axis masks are compile-time constants derived
-// from the reference table.
-template <int R>
-__device__ __forceinline__ unsigned run_mask(unsigned ox, unsigned oy,
-                                             unsigned oz) {
-  constexpr unsigned x0 = axis_mask(R, 0, -1), x1 = axis_mask(R, 0, 0),
-                     x2 = axis_mask(R, 0, 1);
-  constexpr unsigned y0 = axis_mask(R, 1, -1), y1 = axis_mask(R, 1, 0),
-                     y2 = axis_mask(R, 1, 1);
-  constexpr unsigned z0 = axis_mask(R, 2, -1), z1 = axis_mask(R, 2, 0),
-                     z2 = axis_mask(R, 2, 1);
-  unsigned mx = ((ox & 1u) ? x0 : 0u) | ((ox & 2u) ? x1 : 0u) |
-                ((ox & 4u) ? x2 : 0u);
-  unsigned my = ((oy & 1u) ? y0 : 0u) | ((oy & 2u) ? y1 : 0u) |
-                ((oy & 4u) ? y2 : 0u);
-  unsigned mz = ((oz & 1u) ? z0 : 0u) | ((oz & 2u) ? z1 : 0u) |
-                ((oz & 4u) ? z2 : 0u);
-  return mx & my & mz;
 }
 
 // Per-run ownership masks for runtime run indices: kRunAxisMask[r][axis][o+1].
