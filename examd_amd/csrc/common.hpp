@@ -128,6 +128,17 @@ __device__ __forceinline__ long long fix32_scaled(float fs) {
   return __double_as_longlong(t) - 0x4338000000000000LL;
 }
 
+// fix32_scaled with the magic constant's bits left in (bits 51 and up of the
+// constant; its bits below 51 are zero).  The low 51 bits of a sum of such
+// values are the low 51 bits of the sum of the fixed-point values, whatever
+// the count, so a sum of magnitude below 2^50 is recovered by fix_sext51.
+__device__ __forceinline__ long long fix32_scaled_raw(float fs) {
+  return __double_as_longlong((double)fs + kFixMagic);
+}
+__device__ __forceinline__ long long fix_sext51(long long v) {
+  return (long long)((unsigned long long)v << 13) >> 13;
+}
+
 // ---------------------------------------------------------------- grid ----
 struct Grid {
   int nb[3];        // nblocks_xyz

@@ -587,12 +587,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   double *s_y = s_x + cap4;
   double *s_z = s_y + cap4;
   float4 *s_p4 = reinterpret_cast<float4 *>(smem + LL.off_pos);
-  double *s_fx = reinterpret_cast<double *>(smem + LL.off_f);
-  double *s_fy = s_fx + cap4;
-  double *s_fz = s_fy + cap4;
-  long long *s_qx = reinterpret_cast<long long *>(smem + LL.off_f);
-  long long *s_qy = s_qx + cap4;
-  long long *s_qz = s_qy + cap4;
+  // force sums interleaved per atom (x, y, z), so that a pair's three LDS
+  // atomics share one address: FP64 (s_f) or fixed point (s_q)
+  double *s_f = reinterpret_cast<double *>(smem + LL.off_f);
+  long long *s_q = reinterpret_cast<long long *>(smem + LL.off_f);
   unsigned long long *prof =
       A.d.prof ? A.d.prof + (size_t)tile * kProfSlots : nullptr;
   if (prof && tid == 0) prof[0] = clock64();
@@ -673,16 +671,16 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       s_y[s] = Y;
       s_z[s] = Z;
       s_bc[s] = bc;
-      s_fx[s] = 0.0;
-      s_fy[s] = 0.0;
-      s_fz[s] = 0.0;
+      s_f[3 * s] = 0.0;
+      s_f[3 * s + 1] = 0.0;
+      s_f[3 * s + 2] = 0.0;
     } else {
       s_p4[s] = make_float4((float)((X + sx * g.L[0]) - ox),
                             (float)((Y + sy * g.L[1]) - oy),
                             (float)((Z + sz * g.L[2]) - oz), __int_as_float(bc));
-      s_qx[s] = 0;
-      s_qy[s] = 0;
-      s_qz[s] = 0;
+      s_q[3 * s] = 0;
+      s_q[3 * s + 1] = 0;
+      s_q[3 * s + 2] = 0;
     }
   }
   __syncthreads();
@@ -735,6 +733,9 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   //    atomics.
   unsigned long long cand = 0, cut = 0;
   unsigned flags = 0;
+  // mixed: smallest in-cutoff d^2 seen by the lane; the near-overlap flags
+  // are set from it once, after the pair loop
+  unsigned d2min32 = 0x7f7fffffu;   // bits of a non-negative float
   double pe64 = 0.0;
   long long pefix = 0;
   // candidates tested per step: eight (FP32) or four (FP64, whose
@@ -792,9 +793,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       fax += f1;
       fay += f2;
       faz += f3;
-      lds_add(&s_fx[j], -f1);
-      lds_add(&s_fy[j], -f2);
-      lds_add(&s_fz[j], -f3);
+      double *fj = reinterpret_cast<double *>(
+          reinterpret_cast<unsigned char *>(s_f) + __umul24((unsigned)j, 24u));
+      lds_add(fj, -f1);
+      lds_add(fj + 1, -f2);
+      lds_add(fj + 2, -f3);
       if constexpr (ENERGY) pe64 += A.eps4 * (d12 - d6);
       if (d2 < A.d2zero) flags |= kFlagZero;
     };
@@ -804,18 +807,21 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       const float s6 = s2 * s2 * s2;
       // force scalar pre-multiplied by 2^32 (the fixed-point scale)
       const float fs = A.eps24fs * s6 * __builtin_fmaf(2.0f, s6, -1.0f) * r2i;
-      const long long q1 = fix32_scaled(-fs * dx);
-      const long long q2 = fix32_scaled(-fs * dy);
-      const long long q3 = fix32_scaled(-fs * dz);
+      // biased fixed point: the LDS sums are sign-extended from bit 50
+      // when the forces leave the tile
+      const long long q1 = fix32_scaled_raw(-fs * dx);
+      const long long q2 = fix32_scaled_raw(-fs * dy);
+      const long long q3 = fix32_scaled_raw(-fs * dz);
       qax += q1;
       qay += q2;
       qaz += q3;
-      lds_sub(&s_qx[j], q1);
-      lds_sub(&s_qy[j], q2);
-      lds_sub(&s_qz[j], q3);
+      long long *qj = reinterpret_cast<long long *>(
+          reinterpret_cast<unsigned char *>(s_q) + __umul24((unsigned)j, 24u));
+      lds_sub(qj, q1);
+      lds_sub(qj + 1, q2);
+      lds_sub(qj + 2, q3);
       if constexpr (ENERGY) pefix += fix32_scaled(A.eps4fs * s6 * (s6 - 1.0f));
-      if (d2 < A.d2zerof) flags |= kFlagZero;
-      if (d2 < A.d2fixf) flags |= kFlagFixedRange;
+      d2min32 = min(d2min32, __float_as_uint(d2));
     };
     auto home_count = [&](int j) {
       if constexpr (HOMECNT) {
@@ -913,17 +919,21 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
           // mixed: eight candidates per step; hits are evaluated (at once
           // or from the queue) from the LDS positions
           for (int j = j0; j < j1; j += kChunk) {
-            unsigned hits = 0u;
+            // rc^2 - d^2 by three FMAs; its sign bit (set beyond the cutoff)
+            // is shifted into the miss mask, last candidate first, so that
+            // bit k ends up holding candidate k
+            unsigned miss = 0u;
 #pragma unroll
-            for (int k = 0; k < kChunk; ++k) {
+            for (int k = kChunk - 1; k >= 0; --k) {
               const float4 p = s_p4[j + k];
               const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
-              const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-              hits |= (d2 <= A.rc2f) ? (1u << k) : 0u;
+              const float t = __builtin_fmaf(
+                  -dx, dx, __builtin_fmaf(-dy, dy, __builtin_fmaf(-dz, dz, A.rc2f)));
+              miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(t), 31);
             }
             // candidates past the range end are masked off
             const int lim = j1 - j;
-            hits &= lim >= kChunk ? kChunkMask : ((1u << lim) - 1u);
+            unsigned hits = ~miss & ((1u << min(lim, kChunk)) - 1u);
             if (LQ) {
               if (hits) push(j, hits);
             } else {
@@ -987,13 +997,13 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     if (LQ) drain();
     if (active) {
       if constexpr (FP64) {
-        lds_add(&s_fx[a], fax);
-        lds_add(&s_fy[a], fay);
-        lds_add(&s_fz[a], faz);
+        lds_add(&s_f[3 * a], fax);
+        lds_add(&s_f[3 * a + 1], fay);
+        lds_add(&s_f[3 * a + 2], faz);
       } else {
-        lds_add(&s_qx[a], qax);
-        lds_add(&s_qy[a], qay);
-        lds_add(&s_qz[a], qaz);
+        lds_add(&s_q[3 * a], qax);
+        lds_add(&s_q[3 * a + 1], qay);
+        lds_add(&s_q[3 * a + 2], qaz);
       }
     }
   }
@@ -1005,18 +1015,23 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   for (int s = tid; s < R; s += NT) {
     int id = s_id[s];
     if constexpr (FP64) {
-      double f1 = s_fx[s], f2 = s_fy[s], f3 = s_fz[s];
+      double f1 = s_f[3 * s], f2 = s_f[3 * s + 1], f3 = s_f[3 * s + 2];
       if (f1 != 0.0) unsafeAtomicAdd(&A.d.fx[id], f1);
       if (f2 != 0.0) unsafeAtomicAdd(&A.d.fy[id], f2);
       if (f3 != 0.0) unsafeAtomicAdd(&A.d.fz[id], f3);
     } else {
-      long long q1 = s_qx[s], q2 = s_qy[s], q3 = s_qz[s];
+      long long q1 = fix_sext51(s_q[3 * s]), q2 = fix_sext51(s_q[3 * s + 1]),
+                q3 = fix_sext51(s_q[3 * s + 2]);
       if (q1) atomicAdd(&A.d.qx[id], (unsigned long long)q1);
       if (q2) atomicAdd(&A.d.qy[id], (unsigned long long)q2);
       if (q3) atomicAdd(&A.d.qz[id], (unsigned long long)q3);
     }
   }
 
+  if constexpr (!FP64) {
+    if (__uint_as_float(d2min32) < A.d2zerof) flags |= kFlagZero;
+    if (__uint_as_float(d2min32) < A.d2fixf) flags |= kFlagFixedRange;
+  }
   flags = wave_or(flags);
   if ((tid & 63) == 0 && flags)
     atomicOr(&log->flags, (unsigned long long)flags);
@@ -1166,10 +1181,13 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
   A.eps4 = 4.0 * p.eps;
   A.sig2 = p.sig * p.sig;
   A.d2zero = 0.001;
-  // |f| = 24 eps (2 s^12 - s^6)/r < 2^19 for r^2 above d2fix: with
-  // s = sig/r, |f| <= 48 eps sig^12 / r^13, so r^13 >= 48 eps sig^12 / 2^18
-  // keeps a factor-two margin.
-  double r13 = 48.0 * p.eps * pow(p.sig, 12.0) / 262144.0;
+  // The mixed path sums an atom's pair forces of one cell block in 51-bit
+  // fixed point at 2^-32, exact while the sum stays below 2^18.  With no
+  // pair closer than r_fix, |f| = 24 eps (2 s^12 - s^6)/r <= 48 eps
+  // sig^12 / r^13 < 2^14 per pair, and an atom's pairs at r_fix and beyond
+  // (first neighbour shell at most 12, further shells falling as r^-13)
+  // stay below 16 such forces.
+  double r13 = 48.0 * p.eps * pow(p.sig, 12.0) / 16384.0;
   double rmin = pow(r13, 1.0 / 13.0);
   A.d2fix = rmin * rmin;
   A.rc2f = (float)A.rc2;

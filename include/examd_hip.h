@@ -35,8 +35,11 @@ enum {
     EXAMD_ERR_NO_DEVICE = 6,    /* no HIP device */
     EXAMD_ERR_NONFINITE = 7,    /* NaN/Inf force or position */
     EXAMD_ERR_NOT_READY = 8,    /* state not set */
-    EXAMD_ERR_FIXED_RANGE = 9,  /* a pair force exceeded the fixed-point
-                                   conversion range (|f| >= 2^19) */
+    EXAMD_ERR_FIXED_RANGE = 9,  /* mixed path: a pair closer than r_fix,
+                                   48 eps sig^12 / r_fix^13 = 2^14 (0.638
+                                   sig at eps = 1), below which an atom's
+                                   fixed-point force sum could leave its
+                                   51-bit range */
     EXAMD_ERR_ZERO_DISTANCE = 10,/* a pair closer than sqrt(0.001): the
                                    reference prints 'ZERO at' */
     EXAMD_ERR_OUT_OF_BOX = 109  /* block number out of range (ref. 109) */
