@@ -549,7 +549,7 @@ void evaluate(examd_handle *h, int prec, bool energy, bool homecount,
               StepLog *log) {
   launch_step_bin(h->d, table(h, prec), prec == EXAMD_PREC_MIXED, kBinOnly,
                   0.0, 0.0, nullptr, log, h->stream);
-  launch_sort(h->d, h->nblocks, h->stream);
+  launch_sort(h->d, h->nblocks, prec == EXAMD_PREC_FP64, h->stream);
   forces_timed(h, prec, energy, homecount, log);
 }
 
@@ -591,7 +591,7 @@ void enqueue_steps(examd_handle *h, int prec, int64_t count, double dt,
       launch_step_bin(h->d, t, mixed, kKickDriftBin, hk, dt, nullptr, Lc,
                       h->stream);
     }
-    launch_sort(h->d, h->nblocks, h->stream);
+    launch_sort(h->d, h->nblocks, !mixed, h->stream);
     forces_timed(h, prec, is_energy(c), false, Lc);
   }
   if (count > 0) {

@@ -3,8 +3,9 @@
 //   k_step_bin   velocity-Verlet kick/drift/wrap fused with cell assignment
 //                (exaMD_identify_block, SRC/exaMD_module.f90:101-125)
 //   k_scan_*     exclusive scan of per-block counts
-//   k_scatter    counting-sort scatter, then k_block_order restores input
-//                order inside each block (SRC/exaMD_input_formats.f90:80-88)
+//   k_scatter    counting-sort scatter, then (FP64 path) k_block_order
+//                restores input order inside each block
+//                (SRC/exaMD_input_formats.f90:80-88)
 //   k_forces     one workgroup per cell block: gathers the home blocks plus
 //                the rc/2 ghost shell into LDS (the halo exchange of
 //                SRC/exaMD_comms.f90:44-376 done on device), evaluates every
@@ -310,8 +311,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_chunks(int *chunk,
   }
 }
 
+// clear: also zero the counts for the next binning pass (else
+// k_block_order does).
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(
-    const int *count, int64_t n, const int *chunk, int *start) {
+    int *count, int64_t n, const int *chunk, int *start, int clear) {
   __shared__ int s_wave[kScanThreads / 64 + 1];
   int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
   int c[kScanPer];
@@ -320,6 +323,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(
   for (int k = 0; k < kScanPer; ++k) {
     c[k] = base + k < n ? count[base + k] : 0;
     s += c[k];
+  }
+  if (clear) {
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k)
+      if (base + k < n) count[base + k] = 0;
   }
   int total;
   int inc = block_inclusive_scan<kScanThreads>(s, s_wave, &total);
@@ -339,12 +347,13 @@ __global__ void k_scatter(DevState d) {
 
 // Input order inside each block (the order the reference's linked list
 // gives, SRC/exaMD_input_formats.f90:80-88): insertion sort of the few
-// atom ids of one block.  Also clears the block's count for the next
-// binning pass (the force kernel reads counts from start[]).
+// atom ids of one block.  Only the FP64 path needs it (its sums depend on
+// the order of the pairs); the mixed path's fixed-point sums do not.  Also
+// clears the block's count for the next binning pass.
 __global__ void k_block_order(DevState d, int64_t nblocks) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
-  int c = d.count[b];
+  const int c = d.count[b];
   d.count[b] = 0;
   if (c < 2) return;
   int *p = d.sorted + d.start[b];
@@ -402,7 +411,7 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   L.off_id = o;
   o += (size_t)cap * 4;
   L.off_bc = o;
-  if (fp64) o += (size_t)cap * 4;
+  o += (size_t)cap * 4;
   o = align_up(o, 4);
   L.off_perm = o;
   o += (size_t)cap * 2;
@@ -652,6 +661,13 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   const double ox = 0.5 * (double)(h0x + h1x) * g.bl;
   const double oy = 0.5 * (double)(h0y + h1y) * g.bl;
   const double oz = 0.5 * (double)(h0z + h1z) * g.bl;
+  // mixed candidate prefilter: cutoff widened by a margin far above its
+  // FP32 rounding error (a few ulps of the largest |p|^2 in the region,
+  // whose half-extent is t/2 + 1 blocks per axis)
+  const double hxr = (0.5 * tx + 1.0) * g.bl, hyr = (0.5 * ty + 1.0) * g.bl,
+               hzr = (0.5 * tz + 1.0) * g.bl;
+  const float rc2pre =
+      A.rc2f + (float)((hxr * hxr + hyr * hyr + hzr * hzr) * 0x1p-17);
   for (int rb = tid; rb < NRB; rb += NT)
     for (int s = s_off[rb]; s < s_off[rb + 1]; ++s) s_id[s] = rb;
   __syncthreads();
@@ -666,18 +682,21 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
                    ((sy + 1) << 26) | ((sz + 1) << 28);
     const double X = A.d.x[id], Y = A.d.y[id], Z = A.d.z[id];
     s_id[s] = id;
+    s_bc[s] = bc;
     if constexpr (FP64) {
       s_x[s] = X;
       s_y[s] = Y;
       s_z[s] = Z;
-      s_bc[s] = bc;
       s_f[3 * s] = 0.0;
       s_f[3 * s + 1] = 0.0;
       s_f[3 * s + 2] = 0.0;
     } else {
-      s_p4[s] = make_float4((float)((X + sx * g.L[0]) - ox),
-                            (float)((Y + sy * g.L[1]) - oy),
-                            (float)((Z + sz * g.L[2]) - oz), __int_as_float(bc));
+      // w = |p|^2 for the candidate prefilter
+      const float px = (float)((X + sx * g.L[0]) - ox);
+      const float py = (float)((Y + sy * g.L[1]) - oy);
+      const float pz = (float)((Z + sz * g.L[2]) - oz);
+      s_p4[s] = make_float4(px, py, pz,
+                            __builtin_fmaf(px, px, __builtin_fmaf(py, py, pz * pz)));
       s_q[3 * s] = 0;
       s_q[3 * s + 1] = 0;
       s_q[3 * s + 2] = 0;
@@ -694,7 +713,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   for (int b = tid; b <= kWorkBins; b += NT) s_hist[b] = 0;
   __syncthreads();
   auto work_of = [&](int a) {
-    const int bc = FP64 ? s_bc[a] : __float_as_int(s_p4[a].w);
+    const int bc = s_bc[a];
     const unsigned key = own_bits(bc_x(bc), tx) | (own_bits(bc_y(bc), ty) << 3) |
                          (own_bits(bc_z(bc), tz) << 6);
     return (int)kOwnedWork.v[key];
@@ -761,14 +780,16 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       xa = s_x[a];
       ya = s_y[a];
       za = s_z[a];
-      bc = s_bc[a];
     } else {
       const float4 p = s_p4[a];
       xf = p.x;
       yf = p.y;
       zf = p.z;
-      bc = __float_as_int(p.w);
     }
+    bc = s_bc[a];
+    // prefilter constants of atom a: 2a and rc'^2 - |a|^2
+    const float txf = 2.0f * xf, tyf = 2.0f * yf, tzf = 2.0f * zf;
+    const float kaf = rc2pre - __builtin_fmaf(xf, xf, __builtin_fmaf(yf, yf, zf * zf));
     const int ix = bc_x(bc), iy = bc_y(bc), iz = bc_z(bc);
     // an idle lane (group padding) owns nothing
     const unsigned okx = active ? own_bits(ix, tx) : 0u, oky = own_bits(iy, ty),
@@ -825,7 +846,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     };
     auto home_count = [&](int j) {
       if constexpr (HOMECNT) {
-        const int bj = FP64 ? s_bc[j] : __float_as_int(s_p4[j].w);
+        const int bj = s_bc[j];
         const int di = (bc_x(bj) - ix + 2) + 5 * (bc_y(bj) - iy + 2) +
                        25 * (bc_z(bj) - iz + 2);
         const unsigned o = kO1.v[di];
@@ -852,8 +873,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       } else {
         const float4 p = s_p4[j];
         const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
-        pair32(j, dx, dy, dz,
-               __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        // the exact cutoff test behind the candidate prefilter
+        if (d2 > A.rc2f) return;
+        pair32(j, dx, dy, dz, d2);
       }
       home_count(j);
       ++cut;
@@ -919,16 +942,16 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
           // mixed: eight candidates per step; hits are evaluated (at once
           // or from the queue) from the LDS positions
           for (int j = j0; j < j1; j += kChunk) {
-            // rc^2 - d^2 by three FMAs; its sign bit (set beyond the cutoff)
-            // is shifted into the miss mask, last candidate first, so that
-            // bit k ends up holding candidate k
+            // prefilter: rc'^2 - d^2 = (rc'^2 - |a|^2) - |p|^2 + 2 a.p by
+            // one subtraction and three FMAs; its sign bit (set beyond the
+            // cutoff) is shifted into the miss mask, last candidate first,
+            // so that bit k ends up holding candidate k
             unsigned miss = 0u;
 #pragma unroll
             for (int k = kChunk - 1; k >= 0; --k) {
               const float4 p = s_p4[j + k];
-              const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
               const float t = __builtin_fmaf(
-                  -dx, dx, __builtin_fmaf(-dy, dy, __builtin_fmaf(-dz, dz, A.rc2f)));
+                  tzf, p.z, __builtin_fmaf(tyf, p.y, __builtin_fmaf(txf, p.x, kaf - p.w)));
               miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(t), 31);
             }
             // candidates past the range end are masked off
@@ -1154,14 +1177,16 @@ void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
   if (t.nsys > 0) k_sys_ke<<<t.nsys, kBinThreads, 0, s>>>(d, t, log);
 }
 
-void launch_sort(const DevState &d, int64_t nblocks, hipStream_t s) {
+void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
+                 hipStream_t s) {
   int nchunk = (int)((nblocks + kScanChunk - 1) / kScanChunk);
   k_scan_reduce<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk);
   k_scan_chunks<<<1, kScanThreads, 0, s>>>(d.chunk, nchunk);
   k_scan_down<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk,
-                                              d.start);
+                                              d.start, block_order ? 0 : 1);
   if (d.n > 0) k_scatter<<<nblk(d.n, 256), 256, 0, s>>>(d);
-  k_block_order<<<nblk(nblocks, 256), 256, 0, s>>>(d, nblocks);
+  if (block_order)
+    k_block_order<<<nblk(nblocks, 256), 256, 0, s>>>(d, nblocks);
 }
 
 void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,

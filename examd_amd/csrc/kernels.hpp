@@ -78,7 +78,10 @@ void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
 // Kinetic energy of every system into log[sys].ke (sum of v^2).
 void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
                    hipStream_t s);
-void launch_sort(const DevState &d, int64_t nblocks, hipStream_t s);
+// Counting sort of the atoms by block (scan of the counts, which it clears,
+// and scatter); block_order puts each block's atoms in input order.
+void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
+                 hipStream_t s);
 // Tallies of system k go to log[k].
 void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                    const ForceConfig &c, bool mixed, bool energy,
