@@ -128,12 +128,16 @@ __device__ __forceinline__ long long fix32_scaled(float fs) {
   return __double_as_longlong(t) - 0x4338000000000000LL;
 }
 
-// fix32_scaled with the magic constant's bits left in (bits 51 and up of the
-// constant; its bits below 51 are zero).  The low 51 bits of a sum of such
-// values are the low 51 bits of the sum of the fixed-point values, whatever
-// the count, so a sum of magnitude below 2^50 is recovered by fix_sext51.
-__device__ __forceinline__ long long fix32_scaled_raw(float fs) {
-  return __double_as_longlong((double)fs + kFixMagic);
+// Biased fixed point of the product f * x of an FP32 value x and a value f
+// already multiplied by 2^32 (an FP32 value widened to FP64): the product
+// of two FP32 values is exact in FP64, so one FMA into the 1.5*2^52 binade
+// rounds it once, at 2^-32, and the magic constant's bits are left in (bits
+// 51 and up of the constant; its bits below 51 are zero).  The low 51 bits
+// of a sum of such values are the low 51 bits of the sum of the fixed-point
+// values, whatever the count, so a sum of magnitude below 2^50 is recovered
+// by fix_sext51.
+__device__ __forceinline__ long long fix32_prod_raw(double f, float x) {
+  return __double_as_longlong(__builtin_fma(f, (double)x, kFixMagic));
 }
 __device__ __forceinline__ long long fix_sext51(long long v) {
   return (long long)((unsigned long long)v << 13) >> 13;

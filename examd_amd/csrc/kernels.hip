@@ -376,7 +376,7 @@ struct ForceArgs {
   int cap, maxrb, groups, lane_queue;
   double rc2, eps24, eps4, sig2, d2zero, d2fix;
   float rc2f, eps24f, eps4f, sig2f, d2zerof, d2fixf;
-  float eps24fs, eps4fs;   // eps24f, eps4f times 2^32
+  float eps24fs, eps48fs, eps4fs;   // eps24f, 2 eps24f, eps4f times 2^32
 };
 
 __host__ __device__ inline size_t align_up(size_t v, size_t a) {
@@ -750,7 +750,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   //    that the wave empties together, one pair per lane per step.  The
   //    force on a accumulates in registers, the force on j with LDS
   //    atomics.
-  unsigned long long cand = 0, cut = 0;
+  unsigned cand = 0, cut = 0;   // this lane's counts in this cell block
   unsigned flags = 0;
   // mixed: smallest in-cutoff d^2 seen by the lane; the near-overlap flags
   // are set from it once, after the pair loop
@@ -826,13 +826,15 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       const float r2i = __builtin_amdgcn_rcpf(d2);
       const float s2 = A.sig2f * r2i;
       const float s6 = s2 * s2 * s2;
-      // force scalar pre-multiplied by 2^32 (the fixed-point scale)
-      const float fs = A.eps24fs * s6 * __builtin_fmaf(2.0f, s6, -1.0f) * r2i;
-      // biased fixed point: the LDS sums are sign-extended from bit 50
-      // when the forces leave the tile
-      const long long q1 = fix32_scaled_raw(-fs * dx);
-      const long long q2 = fix32_scaled_raw(-fs * dy);
-      const long long q3 = fix32_scaled_raw(-fs * dz);
+      // force scalar 24 eps s6 (2 s6 - 1) / r^2, pre-multiplied by 2^32
+      // (the fixed-point scale)
+      const float fs = __builtin_fmaf(s6, A.eps48fs, -A.eps24fs) * (s6 * r2i);
+      // biased fixed point of the exact products -fs * d (the LDS sums are
+      // sign-extended from bit 50 when the forces leave the tile)
+      const double nfs = -(double)fs;
+      const long long q1 = fix32_prod_raw(nfs, dx);
+      const long long q2 = fix32_prod_raw(nfs, dy);
+      const long long q3 = fix32_prod_raw(nfs, dz);
       qax += q1;
       qay += q2;
       qaz += q3;
@@ -937,7 +939,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         // self run, dx = 0: the block's own triangle, j > a
         const int j0 = (r == 0 && lo == 2) ? a + 1 : s_off[row0 + lo];
         const int j1 = s_off[row0 + hi + 1];
-        cand += (unsigned long long)(j1 > j0 ? j1 - j0 : 0);
+        cand += (unsigned)(j1 > j0 ? j1 - j0 : 0);
         if constexpr (!FP64) {
           // mixed: eight candidates per step; hits are evaluated (at once
           // or from the queue) from the LDS positions
@@ -1060,7 +1062,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     atomicOr(&log->flags, (unsigned long long)flags);
   // tallies (pair counts, energy) only on tally launches
   if constexpr (ENERGY) {
-    Partial mine{0.0, 0.0, 0ull, cand, cut};
+    Partial mine{0.0, 0.0, 0ull, (unsigned long long)cand,
+                 (unsigned long long)cut};
     if constexpr (FP64) mine.pe = pe64;
     else mine.pe_fix = (unsigned long long)pefix;
     write_partial<NT>(mine, A.d.part, tile);
@@ -1219,6 +1222,7 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
   A.eps24f = (float)A.eps24;
   A.eps4f = (float)A.eps4;
   A.eps24fs = A.eps24f * 4294967296.0f;
+  A.eps48fs = 2.0f * A.eps24fs;
   A.eps4fs = A.eps4f * 4294967296.0f;
   A.sig2f = (float)A.sig2;
   A.d2zerof = (float)A.d2zero;
