@@ -3,10 +3,12 @@
 Workload (one GPU's share at real shape): 80^3 FCC lattice, 2,048,000 atoms,
 density 0.8442, Maxwell-Boltzmann velocities at T* = 0.72 (seeded, zero net
 momentum), sigma = epsilon = mass = 1, cutoff 2.5 (unshifted), cubic periodic
-box of 107^3 rc/2 blocks split into 16^3 cell blocks, velocity Verlet
-dt = 0.005, 10,000 steps per precision by default (energies every 10 steps),
-replayed from captured 100-step hipGraph chunks.  Each step rebins on
-device and evaluates the midpoint-method forces once.
+box of 107^3 rc/2 blocks, velocity Verlet dt = 0.005, 10,000 steps per
+precision by default (energies every 10 steps), replayed from captured
+100-step hipGraph chunks.  Each step rebins on device and evaluates the
+midpoint-method forces once.  The launch shape (cell-block map, workgroup
+size, run groups, queue depth, persistent grid) is autotuned per precision
+on the liquid, after 300 untimed steps that melt the lattice.
 
 value = mixed-precision atom-steps/s (FP32 pair arithmetic, int64 fixed-point
 accumulation, FP64 integration); the FP64 path is timed on the same system
@@ -100,16 +102,18 @@ def force_error(m):
 
 
 def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
-              shape=None, graph_chunk=100, report_error=False):
-    """Autotune the launch shape for this precision on the initial state
-    (or apply the given shape), then warm up and time `steps` MD steps,
-    replayed from captured hipGraph chunks of `graph_chunk` steps.  With
-    report_error, the mixed-vs-FP64 force error is taken at the final
+              shape=None, graph_chunk=100, report_error=False, tune_after=300):
+    """Autotune the launch shape for this precision after `tune_after`
+    untimed steps (or apply the given shape), then warm up and time `steps`
+    MD steps, replayed from captured hipGraph chunks of `graph_chunk` steps.
+    With report_error, the mixed-vs-FP64 force error is taken at the final
     positions (outside the timed region)."""
     err = None
     with E.MidpointLJ(len(pos), box, RC, graph_chunk=graph_chunk) as m:
         m.set_state(pos, vel)
         if shape is None:
+            if tune_after:
+                m.run(tune_after, DT, prec, energy_every=0)
             tune = m.autotune(prec, trials=3)
         else:
             m.set_config(prec, **shape)
@@ -285,19 +289,23 @@ def cpu_baseline(pos, vel, box, nsteps):
     return len(pos) * nsteps / dt
 
 
-def pmc_probe(E, shape):
-    """Short program for rocprofv3 --pmc runs: three mixed force evaluations
-    of the bench system at the autotuned shape, then three device-to-device
-    copies of 256 MiB (a plain copy kernel of known size, to calibrate the
-    FETCH_SIZE / WRITE_SIZE counters)."""
+def pmc_probe(E, shape, melt=300):
+    """Short program for rocprofv3 --pmc runs: `melt` MD steps of the bench
+    system (kernels without tallies), then three mixed force evaluations
+    with tallies (k_forces<false, true, false, NT>, the dispatches the
+    summaries use) at the given or autotuned shape, then three
+    device-to-device copies of 256 MiB (a plain copy kernel of known size,
+    to calibrate the FETCH_SIZE / WRITE_SIZE counters)."""
     import ctypes
     pos, vel, box = E.generate_fcc(NC, RHO, TEMP, seed=2048)
     with E.MidpointLJ(len(pos), box, RC) as m:
         m.set_state(pos, vel)
+        if shape is not None:
+            m.set_config("mixed", **shape)
+        if melt:
+            m.run(melt, DT, "mixed", energy_every=0)
         if shape is None:
             print("tune", m.autotune("mixed", trials=2), flush=True)
-        else:
-            m.set_config("mixed", **shape)
         for _ in range(3):
             m.forces("mixed")
     hip = ctypes.CDLL("libamdhip64.so")

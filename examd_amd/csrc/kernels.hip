@@ -384,11 +384,12 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
 }
 
 // LDS image of one cell block: region-block offsets, then atoms.  The atom
-// arrays hold cap + kChunkPad entries: the pair loop reads up to eight
-// candidates at a time and may look up to seven entries past the last atom
-// (masked off).
+// arrays hold cap + kChunkPad entries: the pair loop reads up to sixteen
+// candidates at a time and may look up to fifteen entries past the last
+// atom (masked off).
 constexpr int kWorkBins = 64;   // owned J blocks of an atom: 0..63
-constexpr int kChunkPad = 8;    // atom entries past the last one a chunk may read
+constexpr int kMixedChunk = 16; // mixed candidates per step
+constexpr int kChunkPad = 16;   // atom entries past the last one a chunk may read
 struct LdsLayout {
   size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_perm, off_hist,
       off_q, total;
@@ -742,11 +743,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
 
   // 3. pairs.  Work item w = (run group g, atom perm[i]): the lane lists the
   //    J ranges atom a owns in the group's runs (home block I - o1 inside
-  //    the tile) and tests them four candidates at a time.  Each run group
+  //    the tile) and tests them a chunk at a time.  Each run group
   //    takes whole waves, so the group and run indices are wave-uniform and
   //    the run tables are scalar loads.  In-cutoff pairs are either
   //    evaluated at once or, with lane queues, deferred into a per-lane LDS
-  //    queue (one entry per four candidates with a hit: j << 4 | hit bits)
+  //    queue (one entry per chunk with a hit: j << chunk | hit bits)
   //    that the wave empties together, one pair per lane per step.  The
   //    force on a accumulates in registers, the force on j with LDS
   //    atomics.
@@ -757,9 +758,9 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   unsigned d2min32 = 0x7f7fffffu;   // bits of a non-negative float
   double pe64 = 0.0;
   long long pefix = 0;
-  // candidates tested per step: eight (FP32) or four (FP64, whose
-  // differences take twice the registers)
-  constexpr int kChunk = FP64 ? 4 : 8;
+  // candidates tested per step (a chunk): sixteen (mixed) or four (FP64,
+  // whose differences take twice the registers)
+  constexpr int kChunk = FP64 ? 4 : kMixedChunk;
   constexpr unsigned kChunkMask = (1u << kChunk) - 1u;
   const int NG = A.groups;
   const int RWp = (RW + 63) & ~63;   // atoms per group, padded to whole waves
@@ -941,7 +942,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         const int j1 = s_off[row0 + hi + 1];
         cand += (unsigned)(j1 > j0 ? j1 - j0 : 0);
         if constexpr (!FP64) {
-          // mixed: eight candidates per step; hits are evaluated (at once
+          // mixed: sixteen candidates per step; hits are evaluated (at once
           // or from the queue) from the LDS positions
           for (int j = j0; j < j1; j += kChunk) {
             // prefilter: rc'^2 - d^2 = (rc'^2 - |a|^2) - |p|^2 + 2 a.p by

@@ -190,7 +190,9 @@ typedef struct examd_tune {
     int32_t run_groups;
     int32_t lane_queue;   /* entries of the per-lane queues of deferred
                              in-cutoff pairs (an entry holds the hits among
-                             four candidates): 0 (evaluate at once), 4 or 8;
+                             one chunk of candidates, sixteen in the mixed
+                             path, four in FP64): 0 (evaluate at once), 4
+                             or 8;
                              other values round up to 4 or down to 8 */
     int32_t tile_edge;    /* set_config: when tiles are 0, every system's map
                              is T = max(1, round(nblocks_xyz / tile_edge));
