@@ -45,7 +45,10 @@ EXPORTS = (
     "examd_hip_set_config", "examd_hip_set_graph_chunk",
     "examd_hip_batch_create", "examd_hip_batch_info",
     "examd_hip_generate_vacancy_fcc", "examd_hip_generate_slab",
+    "examd_hip_set_species",
 )
+
+MAX_SPECIES = 8
 
 
 class ExamdError(RuntimeError):
@@ -135,6 +138,8 @@ def lib():
         L.examd_hip_generate_slab.argtypes = [
             dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
             C.c_double, C.c_double, C.c_uint64, dp, dp, C.c_int64, ip64]
+        L.examd_hip_set_species.argtypes = [C.c_void_p, C.c_int32, ip32,
+                                            C.c_int64, dp, dp, dp]
         L.examd_hip_strerror.argtypes = [C.c_int]
         L.examd_hip_strerror.restype = C.c_char_p
         for name in EXPORTS:
@@ -275,6 +280,24 @@ class _Handle:
         assert pos.shape == (self.n, 3)
         _check(lib().examd_hip_set_state(self._h, _dp(pos), _dp(vel), self.n),
                "examd_hip_set_state")
+
+    def set_species(self, types=None, eps=None, sigma=None, cutoff=None):
+        """Several species: types (n,) in [0, k), symmetric (k, k) tables of
+        eps, sigma and cutoff (each cutoff at most the handle's).  With no
+        arguments, back to the handle's one species."""
+        if types is None:
+            _check(lib().examd_hip_set_species(self._h, 0, None, self.n, None,
+                                               None, None),
+                   "examd_hip_set_species")
+            return
+        types = np.ascontiguousarray(types, dtype=np.int32).reshape(self.n)
+        tab = [np.ascontiguousarray(x, dtype=np.float64) for x in
+               (eps, sigma, cutoff)]
+        k = tab[0].shape[0]
+        assert all(t.shape == (k, k) for t in tab)
+        _check(lib().examd_hip_set_species(
+            self._h, k, types.ctypes.data_as(C.POINTER(C.c_int32)), self.n,
+            _dp(tab[0]), _dp(tab[1]), _dp(tab[2])), "examd_hip_set_species")
 
     def get_state(self, forces=False):
         pos = np.zeros((self.n, 3))

@@ -146,7 +146,7 @@ void free_state(examd_handle *h) {
   void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
                   d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.rank,
                   d.count, d.start, d.sorted, d.chunk, d.home, d.part, h->log,
-                  h->atom_seg};
+                  h->atom_seg, d.type, d.pair32, d.pair64};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DevState{};
@@ -398,6 +398,7 @@ Tri tiles_from_edge(const Grid &g, double edge) {
 int choose_caps(examd_handle *h, int prec, const double *pos) {
   TileConfig &c = h->tc[prec];
   const bool fp64 = prec == EXAMD_PREC_FP64;
+  const bool typed = h->d.ntypes > 0;
   const int nsys = h->nsys;
   for (int iter = 0; iter < 4096; ++iter) {
     bool ok = true;
@@ -436,9 +437,11 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
     int64_t cap = want;
     if (ok) {
       while (cap > need &&
-             force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit)
+             force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue,
+                             typed) > h->lds_limit)
         cap -= 16;
-      if (force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue) > h->lds_limit ||
+      if (force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue,
+                          typed) > h->lds_limit ||
           cap > 65535)
         ok = false;
     }
@@ -974,6 +977,80 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
   }
   h->state_set = true;
   h->forces_valid = -1;
+  return EXAMD_OK;
+}
+
+int examd_hip_set_species(examd_handle *h, int32_t ntypes,
+                          const int32_t *type, int64_t n, const double *eps,
+                          const double *sigma, const double *cutoff) {
+  if (!h || ntypes < 0 || ntypes > EXAMD_MAX_SPECIES) return EXAMD_ERR_ARG;
+  DevState &d = h->d;
+  std::vector<float4> p32;
+  std::vector<Pair64> p64;
+  std::vector<unsigned char> ty;
+  double max_es12 = 0.0;
+  if (ntypes > 0) {
+    if (!type || n != h->n || !eps || !sigma || !cutoff) return EXAMD_ERR_ARG;
+    const int nt = ntypes;
+    for (int a = 0; a < nt; ++a)
+      for (int b = 0; b < nt; ++b) {
+        const int k = a * nt + b, kt = b * nt + a;
+        if (!(std::isfinite(eps[k]) && eps[k] >= 0.0) ||
+            !(std::isfinite(sigma[k]) && sigma[k] > 0.0) ||
+            !(cutoff[k] > 0.0 && cutoff[k] <= h->p.cutoff) ||
+            eps[k] != eps[kt] || sigma[k] != sigma[kt] ||
+            cutoff[k] != cutoff[kt])
+          return EXAMD_ERR_ARG;
+      }
+    ty.resize((size_t)std::max<int64_t>(n, 1));
+    for (int64_t i = 0; i < n; ++i) {
+      if (type[i] < 0 || type[i] >= nt) return EXAMD_ERR_ARG;
+      ty[i] = (unsigned char)type[i];
+    }
+    p32.resize(kMaxSpecies * kMaxSpecies);
+    p64.resize(kMaxSpecies * kMaxSpecies);
+    for (int k = 0; k < nt * nt; ++k) {
+      // the same roundings as the one-species constants of launch_forces
+      const double sig2 = sigma[k] * sigma[k], rc2 = cutoff[k] * cutoff[k];
+      const double e24 = 24.0 * eps[k], e4 = 4.0 * eps[k];
+      p64[k] = Pair64{sig2, e24, e4, rc2};
+      p32[k] = make_float4((float)sig2, (float)e24 * 4294967296.0f,
+                           (float)e4 * 4294967296.0f, (float)rc2);
+      max_es12 = std::max(max_es12, eps[k] * std::pow(sigma[k], 12.0));
+    }
+  }
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const DevState saved = d;
+  if (ntypes > 0) {
+    if (!d.type && dalloc(&d.type, h->n)) return EXAMD_ERR_NOMEM;
+    if (!d.pair32 && dalloc(&d.pair32, kMaxSpecies * kMaxSpecies))
+      return EXAMD_ERR_NOMEM;
+    if (!d.pair64 && dalloc(&d.pair64, kMaxSpecies * kMaxSpecies))
+      return EXAMD_ERR_NOMEM;
+    HIPCHK(hipMemcpy(d.type, ty.data(), (size_t)n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.pair32, p32.data(), p32.size() * sizeof(float4),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.pair64, p64.data(), p64.size() * sizeof(Pair64),
+                     hipMemcpyHostToDevice));
+  }
+  d.ntypes = ntypes;
+  d.max_eps_sig12 = max_es12;
+  drop_graph(h);
+  h->forces_valid = -1;
+  if (h->state_set) {
+    // the species bytes and the pair table take LDS: re-check the maps
+    std::vector<double> pos;
+    int st = download_positions(h, pos);
+    for (int prec = 0; st == EXAMD_OK && prec < 2; ++prec)
+      st = choose_caps(h, prec, pos.data());
+    if (st != EXAMD_OK) {
+      d.ntypes = saved.ntypes;
+      d.max_eps_sig12 = saved.max_eps_sig12;
+      for (int prec = 0; prec < 2; ++prec) (void)choose_caps(h, prec, pos.data());
+      return st;
+    }
+  }
   return EXAMD_OK;
 }
 

@@ -143,6 +143,14 @@ __device__ __forceinline__ long long fix_sext51(long long v) {
   return (long long)((unsigned long long)v << 13) >> 13;
 }
 
+// ------------------------------------------------------------- species ----
+// Several species: at most kMaxSpecies, a pair (i, j) uses row
+// type[i] * ntypes + type[j] of a symmetric table.
+constexpr int kMaxSpecies = 8;
+struct Pair64 {
+  double sig2, eps24, eps4, rc2;
+};
+
 // ---------------------------------------------------------------- grid ----
 struct Grid {
   int nb[3];        // nblocks_xyz

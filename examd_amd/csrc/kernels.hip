@@ -392,10 +392,11 @@ constexpr int kMixedChunk = 16; // mixed candidates per step
 constexpr int kChunkPad = 16;   // atom entries past the last one a chunk may read
 struct LdsLayout {
   size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_perm, off_hist,
-      off_q, total;
+      off_q, off_pair, off_ty, total;
 };
 __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
-                                                int nwaves, int qdepth) {
+                                                int nwaves, int qdepth,
+                                                bool typed) {
   cap += kChunkPad;
   LdsLayout L;
   size_t o = (size_t)(maxrb + 1) * 4;  // s_off
@@ -422,6 +423,12 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   o = align_up(o, 16);
   L.off_q = o;
   o += (size_t)nwaves * 64 * qdepth * 4;
+  // several species: the species-pair table, then one species byte per atom
+  o = align_up(o, 16);
+  L.off_pair = o;
+  if (typed) o += (size_t)kMaxSpecies * kMaxSpecies * sizeof(Pair64);
+  L.off_ty = o;
+  if (typed) o += (size_t)cap;
   L.total = align_up(o, 16);
   return L;
 }
@@ -550,7 +557,7 @@ __device__ __forceinline__ int bc_s(int bc, int axis) {
 }
 
 // One cell block (tile) of one system, evaluated by the whole workgroup.
-template <bool FP64, bool ENERGY, bool HOMECNT, int NT>
+template <bool FP64, bool ENERGY, bool HOMECNT, bool TYPED, int NT>
 __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
                                            unsigned char *smem) {
   constexpr int NW = NT / 64;
@@ -584,7 +591,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   const int RXY = RX * RY, NRB = RXY * RZ;
   const int nbx = g.nb[0], nby = g.nb[1], nbz = g.nb[2];
 
-  const LdsLayout LL = lds_layout(FP64, A.cap, A.maxrb, NW, A.lane_queue);
+  const LdsLayout LL = lds_layout(FP64, A.cap, A.maxrb, NW, A.lane_queue, TYPED);
+  // several species: species of each region atom, species-pair rows
+  unsigned char *s_ty = smem + LL.off_ty;
+  const float4 *s_pr32 = reinterpret_cast<const float4 *>(smem + LL.off_pair);
+  const Pair64 *s_pr64 = reinterpret_cast<const Pair64 *>(smem + LL.off_pair);
   int *s_off = reinterpret_cast<int *>(smem);
   int *s_gst = reinterpret_cast<int *>(smem + LL.off_gst);
   int *s_wave = reinterpret_cast<int *>(smem + LL.off_wave);
@@ -684,6 +695,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     const double X = A.d.x[id], Y = A.d.y[id], Z = A.d.z[id];
     s_id[s] = id;
     s_bc[s] = bc;
+    if constexpr (TYPED) s_ty[s] = A.d.type[id];
     if constexpr (FP64) {
       s_x[s] = X;
       s_y[s] = Y;
@@ -701,6 +713,15 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       s_q[3 * s] = 0;
       s_q[3 * s + 1] = 0;
       s_q[3 * s + 2] = 0;
+    }
+  }
+  if constexpr (TYPED) {
+    // the species-pair table (the mixed rows and the FP64 rows share the
+    // slot; the FP64 rows are the larger)
+    const int np = A.d.ntypes * A.d.ntypes;
+    for (int k = tid; k < np; k += NT) {
+      if constexpr (FP64) const_cast<Pair64 *>(s_pr64)[k] = A.d.pair64[k];
+      else const_cast<float4 *>(s_pr32)[k] = A.d.pair32[k];
     }
   }
   __syncthreads();
@@ -801,16 +822,19 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     double fax = 0.0, fay = 0.0, faz = 0.0;
     long long qax = 0, qay = 0, qaz = 0;
     int qn = 0;   // queued pairs of this lane
+    // several species: first row of atom a's species in the pair table
+    const int trow = TYPED ? (int)s_ty[a] * A.d.ntypes : 0;
 
-    // in-cutoff pair (a, j), table orientation: +f on a, -f on j
-    // (SRC/exaMD_forces.f90:538-542)
-    auto pair64 = [&](int j, double dx, double dy, double dz, double d2) {
+    // in-cutoff pair (a, j) with the pair's parameters P, table
+    // orientation: +f on a, -f on j (SRC/exaMD_forces.f90:538-542)
+    auto pair64 = [&](int j, double dx, double dy, double dz, double d2,
+                      const Pair64 &P) {
       // user_func.f90:58-69
       const double d2i = 1.0 / d2;
-      const double sg = A.sig2 * d2i;
+      const double sg = P.sig2 * d2i;
       const double d6 = sg * sg * sg;
       const double d12 = d6 * d6;
-      const double dg = -(A.eps24 * (2.0 * d12 - d6) * d2i);
+      const double dg = -(P.eps24 * (2.0 * d12 - d6) * d2i);
       const double f1 = dg * dx, f2 = dg * dy, f3 = dg * dz;
       fax += f1;
       fay += f2;
@@ -820,16 +844,19 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       lds_add(fj, -f1);
       lds_add(fj + 1, -f2);
       lds_add(fj + 2, -f3);
-      if constexpr (ENERGY) pe64 += A.eps4 * (d12 - d6);
+      if constexpr (ENERGY) pe64 += P.eps4 * (d12 - d6);
       if (d2 < A.d2zero) flags |= kFlagZero;
     };
-    auto pair32 = [&](int j, float dx, float dy, float dz, float d2) {
+    // P = (sig^2, 24 eps 2^32, 4 eps 2^32, rc^2) of the pair
+    auto pair32 = [&](int j, float dx, float dy, float dz, float d2,
+                      const float4 &P) {
       const float r2i = __builtin_amdgcn_rcpf(d2);
-      const float s2 = A.sig2f * r2i;
+      const float s2 = P.x * r2i;
       const float s6 = s2 * s2 * s2;
       // force scalar 24 eps s6 (2 s6 - 1) / r^2, pre-multiplied by 2^32
       // (the fixed-point scale)
-      const float fs = __builtin_fmaf(s6, A.eps48fs, -A.eps24fs) * (s6 * r2i);
+      const float fs = __builtin_fmaf(s6, TYPED ? 2.0f * P.y : A.eps48fs, -P.y) *
+                       (s6 * r2i);
       // biased fixed point of the exact products -fs * d (the LDS sums are
       // sign-extended from bit 50 when the forces leave the tile)
       const double nfs = -(double)fs;
@@ -844,7 +871,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       lds_sub(qj, q1);
       lds_sub(qj + 1, q2);
       lds_sub(qj + 2, q3);
-      if constexpr (ENERGY) pefix += fix32_scaled(A.eps4fs * s6 * (s6 - 1.0f));
+      if constexpr (ENERGY) pefix += fix32_scaled(P.z * s6 * (s6 - 1.0f));
       d2min32 = min(d2min32, __float_as_uint(d2));
     };
     auto home_count = [&](int j) {
@@ -859,6 +886,24 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         atomicAdd(&A.d.home[block0 + hx + nbx * (hy + nby * hz)], 1ull);
       }
     };
+    // the pair's parameters: its species row, or the one species
+    auto params64 = [&](int j) {
+      if constexpr (TYPED) return s_pr64[trow + s_ty[j]];
+      else return Pair64{A.sig2, A.eps24, A.eps4, A.rc2};
+    };
+    auto params32 = [&](int j) {
+      if constexpr (TYPED) return s_pr32[trow + s_ty[j]];
+      else return make_float4(A.sig2f, A.eps24fs, A.eps4fs, A.rc2f);
+    };
+    // an FP64 pair that passed the candidate test: with several species
+    // it must also lie within its own species pair's cutoff
+    auto hit64 = [&](int j, double dx, double dy, double dz, double d2) {
+      const Pair64 P = params64(j);
+      if (TYPED && d2 > P.rc2) return;
+      pair64(j, dx, dy, dz, d2, P);
+      home_count(j);
+      ++cut;
+    };
     // evaluate a queued pair (a, j) from positions
     auto eval_queued = [&](int j) {
       if constexpr (FP64) {
@@ -872,17 +917,19 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         const double dx = (s_x[j] - xa) - btx;
         const double dy = (s_y[j] - ya) - bty;
         const double dz = (s_z[j] - za) - btz;
-        pair64(j, dx, dy, dz, dx * dx + dy * dy + dz * dz);
+        hit64(j, dx, dy, dz, dx * dx + dy * dy + dz * dz);
       } else {
         const float4 p = s_p4[j];
         const float dx = p.x - xf, dy = p.y - yf, dz = p.z - zf;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        // the exact cutoff test behind the candidate prefilter
-        if (d2 > A.rc2f) return;
-        pair32(j, dx, dy, dz, d2);
+        // the exact cutoff test behind the candidate prefilter (the pair's
+        // own species cutoff with several species)
+        const float4 P = params32(j);
+        if (d2 > P.w) return;
+        pair32(j, dx, dy, dz, d2, P);
+        home_count(j);
+        ++cut;
       }
-      home_count(j);
-      ++cut;
     };
     // empty the queues of the wave's lanes together, one pair per lane per
     // step: a lane takes its next entry when the hit bits of the current
@@ -1009,9 +1056,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
                 const double dy = k == 0 ? ddy[0] : k == 1 ? ddy[1] : k == 2 ? ddy[2] : ddy[3];
                 const double dz = k == 0 ? ddz[0] : k == 1 ? ddz[1] : k == 2 ? ddz[2] : ddz[3];
                 const double d2 = k == 0 ? dd2[0] : k == 1 ? dd2[1] : k == 2 ? dd2[2] : dd2[3];
-                pair64(j + k, dx, dy, dz, d2);
-                home_count(j + k);
-                ++cut;
+                hit64(j + k, dx, dy, dz, d2);
               }
             }
           }
@@ -1077,13 +1122,13 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
 // Tiles are taken in the XCD-aware order, so the tiles of one XCD's
 // workgroups are contiguous in space (the grid is a multiple of 8 when it
 // loops).
-template <bool FP64, bool ENERGY, bool HOMECNT, int NT>
+template <bool FP64, bool ENERGY, bool HOMECNT, bool TYPED, int NT>
 __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int ntiles = A.t.ntiles;
   for (int wg = blockIdx.x; wg < ntiles; wg += gridDim.x) {
     if (wg != (int)blockIdx.x) __syncthreads();  // LDS of the last tile
-    force_tile<FP64, ENERGY, HOMECNT, NT>(A, xcd_remap(wg, ntiles), smem);
+    force_tile<FP64, ENERGY, HOMECNT, TYPED, NT>(A, xcd_remap(wg, ntiles), smem);
   }
 }
 
@@ -1103,18 +1148,18 @@ __global__ void k_forces_to_fp64(DevState d, double *out, bool mixed) {
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-template <bool FP64, bool ENERGY, bool HOMECNT>
+template <bool FP64, bool ENERGY, bool HOMECNT, bool TYPED>
 void launch_forces_t(const ForceArgs &A, int threads, size_t lds, int G,
                      hipStream_t s) {
   switch (threads) {
     case 512:
-      k_forces<FP64, ENERGY, HOMECNT, 512><<<G, 512, lds, s>>>(A);
+      k_forces<FP64, ENERGY, HOMECNT, TYPED, 512><<<G, 512, lds, s>>>(A);
       break;
     case 1024:
-      k_forces<FP64, ENERGY, HOMECNT, 1024><<<G, 1024, lds, s>>>(A);
+      k_forces<FP64, ENERGY, HOMECNT, TYPED, 1024><<<G, 1024, lds, s>>>(A);
       break;
     default:
-      k_forces<FP64, ENERGY, HOMECNT, 256><<<G, 256, lds, s>>>(A);
+      k_forces<FP64, ENERGY, HOMECNT, TYPED, 256><<<G, 256, lds, s>>>(A);
       break;
   }
 }
@@ -1122,8 +1167,8 @@ void launch_forces_t(const ForceArgs &A, int threads, size_t lds, int G,
 }  // namespace
 
 size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads,
-                       int qdepth) {
-  return lds_layout(fp64, cap, maxrb, threads / 64, qdepth).total;
+                       int qdepth, bool typed) {
+  return lds_layout(fp64, cap, maxrb, threads / 64, qdepth, typed).total;
 }
 
 void launch_fill_i32(int *p, int v, int64_t n, hipStream_t s) {
@@ -1216,7 +1261,9 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
   // sig^12 / r^13 < 2^14 per pair, and an atom's pairs at r_fix and beyond
   // (first neighbour shell at most 12, further shells falling as r^-13)
   // stay below 16 such forces.
-  double r13 = 48.0 * p.eps * pow(p.sig, 12.0) / 16384.0;
+  // (several species: the largest 48 eps sig^12 of the pair table)
+  double r13 = 48.0 * (d.ntypes > 0 ? d.max_eps_sig12 : p.eps * pow(p.sig, 12.0)) /
+               16384.0;
   double rmin = pow(r13, 1.0 / 13.0);
   A.d2fix = rmin * rmin;
   A.rc2f = (float)A.rc2;
@@ -1229,12 +1276,16 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
   A.d2zerof = (float)A.d2zero;
   A.d2fixf = (float)A.d2fix;
   const bool fp64 = !mixed;
-  const size_t lds = force_lds_bytes(fp64, c.cap, c.maxrb, c.threads, c.lane_queue);
+  const bool typed = d.ntypes > 0;
+  const size_t lds =
+      force_lds_bytes(fp64, c.cap, c.maxrb, c.threads, c.lane_queue, typed);
   // persistent grids are multiples of 8 (one tile order per XCD)
   int G = t.ntiles;
   if (c.workgroups > 0 && c.workgroups < G) G = (c.workgroups + 7) / 8 * 8;
   if (G > t.ntiles) G = t.ntiles;
-#define EXAMD_F(F64, E, H) launch_forces_t<F64, E, H>(A, c.threads, lds, G, s)
+#define EXAMD_F(F64, E, H)                                                  \
+  (typed ? launch_forces_t<F64, E, H, true>(A, c.threads, lds, G, s)        \
+         : launch_forces_t<F64, E, H, false>(A, c.threads, lds, G, s))
   if (homecount) {
     if (fp64) EXAMD_F(true, false, true);
     else EXAMD_F(false, false, true);

@@ -23,6 +23,13 @@ struct DevState {
   unsigned long long *home = nullptr;  // home-block pair counts (diagnostic)
   Partial *part = nullptr; // per-workgroup partial tallies
   unsigned long long *prof = nullptr;  // phase timestamps (diagnostic)
+  // several species (ntypes > 0): species of each atom and the species-pair
+  // table, row t_i * ntypes + t_j, in both precisions
+  unsigned char *type = nullptr;
+  float4 *pair32 = nullptr;   // sig^2, 24 eps 2^32, 4 eps 2^32, rc^2
+  Pair64 *pair64 = nullptr;
+  int ntypes = 0;
+  double max_eps_sig12 = 0.0; // largest eps sig^12 of the table
 };
 
 constexpr int kProfSlots = 5;   // timestamps per cell block
@@ -59,7 +66,7 @@ struct ForceConfig {
 };
 
 size_t force_lds_bytes(bool fp64, int cap, int maxrb, int threads,
-                       int qdepth);
+                       int qdepth, bool typed);
 
 // Binning / integration modes of launch_step_bin.
 enum BinMode { kBinOnly = 0, kKickDriftBin = 1, kFinishKickDriftBin = 2 };

@@ -91,6 +91,16 @@ Module exaMD_hip_binding
        Integer(c_int64_t), Value :: n
      End Function examd_hip_set_state
 
+     Integer(c_int) Function examd_hip_set_species(handle, ntypes, type, n, &
+          eps, sigma, cutoff) Bind(C, name="examd_hip_set_species")
+       Import :: c_int, c_ptr, c_int32_t, c_int64_t
+       Type(c_ptr), Value :: handle
+       Integer(c_int32_t), Value :: ntypes
+       Type(c_ptr), Value :: type
+       Integer(c_int64_t), Value :: n
+       Type(c_ptr), Value :: eps, sigma, cutoff
+     End Function examd_hip_set_species
+
      Integer(c_int) Function examd_hip_get_state(handle, pos, vel, force, n) &
           Bind(C, name="examd_hip_get_state")
        Import :: c_int, c_ptr, c_int64_t

@@ -387,7 +387,7 @@ Module exaMD_forces_mod
   Use, Intrinsic :: iso_c_binding
   Use exaMD_mod
   Use exaMD_hip_binding
-  Use user_func_mod, Only: prmvdw
+  Use user_func_mod, Only: prmvdw, kktype, ltpvdw
   Implicit None
 
   !! Arithmetic of the force routines: EXAMD_PREC_FP64 (default, the
@@ -398,6 +398,7 @@ Module exaMD_forces_mod
 
   Type(c_ptr), Private :: handle = c_null_ptr
   Integer(c_int64_t), Private :: handle_n = -1
+  Logical, Private :: handle_species = .False.
 
 Contains
 
@@ -406,6 +407,7 @@ Contains
     If (c_associated(handle)) st = examd_hip_destroy(handle)
     handle = c_null_ptr
     handle_n = -1
+    handle_species = .False.
   End Subroutine exaMD_release_device
 
   !! The device path evaluates the Lennard-Jones pair; check that user_func
@@ -484,6 +486,61 @@ Contains
     handle_n = n
   End Subroutine ensure_handle
 
+  !! Several species: grid%ltype (species of each atom, 1-based), kktype
+  !! (species pair -> row of prmvdw), ltpvdw (kind of each row: 2, the
+  !! Lennard-Jones pair, is the only one the device evaluates) and prmvdw
+  !! (row k: eps, sigma, -, -, cutoff), as exaMD.f90:199-210 sets them up for
+  !! one species.  With more than one species the device evaluates each pair
+  !! with its own row; with one, the handle's parameters (prmvdw(1,:)).
+  Subroutine set_device_species(n)
+    Integer(c_int64_t), Intent(in) :: n
+    Integer(c_int32_t), Allocatable, Target :: ty(:)
+    Real(c_double), Allocatable, Target :: e(:,:), s(:,:), c(:,:)
+    Integer :: nt, a, b, kr, m, bl, i, j, k, na
+    Integer(c_int) :: st
+    nt = 0
+    If (Allocated(kktype)) nt = Size(kktype, 1)
+    If (nt <= 1) Then
+       If (handle_species) Then
+          st = examd_hip_set_species(handle, 0_c_int32_t, c_null_ptr, n, &
+               c_null_ptr, c_null_ptr, c_null_ptr)
+          Call check_status(st, 'examd_hip_set_species')
+          handle_species = .False.
+       End If
+       Return
+    End If
+    Allocate(e(nt, nt), s(nt, nt), c(nt, nt), ty(Max(n, 1_c_int64_t)))
+    Do a = 1, nt
+       Do b = 1, nt
+          kr = kktype(a, b)
+          If (ltpvdw(kr) /= 2) Then
+             Write (*, *) 'exaMD: the GPU path evaluates only the ', &
+                  'Lennard-Jones pair (ltpvdw = 2)'
+             Call exaMD_abort(101)
+          End If
+          ! row-major [a][b] on the C side is e(b, a) here
+          e(b, a) = Real(prmvdw(kr, 1), c_double)
+          s(b, a) = Real(prmvdw(kr, 2), c_double)
+          c(b, a) = Real(prmvdw(kr, 5), c_double)
+       End Do
+    End Do
+    m = 0
+    Do bl = 1, nblocks_loc
+       i = locblocks(1, bl)
+       j = locblocks(2, bl)
+       k = locblocks(3, bl)
+       na = grid(i, j, k)%natms
+       If (na > 0) Then
+          ty(m + 1:m + na) = Int(grid(i, j, k)%ltype(1:na) - 1, c_int32_t)
+          m = m + na
+       End If
+    End Do
+    st = examd_hip_set_species(handle, Int(nt, c_int32_t), c_loc(ty), n, &
+         c_loc(e), c_loc(s), c_loc(c))
+    Call check_status(st, 'examd_hip_set_species')
+    handle_species = .True.
+  End Subroutine set_device_species
+
   Subroutine check_status(st, where)
     Integer(c_int), Intent(in) :: st
     Character(len=*), Intent(in) :: where
@@ -508,6 +565,7 @@ Contains
     Call check_user_func(user_func, cutoff)
     Call gather_positions(pos, n)
     Call ensure_handle(n, cutoff)
+    Call set_device_species(n)
     st = examd_hip_set_state(handle, pos, c_null_ptr, n)
     Call check_status(st, 'examd_hip_set_state')
     Allocate(f(3, Max(n, 1_c_int64_t)))
@@ -552,6 +610,7 @@ Contains
     Integer(c_int) :: st
     Call gather_positions(pos, n)
     Call ensure_handle(n, cutoff)
+    Call set_device_species(n)
     st = examd_hip_set_state(handle, pos, c_loc(vel), n)
     Call check_status(st, 'examd_hip_set_state')
     st = examd_hip_timing(handle, 1_c_int, force_ms, launches, 1_c_int)

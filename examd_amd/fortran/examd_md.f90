@@ -15,6 +15,10 @@
 !!   -seed s         generator seed (default 1)
 !!   -restart file   start from a checkpoint (positions, velocities, box, step)
 !!   -checkpoint file write the state after the MD steps as a checkpoint
+!!   -binary         two species (Kob-Andersen): every fifth atom in input
+!!                   order is species 2; pair rows AA (1, 1, rc 2.5),
+!!                   AB (1.5, 0.8, 2.0), BB (0.5, 0.88, 2.2) as (eps, sigma,
+!!                   cutoff), set through grid%ltype, kktype, ltpvdw, prmvdw
 
 Program examd_md
 
@@ -41,7 +45,8 @@ Program examd_md
   Type(examd_stats), Allocatable :: stats(:)
   Character(len=256) :: arg, restart_file, checkpoint_file
   Integer(8) :: step0
-  Logical :: inputcase, overlapcase, writeout
+  Logical :: inputcase, overlapcase, writeout, binary
+  Integer :: m, p
   Integer(8) :: c0, c1, crate
   Real(c_double) :: force_ms, e0, drift
 
@@ -57,6 +62,7 @@ Program examd_md
   inputcase = .False.
   overlapcase = .False.
   writeout = .False.
+  binary = .False.
   restart_file = ''
   checkpoint_file = ''
   step0 = 0
@@ -72,6 +78,8 @@ Program examd_md
         inputcase = .True.
      Case ('-writefile')
         writeout = .True.
+     Case ('-binary')
+        binary = .True.
      Case ('-atoms')
         ia = ia + 1
         Call get_command_argument(ia, arg)
@@ -166,10 +174,33 @@ Program examd_md
   Call exaMD_partition_work(particle_positions, ntot, first_particle, &
        next_particle, nblocks, nblocks_xyz, pid, noprocs)
 
-  Allocate(kktype(1, 1), ltpvdw(1), prmvdw(1, 5))
-  kktype = 1
-  ltpvdw = 2
-  prmvdw(1, 1:5) = (/ one, one, zero, zero, cutoff /)
+  If (binary) Then
+     ! species of every atom through its block chain (input order), then
+     ! the species-pair rows
+     Do b = 1, nblocks
+        i = Mod(b - 1, nblocks_xyz(1)) + 1
+        j = Mod((b - 1) / nblocks_xyz(1), nblocks_xyz(2)) + 1
+        k = (b - 1) / (nblocks_xyz(1) * nblocks_xyz(2)) + 1
+        p = first_particle(b)
+        Do m = 1, ntot(b)
+           grid(i, j, k)%ltype(m) = 1
+           If (Mod(p, 5) == 0) grid(i, j, k)%ltype(m) = 2
+           p = next_particle(p)
+        End Do
+     End Do
+     Allocate(kktype(2, 2), ltpvdw(3), prmvdw(3, 5))
+     kktype = Reshape((/ 1, 2, 2, 3 /), (/ 2, 2 /))
+     ltpvdw = 2
+     prmvdw(1, 1:5) = (/ one, one, zero, zero, cutoff /)
+     prmvdw(2, 1:5) = (/ 1.5_PRC, 0.8_PRC, zero, zero, Min(cutoff, 2.0_PRC) /)
+     prmvdw(3, 1:5) = (/ half, 0.88_PRC, zero, zero, Min(cutoff, 2.2_PRC) /)
+     Write (*, *) 'Species: 2 (every fifth atom is B; AA, AB, BB rows)'
+  Else
+     Allocate(kktype(1, 1), ltpvdw(1), prmvdw(1, 5))
+     kktype = 1
+     ltpvdw = 2
+     prmvdw(1, 1:5) = (/ one, one, zero, zero, cutoff /)
+  End If
   Call exaMD_setup_midpoint_interactions()
 
   ! one force evaluation, timed like the reference (exaMD.f90:241-253)

@@ -138,6 +138,22 @@ int examd_hip_destroy(examd_handle *h);
 int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
                         int64_t n);
 
+/* Several species (no counterpart in the reference's evaluation, which reads
+ * only the first row of its parameter table, prmvdw(1,:), user_func.f90:
+ * 55-61; the table form follows the declarations of user_func.f90:16-18,
+ * kktype / ltpvdw / prmvdw).  Atom i has species type[i] in [0, ntypes); the
+ * pair (i, j) is the Lennard-Jones pair with eps, sigma and cutoff at index
+ * type[i] * ntypes + type[j] of the ntypes x ntypes row-major tables, which
+ * must be symmetric, with 0 < cutoff <= the handle's cutoff (which sizes the
+ * blocks) and sigma > 0.  n is the handle's atom count (all systems of a
+ * batch).  ntypes = 0 returns to one species (the handle's epsilon, sigma
+ * and cutoff).  The species and the table hold for every later evaluation
+ * and run of both precisions.  At most EXAMD_MAX_SPECIES species. */
+#define EXAMD_MAX_SPECIES 8
+int examd_hip_set_species(examd_handle *h, int32_t ntypes,
+                          const int32_t *type, int64_t n, const double *eps,
+                          const double *sigma, const double *cutoff);
+
 /* Download the current state; any pointer may be NULL.  force is the last
  * evaluated force (FP64 or fixed point converted to FP64). */
 int examd_hip_get_state(examd_handle *h, double *pos, double *vel,

@@ -127,13 +127,17 @@ int or_lennardjones_12(const or_lj *p, double dx, double dy, double dz,
     return 0;
 }
 
-/* SRC/exaMD_forces.f90:492-551. */
+/* SRC/exaMD_forces.f90:492-551.  With species, the pair's parameters are
+ * the row of the species pair (type[i1], type[i2]). */
 static inline void or_particle_interaction(const or_domain *d, const or_lj *p,
+                                           const or_species *sp,
                                            const double *pos, double *forces,
                                            int64_t i1, int64_t i2,
                                            int64_t home, int64_t *home_counts,
                                            or_tally *t)
 {
+    if (sp)
+        p = &sp->pair[sp->type[i1] * sp->ntypes + sp->type[i2]];
     const double *x1 = &pos[3 * i1], *x2 = &pos[3 * i2];
     double dx = x2[0] - x1[0];
     double dy = x2[1] - x1[1];
@@ -165,11 +169,19 @@ static inline void or_particle_interaction(const or_domain *d, const or_lj *p,
 /* SRC/exaMD_forces.f90:308-390 on one rank: every block is interior
  * (SRC/exaMD_partitioning.f90:735-746) and interior_blks is filled with
  * k outermost, i fastest (:763-781). */
-int or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
+static int forces_midpoint(const or_domain *d, const or_lj *p,
+                           const or_species *sp, int64_t n,
                            const double *pos, double *forces,
                            int64_t *home_counts, or_tally *t)
 {
     memset(t, 0, sizeof(*t));
+    if (sp) {
+        if (sp->ntypes < 1) return 2;
+        for (int32_t a = 0; a < sp->ntypes * sp->ntypes; ++a)
+            if (sp->pair[a].rc > p->rc) return 2;
+        for (int64_t i = 0; i < n; ++i)
+            if (sp->type[i] < 0 || sp->type[i] >= sp->ntypes) return 2;
+    }
     for (int a = 0; a < 3; ++a)
         if (d->nb[a] < 5 || !(p->rc < 0.5 * d->box[a])) return 4;
     int64_t nbk = d->nblocks;
@@ -207,7 +219,7 @@ int or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
                 int64_t b0 = start[home], b1 = start[home + 1];
                 for (int64_t a1 = b0; a1 < b1 - 1; ++a1)
                     for (int64_t a2 = a1 + 1; a2 < b1; ++a2)
-                        or_particle_interaction(d, p, pos, forces, order[a1],
+                        or_particle_interaction(d, p, sp, pos, forces, order[a1],
                                                 order[a2], home, home_counts, t);
                 /* pairs 2..63 (:359-378) */
                 for (int r = 1; r < 63; ++r) {
@@ -217,7 +229,7 @@ int or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
                     int64_t s2 = start[g2], e2 = start[g2 + 1];
                     for (int64_t a1 = s1; a1 < e1; ++a1)
                         for (int64_t a2 = s2; a2 < e2; ++a2)
-                            or_particle_interaction(d, p, pos, forces,
+                            or_particle_interaction(d, p, sp, pos, forces,
                                                     order[a1], order[a2], home,
                                                     home_counts, t);
                 }
@@ -226,6 +238,21 @@ int or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
     free(order);
     free(bid);
     return 0;
+}
+
+int or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
+                           const double *pos, double *forces,
+                           int64_t *home_counts, or_tally *t)
+{
+    return forces_midpoint(d, p, NULL, n, pos, forces, home_counts, t);
+}
+
+int or_forces_midpoint_species(const or_domain *d, const or_lj *p,
+                               const or_species *sp, int64_t n,
+                               const double *pos, double *forces,
+                               int64_t *home_counts, or_tally *t)
+{
+    return forces_midpoint(d, p, sp, n, pos, forces, home_counts, t);
 }
 
 double or_wrap(double x, double L)
@@ -239,6 +266,15 @@ int or_run_vv(const or_domain *d, const or_lj *p, int64_t n, double mass,
               double dt, int64_t nsteps, double *pos, double *vel,
               double *forces, double *ke_out, double *pe_out, int64_t *cut_out)
 {
+    return or_run_vv_species(d, p, NULL, n, mass, dt, nsteps, pos, vel,
+                             forces, ke_out, pe_out, cut_out);
+}
+
+int or_run_vv_species(const or_domain *d, const or_lj *p, const or_species *sp,
+                      int64_t n, double mass, double dt, int64_t nsteps,
+                      double *pos, double *vel, double *forces, double *ke_out,
+                      double *pe_out, int64_t *cut_out)
+{
     double h = (0.5 * dt) / mass;
     for (int64_t s = 0; s < nsteps; ++s) {
         for (int64_t i = 0; i < 3 * n; ++i) {
@@ -249,7 +285,7 @@ int or_run_vv(const or_domain *d, const or_lj *p, int64_t n, double mass,
         }
         memset(forces, 0, (size_t)(3 * n) * sizeof(double));
         or_tally t;
-        int rc = or_forces_midpoint_all(d, p, n, pos, forces, NULL, &t);
+        int rc = forces_midpoint(d, p, sp, n, pos, forces, NULL, &t);
         if (rc) return rc;
         double ke = 0.0;
         for (int64_t i = 0; i < 3 * n; ++i) {

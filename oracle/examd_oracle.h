@@ -33,6 +33,19 @@ typedef struct or_lj {
     double eps, sig, rc;
 } or_lj;
 
+/* Several species: atom i has species type[i] in [0, ntypes), and the pair
+ * (i, j) uses pair[type[i] * ntypes + type[j]] (eps, sig, rc; a symmetric
+ * table, each rc at most the handle-wide cutoff that sizes the blocks).
+ * The reference declares these tables for its user function
+ * (user_func.f90:16-18: kktype, ltpvdw, prmvdw) but evaluates the first row
+ * only (prmvdw(1,:), user_func.f90:55-61); this is lennardjones_12 with the
+ * pair's own row. */
+typedef struct or_species {
+    int32_t ntypes;
+    const int32_t *type;
+    const or_lj *pair;
+} or_species;
+
 /* Per-evaluation tallies.  cand = candidate pairs visited by the midpoint
  * stencil, cut = pairs with d2 <= rc^2, zero = pairs flagged by the
  * "ZERO at" guard (|d2| < 0.001, SRC/exaMD_forces.f90:525-527), pe = sum
@@ -78,6 +91,13 @@ int  or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
                             const double *pos, double *forces,
                             int64_t *home_counts, or_tally *t);
 
+/* or_forces_midpoint_all with several species (p->rc sizes the blocks;
+ * every pair uses its species row).  Returns 2 for a bad table or type. */
+int  or_forces_midpoint_species(const or_domain *d, const or_lj *p,
+                                const or_species *sp, int64_t n,
+                                const double *pos, double *forces,
+                                int64_t *home_counts, or_tally *t);
+
 /* LJ pair of user_func.f90:40-75 plus the pair energy.  Returns retval
  * (0 = apply, 1 = beyond cutoff). */
 int  or_lennardjones_12(const or_lj *p, double dx, double dy, double dz,
@@ -92,6 +112,12 @@ int  or_run_vv(const or_domain *d, const or_lj *p, int64_t n, double mass,
                double dt, int64_t nsteps, double *pos, double *vel,
                double *forces, double *ke_out, double *pe_out,
                int64_t *cut_out);
+
+int  or_run_vv_species(const or_domain *d, const or_lj *p,
+                       const or_species *sp, int64_t n, double mass, double dt,
+                       int64_t nsteps, double *pos, double *vel,
+                       double *forces, double *ke_out, double *pe_out,
+                       int64_t *cut_out);
 
 /* Wrap one coordinate into [0, L): the build's own rule, shared with the
  * device integrator. */

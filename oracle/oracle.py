@@ -29,6 +29,11 @@ class _LJ(C.Structure):
     _fields_ = [("eps", C.c_double), ("sig", C.c_double), ("rc", C.c_double)]
 
 
+class _Species(C.Structure):
+    _fields_ = [("ntypes", C.c_int32), ("type", C.POINTER(C.c_int32)),
+                ("pair", C.POINTER(_LJ))]
+
+
 class _Tally(C.Structure):
     _fields_ = [("cand", C.c_int64), ("cut", C.c_int64), ("zero", C.c_int64),
                 ("pe", C.c_double)]
@@ -60,6 +65,13 @@ def lib():
         L.or_forces_midpoint_all.argtypes = [P(_Domain), P(_LJ), C.c_int64,
                                              P(C.c_double), P(C.c_double),
                                              P(C.c_int64), P(_Tally)]
+        L.or_forces_midpoint_species.argtypes = [
+            P(_Domain), P(_LJ), P(_Species), C.c_int64, P(C.c_double),
+            P(C.c_double), P(C.c_int64), P(_Tally)]
+        L.or_run_vv_species.argtypes = [
+            P(_Domain), P(_LJ), P(_Species), C.c_int64, C.c_double,
+            C.c_double, C.c_int64, P(C.c_double), P(C.c_double),
+            P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_int64)]
         L.or_lennardjones_12.argtypes = [P(_LJ), C.c_double, C.c_double,
                                          C.c_double, C.c_double,
                                          P(C.c_double), P(C.c_double)]
@@ -144,8 +156,25 @@ class Tally:
     pe: float
 
 
-def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False):
-    """One exaMD_forces_midpoint_all evaluation from zeroed forces.
+def _species(species, n):
+    """(types (n,), eps, sig, rc (k, k) each) -> (_Species, keep-alive)."""
+    types, eps, sig, rc = species
+    types = np.ascontiguousarray(types, dtype=np.int32).reshape(n)
+    k = np.asarray(eps).shape[0]
+    table = (_LJ * (k * k))()
+    for a in range(k):
+        for b in range(k):
+            table[a * k + b] = _LJ(float(eps[a][b]), float(sig[a][b]),
+                                   float(rc[a][b]))
+    sp = _Species(k, types.ctypes.data_as(C.POINTER(C.c_int32)), table)
+    return sp, (types, table)
+
+
+def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
+           species=None):
+    """One exaMD_forces_midpoint_all evaluation from zeroed forces.  With
+    species = (types, eps, sig, rc), every pair uses the parameters of its
+    species pair (cutoff sizes the blocks).
 
     Returns (forces (n,3), Tally, home_counts or None)."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
@@ -155,9 +184,15 @@ def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False):
     hc = np.zeros(dom.nblocks, dtype=np.int64) if home_counts else None
     t = _Tally()
     p = _LJ(eps, sig, cutoff)
-    rc = lib().or_forces_midpoint_all(
-        C.byref(dom._c), C.byref(p), n, _dp(pos), _dp(f),
-        _ip(hc) if hc is not None else None, C.byref(t))
+    if species is None:
+        rc = lib().or_forces_midpoint_all(
+            C.byref(dom._c), C.byref(p), n, _dp(pos), _dp(f),
+            _ip(hc) if hc is not None else None, C.byref(t))
+    else:
+        sp, keep = _species(species, n)
+        rc = lib().or_forces_midpoint_species(
+            C.byref(dom._c), C.byref(p), C.byref(sp), n, _dp(pos), _dp(f),
+            _ip(hc) if hc is not None else None, C.byref(t))
     if rc:
         raise ValueError(f"or_forces_midpoint_all failed ({rc})")
     return f, Tally(t.cand, t.cut, t.zero, t.pe), hc
@@ -172,21 +207,28 @@ def lennardjones_12(dx, dy, dz, d2, cutoff, eps=1.0, sig=1.0):
 
 
 def run_vv(pos, vel, box, cutoff, dt, nsteps, mass=1.0, eps=1.0, sig=1.0,
-           forces0=None):
+           forces0=None, species=None):
     """Velocity Verlet from (pos, vel); returns (pos, vel, forces, ke, pe, cut)."""
     pos = np.array(pos, dtype=np.float64, order="C")
     vel = np.array(vel, dtype=np.float64, order="C")
     n = pos.shape[0]
     if forces0 is None:
-        forces0, _, _ = forces(pos, box, cutoff, eps, sig)
+        forces0, _, _ = forces(pos, box, cutoff, eps, sig, species=species)
     f = np.array(forces0, dtype=np.float64, order="C")
     dom = domain_create(box, 0.5 * cutoff)
     ke = np.zeros(nsteps)
     pe = np.zeros(nsteps)
     cut = np.zeros(nsteps, dtype=np.int64)
     p = _LJ(eps, sig, cutoff)
-    rc = lib().or_run_vv(C.byref(dom._c), C.byref(p), n, mass, dt, nsteps,
-                         _dp(pos), _dp(vel), _dp(f), _dp(ke), _dp(pe), _ip(cut))
+    if species is None:
+        rc = lib().or_run_vv(C.byref(dom._c), C.byref(p), n, mass, dt, nsteps,
+                             _dp(pos), _dp(vel), _dp(f), _dp(ke), _dp(pe),
+                             _ip(cut))
+    else:
+        sp, keep = _species(species, n)
+        rc = lib().or_run_vv_species(C.byref(dom._c), C.byref(p), C.byref(sp),
+                                     n, mass, dt, nsteps, _dp(pos), _dp(vel),
+                                     _dp(f), _dp(ke), _dp(pe), _ip(cut))
     if rc:
         raise ValueError(f"or_run_vv failed ({rc})")
     return pos, vel, f, ke, pe, cut

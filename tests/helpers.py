@@ -126,3 +126,45 @@ def perturbed_fcc(nc, rho, amp, seed):
     pos = np.mod(pos + rng.uniform(-amp, amp, pos.shape), L)
     pos[pos >= L] = 0.0
     return pos, L
+
+
+# Kob-Andersen binary mixture (A 80%, B 20%): eps, sigma and cutoffs 2.5 sigma
+KA_EPS = [[1.0, 1.5], [1.5, 0.5]]
+KA_SIG = [[1.0, 0.8], [0.8, 0.88]]
+KA_RC = [[2.5, 2.0], [2.0, 2.2]]
+
+
+def species_brute_force(pos, box, types, eps, sig, rc):
+    """All-pairs minimum-image LJ with per-species-pair parameters:
+    (forces, pe, n_cut).  Pair (i, j) uses eps[ti][tj], sig[ti][tj] and
+    applies the force when r^2 <= rc[ti][tj]^2."""
+    pos = np.asarray(pos, dtype=np.float64)
+    box = np.asarray(box, dtype=np.float64).reshape(3)
+    types = np.asarray(types)
+    E, S, R = (np.asarray(x, dtype=np.float64) for x in (eps, sig, rc))
+    n = len(pos)
+    f = np.zeros((n, 3))
+    pe = 0.0
+    ncut = 0
+    for i in range(n):
+        d = pos - pos[i]
+        d -= np.round(d / box) * box
+        r2 = (d ** 2).sum(1)
+        r2[i] = np.inf
+        e, s, c = E[types[i], types], S[types[i], types], R[types[i], types]
+        m = r2 <= c * c
+        r2m = np.where(m, r2, 1.0)
+        sr6 = np.where(m, (s * s / r2m) ** 3, 0.0)
+        dg = np.where(m, -24 * e * (2 * sr6 * sr6 - sr6) / r2m, 0.0)
+        f[i] += (dg[:, None] * d).sum(0)
+        pe += 0.5 * np.where(m, 4 * e * (sr6 * sr6 - sr6), 0.0).sum()
+        ncut += int(m.sum())
+    return f, pe, ncut // 2
+
+
+def binary_types(n, frac_b, seed):
+    """Species 0/1 with round(frac_b n) atoms of species 1 at random."""
+    rng = np.random.default_rng(seed)
+    t = np.zeros(n, dtype=np.int32)
+    t[rng.choice(n, int(round(frac_b * n)), replace=False)] = 1
+    return t

@@ -91,3 +91,25 @@ def test_driver_checkpoint_restart_is_exact(tmp_path):
     b = open(tmp_path / "b2.chk").read().split("\n")
     assert a[0].split() == ["4000", "40"] and b[0].split() == ["4000", "40"]
     assert a[1:] == b[1:]
+
+
+@pytest.mark.gpu
+def test_driver_binary_mixture_pair_counts(tmp_path, oracle):
+    """-binary: species through grid%ltype, kktype, ltpvdw and prmvdw (every
+    fifth atom B); the driver's pair counts against the oracle's
+    species evaluation of the same lattice."""
+    from tests import helpers as H
+    r = subprocess.run([_driver(), "-atoms", "4000", "-rho", "1.2", "-binary",
+                        "-steps", "50", "-precision", "mixed"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    out = r.stdout
+    print(out[-3000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"Pairs: candidate (\d+), in cutoff (\d+)", out)
+    pos, L = H.fcc(10, 1.2)
+    types = np.array([1 if (i + 1) % 5 == 0 else 0 for i in range(len(pos))],
+                     dtype=np.int32)
+    _, t, _ = oracle.forces(pos, [L, L, L], 2.5,
+                            species=(types, H.KA_EPS, H.KA_SIG, H.KA_RC))
+    assert int(m.group(1)) == t.cand
+    assert abs(int(m.group(2)) - t.cut) <= 2

@@ -282,6 +282,76 @@ def test_particle_file_round_trip(oracle, tmp_path):
 
 
 # ------------------------------------------------------ golden files ----
+# ----------------------------------------------------------- species ----
+def _ka(pos):
+    return (H.binary_types(len(pos), 0.2, 5), H.KA_EPS, H.KA_SIG, H.KA_RC)
+
+
+def test_species_forces_match_all_pairs(oracle):
+    """Binary mixture with per-pair eps, sigma and cutoff against an
+    all-pairs sum that looks the parameters up pair by pair."""
+    pos, L = H.perturbed_fcc(6, 1.2, 0.1, 21)
+    sp = _ka(pos)
+    f, t, _ = oracle.forces(pos, [L] * 3, 2.5, species=sp)
+    fb, pe, ncut = H.species_brute_force(pos, [L] * 3, *sp)
+    scale = np.sqrt((fb ** 2).sum(1).mean())
+    assert np.abs(f - fb).max() / scale < 1e-12
+    assert t.cut == ncut
+    assert abs(t.pe - pe) < 1e-10 * abs(pe)
+    assert np.abs(f.sum(0)).max() < 1e-11 * np.abs(f).sum()
+
+
+def test_species_identical_rows_reduce_to_one_species(oracle):
+    """Every row equal to (eps, sig, rc): bit-identical to the one-species
+    evaluation, tallies included."""
+    pos, L = H.perturbed_fcc(6, 0.8442, 0.1, 22)
+    types = H.binary_types(len(pos), 0.5, 6)
+    one = [[1.3, 1.3], [1.3, 1.3]], [[0.9, 0.9], [0.9, 0.9]], [[2.5, 2.5], [2.5, 2.5]]
+    f1, t1, h1 = oracle.forces(pos, [L] * 3, 2.5, eps=1.3, sig=0.9, home_counts=True)
+    f2, t2, h2 = oracle.forces(pos, [L] * 3, 2.5, home_counts=True,
+                               species=(types,) + one)
+    assert np.array_equal(f1, f2)
+    assert (t1.cand, t1.cut, t1.pe) == (t2.cand, t2.cut, t2.pe)
+    np.testing.assert_array_equal(h1, h2)
+
+
+def test_species_pair_closed_form_and_own_cutoff(oracle):
+    """One A and one B atom: the force is the AB row's LJ at r, and it
+    vanishes between the AB cutoff (2.0) and the block cutoff (2.5)."""
+    L = 13.0
+    types = np.array([0, 1], dtype=np.int32)
+    for r, inside in ((0.9, True), (1.95, True), (2.05, False), (2.4, False)):
+        pos = np.array([[4.0, 6.0, 6.0], [4.0 + r, 6.0, 6.0]])
+        f, t, _ = oracle.forces(pos, [L] * 3, 2.5,
+                                species=(types, H.KA_EPS, H.KA_SIG, H.KA_RC))
+        s6 = (0.8 / r) ** 6
+        mag = 24 * 1.5 * (2 * s6 * s6 - s6) / r
+        if inside:
+            assert t.cut == 1
+            assert abs(f[1, 0] - mag) < 1e-12 * abs(mag)   # on B, along +x
+            assert abs(f[0, 0] + mag) < 1e-12 * abs(mag)
+            assert abs(t.pe - 4 * 1.5 * (s6 * s6 - s6)) < 1e-12
+        else:
+            assert t.cut == 0 and not f.any()
+    # the same two atoms as A-A interact out to 2.5
+    pos = np.array([[4.0, 6.0, 6.0], [6.4, 6.0, 6.0]])
+    _, t, _ = oracle.forces(pos, [L] * 3, 2.5, species=(
+        np.array([0, 0], dtype=np.int32), H.KA_EPS, H.KA_SIG, H.KA_RC))
+    assert t.cut == 1
+
+
+def test_species_bad_table_rejected(oracle):
+    pos, L = H.perturbed_fcc(5, 0.8442, 0.1, 23)
+    types = H.binary_types(len(pos), 0.2, 7)
+    rc = [[2.5, 2.6], [2.6, 2.5]]                  # beyond the block cutoff
+    with pytest.raises(ValueError):
+        oracle.forces(pos, [L] * 3, 2.5, species=(types, H.KA_EPS, H.KA_SIG, rc))
+    bad = types.copy()
+    bad[3] = 2                                     # no such species
+    with pytest.raises(ValueError):
+        oracle.forces(pos, [L] * 3, 2.5, species=(bad, H.KA_EPS, H.KA_SIG, H.KA_RC))
+
+
 def test_golden_kats(oracle):
     """Committed known-answer vectors (tests/golden/make_golden.py)."""
     path = os.path.join(GOLDEN, "kats.json")
