@@ -246,6 +246,53 @@ def slab_bench(E, steps, warmup):
     }
 
 
+KA_EPS = [[1.0, 1.5], [1.5, 0.5]]      # Kob-Andersen 80:20 binary mixture
+KA_SIG = [[1.0, 0.8], [0.8, 0.88]]
+KA_RC = [[2.5, 2.0], [2.0, 2.2]]       # 2.5 sigma of each pair
+
+
+def species_bench(E, steps, warmup, tune_after=300):
+    """Several species: Kob-Andersen 80:20 binary LJ mixture, 2,048,000 atoms
+    (80^3 FCC at rho 1.2, 20% species B at random, seeded), T* 1.0, per-pair
+    eps, sigma and cutoff 2.5 sigma_ab, mixed precision, dt 0.003, autotuned
+    after `tune_after` untimed steps; mixed atom-steps/s and the FP64 force
+    time at the same state."""
+    dt = 0.003
+    pos, vel, box = E.generate_fcc(NC, 1.2, 1.0, seed=1200)
+    rng = np.random.default_rng(1200)
+    types = np.zeros(len(pos), dtype=np.int32)
+    types[rng.choice(len(pos), len(pos) // 5, replace=False)] = 1
+    n = len(pos)
+    with E.MidpointLJ(n, box, RC, graph_chunk=100) as m:
+        m.set_state(pos, vel)
+        m.set_species(types, KA_EPS, KA_SIG, KA_RC)
+        m.run(tune_after, dt, "mixed", energy_every=0)
+        tune = m.autotune("mixed", trials=2)
+        if warmup:
+            m.run(warmup, dt, "mixed", energy_every=0)
+        m.timing(enable=True, reset=True)
+        t0 = time.perf_counter()
+        st = m.run(steps, dt, "mixed", energy_every=100)
+        wall = time.perf_counter() - t0
+        force_ms, launches = m.timing(enable=False, reset=True)
+        tune64 = m.autotune("fp64", trials=2)
+    e0 = st[0].ke + st[0].pe_shifted      # each pair's own row's V(rc)
+    e1 = st[-1].ke + st[-1].pe_shifted
+    return {
+        "workload": "Kob-Andersen 80:20 binary LJ mixture, 2,048,000 atoms "
+                    "(80^3 FCC at rho 1.2, 20% B), T* 1.0, eps/sigma/cutoff "
+                    "per species pair (cutoff 2.5 sigma_ab), mixed precision, "
+                    "dt 0.003, 100-step hipGraph chunks",
+        "atoms": n, "steps": steps,
+        "atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "force_ms": force_ms / max(launches, 1),
+        "force_ms_fp64": tune64["force_ms"],
+        "pairs_cut": st[-1].pairs_cut,
+        "shifted_energy_change_per_atom": (e1 - e0) / n,
+        "launch": tune, "launch_fp64": tune64,
+    }
+
+
 def config1(E, steps=100):
     """The reference's CPU-runnable case: 32,000 atoms (20^3 FCC, rho 0.8442,
     T* 0.72), rc 2.5, 4x4x4 cell blocks, FP64, 100 velocity-Verlet steps;
@@ -335,6 +382,7 @@ def main():
     ap.add_argument("--config1-steps", type=int, default=100)
     ap.add_argument("--loop-steps", type=int, default=200)
     ap.add_argument("--slab-steps", type=int, default=2000)
+    ap.add_argument("--species-steps", type=int, default=2000)
     ap.add_argument("--graph-chunk", type=int, default=100,
                     help="MD steps per captured hipGraph chunk (0 = eager)")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -387,6 +435,9 @@ def main():
     slab = None
     if args.slab_steps > 0:
         slab = slab_bench(E, args.slab_steps, args.warmup)
+    species = None
+    if args.species_steps > 0:
+        species = species_bench(E, args.species_steps, args.warmup)
 
     c1 = None
     if not args.no_cpu and args.config1_steps > 0:
@@ -445,6 +496,7 @@ def main():
             "graph_chunk_steps": args.graph_chunk,
             "batch": batch,
             "slab": slab,
+            "species": species,
             "config1": c1,
             "parallelism": "1 GPU",
         },

@@ -146,7 +146,7 @@ void free_state(examd_handle *h) {
   void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
                   d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.rank,
                   d.count, d.start, d.sorted, d.chunk, d.home, d.part, h->log,
-                  h->atom_seg, d.type, d.pair32, d.pair64};
+                  h->atom_seg, d.type, d.pair32, d.pair64, d.pair_shift};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DevState{};
@@ -663,7 +663,9 @@ void fill_stats(const examd_handle *h, const StepLog &L, int prec,
   s->step = step;
   s->ke = 0.5 * h->p.mass * L.ke;
   s->pe = pe;
-  s->pe_shifted = pe - (double)L.cut * vrc;
+  // several species: the V(rc) of each pair's own row, tallied on device
+  s->pe_shifted = h->d.ntypes > 0 ? pe - (double)(long long)L.shift_fix * kFixInv
+                                  : pe - (double)L.cut * vrc;
   s->etot = s->ke + pe;
   s->pairs_cand = (int64_t)L.cand;
   s->pairs_cut = (int64_t)L.cut;
@@ -987,6 +989,7 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
   DevState &d = h->d;
   std::vector<float4> p32;
   std::vector<Pair64> p64;
+  std::vector<long long> vrc;
   std::vector<unsigned char> ty;
   double max_es12 = 0.0;
   if (ntypes > 0) {
@@ -1009,6 +1012,7 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
     }
     p32.resize(kMaxSpecies * kMaxSpecies);
     p64.resize(kMaxSpecies * kMaxSpecies);
+    vrc.resize(kMaxSpecies * kMaxSpecies);
     for (int k = 0; k < nt * nt; ++k) {
       // the same roundings as the one-species constants of launch_forces
       const double sig2 = sigma[k] * sigma[k], rc2 = cutoff[k] * cutoff[k];
@@ -1017,6 +1021,8 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
       p32[k] = make_float4((float)sig2, (float)e24 * 4294967296.0f,
                            (float)e4 * 4294967296.0f, (float)rc2);
       max_es12 = std::max(max_es12, eps[k] * std::pow(sigma[k], 12.0));
+      const double sr2 = sig2 / rc2, sr6 = sr2 * sr2 * sr2;
+      vrc[k] = std::llround(4.0 * eps[k] * (sr6 * sr6 - sr6) * 4294967296.0);
     }
   }
   HIPCHK(hipSetDevice(h->dev));
@@ -1028,10 +1034,14 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
       return EXAMD_ERR_NOMEM;
     if (!d.pair64 && dalloc(&d.pair64, kMaxSpecies * kMaxSpecies))
       return EXAMD_ERR_NOMEM;
+    if (!d.pair_shift && dalloc(&d.pair_shift, kMaxSpecies * kMaxSpecies))
+      return EXAMD_ERR_NOMEM;
     HIPCHK(hipMemcpy(d.type, ty.data(), (size_t)n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.pair32, p32.data(), p32.size() * sizeof(float4),
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.pair64, p64.data(), p64.size() * sizeof(Pair64),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.pair_shift, vrc.data(), vrc.size() * sizeof(long long),
                      hipMemcpyHostToDevice));
   }
   d.ntypes = ntypes;

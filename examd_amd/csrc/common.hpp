@@ -95,6 +95,8 @@ struct StepLog {
   unsigned long long cand;   // candidate pairs
   unsigned long long cut;    // in-cutoff pairs
   unsigned long long flags;  // Flag bits
+  unsigned long long shift_fix;  // several species: sum of V(rc) of each
+                                 // in-cutoff pair's species row, 2^-32
 };
 
 // Partial tallies of one workgroup, written with plain stores and summed in
@@ -106,6 +108,7 @@ struct Partial {
   unsigned long long pe_fix;
   unsigned long long cand;
   unsigned long long cut;
+  unsigned long long shift_fix;
 };
 
 constexpr double kFixScale = 4294967296.0;          // 2^32

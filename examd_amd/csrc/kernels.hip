@@ -43,6 +43,7 @@ __device__ __forceinline__ void add_partial(Partial &a, const Partial &b) {
   a.pe_fix += b.pe_fix;
   a.cand += b.cand;
   a.cut += b.cut;
+  a.shift_fix += b.shift_fix;
 }
 
 // Workgroup sum of one Partial per thread, stored at out[slot] (all
@@ -55,6 +56,7 @@ __device__ void write_partial(Partial v, Partial *out, int slot) {
   v.pe_fix = wave_sum(v.pe_fix);
   v.cand = wave_sum(v.cand);
   v.cut = wave_sum(v.cut);
+  v.shift_fix = wave_sum(v.shift_fix);
   if ((threadIdx.x & 63) == 0) s_p[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_partials(
     const Partial *p, const int *seg, int n, StepLog *log, unsigned what) {
   const int k = blockIdx.x;
   const int i0 = seg ? seg[k] : 0, i1 = seg ? seg[k + 1] : n;
-  Partial acc{0.0, 0.0, 0ull, 0ull, 0ull};
+  Partial acc{0.0, 0.0, 0ull, 0ull, 0ull, 0ull};
   for (int i = i0 + (int)threadIdx.x; i < i1; i += kReduceThreads)
     add_partial(acc, p[i]);
   __shared__ Partial s_tot;
@@ -88,6 +90,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_partials(
       log->pe_fix += s_tot.pe_fix;
       log->cand += s_tot.cand;
       log->cut += s_tot.cut;
+      log->shift_fix += s_tot.shift_fix;
     }
   }
 }
@@ -103,7 +106,7 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n) {
 }
 __global__ void k_fill_log(StepLog *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = StepLog{0.0, 0.0, 0ull, 0ull, 0ull, 0ull};
+  if (i < n) p[i] = StepLog{0.0, 0.0, 0ull, 0ull, 0ull, 0ull, 0ull};
 }
 
 // ---------------------------------------------------------- binning ----
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     d.rank[i] = atomicAdd(&d.count[cell], 1);
   }
   if constexpr (MODE == kFinishKickDriftBin) {
-    if (want_ke) write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull},
+    if (want_ke) write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull, 0ull},
                                             d.part, blockIdx.x);
   }
   flags = wave_or(flags);
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(kBinThreads) void k_kick(DevState d, double h,
     ke = Vx * Vx + Vy * Vy + Vz * Vz;
   }
   if (want_ke)
-    write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull}, d.part,
+    write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull, 0ull}, d.part,
                                blockIdx.x);
 }
 
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(kBinThreads) void k_sys_ke(DevState d,
     ke += Vx * Vx + Vy * Vy + Vz * Vz;
   }
   __shared__ Partial s_tot;
-  write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull}, &s_tot, 0);
+  write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull, 0ull}, &s_tot, 0);
   __syncthreads();
   if (threadIdx.x == 0) log[k].ke += s_tot.ke;
 }
@@ -426,7 +429,7 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   // several species: the species-pair table, then one species byte per atom
   o = align_up(o, 16);
   L.off_pair = o;
-  if (typed) o += (size_t)kMaxSpecies * kMaxSpecies * sizeof(Pair64);
+  if (typed) o += (size_t)kMaxSpecies * kMaxSpecies * (sizeof(Pair64) + 8);
   L.off_ty = o;
   if (typed) o += (size_t)cap;
   L.total = align_up(o, 16);
@@ -596,6 +599,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   unsigned char *s_ty = smem + LL.off_ty;
   const float4 *s_pr32 = reinterpret_cast<const float4 *>(smem + LL.off_pair);
   const Pair64 *s_pr64 = reinterpret_cast<const Pair64 *>(smem + LL.off_pair);
+  const long long *s_vrc = reinterpret_cast<const long long *>(
+      smem + LL.off_pair + kMaxSpecies * kMaxSpecies * sizeof(Pair64));
   int *s_off = reinterpret_cast<int *>(smem);
   int *s_gst = reinterpret_cast<int *>(smem + LL.off_gst);
   int *s_wave = reinterpret_cast<int *>(smem + LL.off_wave);
@@ -655,7 +660,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   if (R > A.cap) {
     if (tid == 0) {
       atomicOr(&log->flags, (unsigned long long)kFlagOverflow);
-      if (ENERGY) A.d.part[tile] = Partial{0.0, 0.0, 0ull, 0ull, 0ull};
+      if (ENERGY) A.d.part[tile] = Partial{0.0, 0.0, 0ull, 0ull, 0ull, 0ull};
       if (prof) prof[2] = prof[3] = prof[4] = prof[1];
     }
     return;
@@ -722,6 +727,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     for (int k = tid; k < np; k += NT) {
       if constexpr (FP64) const_cast<Pair64 *>(s_pr64)[k] = A.d.pair64[k];
       else const_cast<float4 *>(s_pr32)[k] = A.d.pair32[k];
+      if constexpr (ENERGY) const_cast<long long *>(s_vrc)[k] = A.d.pair_shift[k];
     }
   }
   __syncthreads();
@@ -779,6 +785,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   unsigned d2min32 = 0x7f7fffffu;   // bits of a non-negative float
   double pe64 = 0.0;
   long long pefix = 0;
+  long long shiftfix = 0;   // several species, tally launches: sum of V(rc)
   // candidates tested per step (a chunk): sixteen (mixed) or four (FP64,
   // whose differences take twice the registers)
   constexpr int kChunk = FP64 ? 4 : kMixedChunk;
@@ -901,6 +908,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       const Pair64 P = params64(j);
       if (TYPED && d2 > P.rc2) return;
       pair64(j, dx, dy, dz, d2, P);
+      if constexpr (TYPED && ENERGY) shiftfix += s_vrc[trow + s_ty[j]];
       home_count(j);
       ++cut;
     };
@@ -927,6 +935,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         const float4 P = params32(j);
         if (d2 > P.w) return;
         pair32(j, dx, dy, dz, d2, P);
+        if constexpr (TYPED && ENERGY) shiftfix += s_vrc[trow + s_ty[j]];
         home_count(j);
         ++cut;
       }
@@ -1109,7 +1118,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   // tallies (pair counts, energy) only on tally launches
   if constexpr (ENERGY) {
     Partial mine{0.0, 0.0, 0ull, (unsigned long long)cand,
-                 (unsigned long long)cut};
+                 (unsigned long long)cut, (unsigned long long)shiftfix};
     if constexpr (FP64) mine.pe = pe64;
     else mine.pe_fix = (unsigned long long)pefix;
     write_partial<NT>(mine, A.d.part, tile);

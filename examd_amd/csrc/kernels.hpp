@@ -28,6 +28,7 @@ struct DevState {
   unsigned char *type = nullptr;
   float4 *pair32 = nullptr;   // sig^2, 24 eps 2^32, 4 eps 2^32, rc^2
   Pair64 *pair64 = nullptr;
+  long long *pair_shift = nullptr;  // V(rc) of each row, 2^-32 fixed point
   int ntypes = 0;
   double max_eps_sig12 = 0.0; // largest eps sig^12 of the table
 };

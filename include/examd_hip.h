@@ -82,7 +82,9 @@ typedef struct examd_stats {
     int64_t step;         /* step index (0 = the state given) */
     double ke;            /* 1/2 m sum v^2 (FP64) */
     double pe;            /* sum 4 eps (s^12 - s^6) over in-cutoff pairs */
-    double pe_shifted;    /* pe - pairs_cut * V(rc) */
+    double pe_shifted;    /* pe - pairs_cut * V(rc); with several species,
+                             pe minus the V(rc) of each in-cutoff pair's
+                             own row (tallied in 2^-32 fixed point) */
     double etot;          /* ke + pe */
     int64_t pairs_cand;   /* candidate pairs visited by the midpoint stencil */
     int64_t pairs_cut;    /* pairs with r^2 <= rc^2 */

@@ -168,3 +168,22 @@ def binary_types(n, frac_b, seed):
     t = np.zeros(n, dtype=np.int32)
     t[rng.choice(n, int(round(frac_b * n)), replace=False)] = 1
     return t
+
+
+def species_shift_sum(pos, box, types, eps, sig, rc):
+    """Sum over in-cutoff pairs of V(rc) of the pair's own species row,
+    V(rc) = 4 eps ((sig/rc)^12 - (sig/rc)^6)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    box = np.asarray(box, dtype=np.float64).reshape(3)
+    types = np.asarray(types)
+    E, S, R = (np.asarray(x, dtype=np.float64) for x in (eps, sig, rc))
+    V = 4 * E * ((S / R) ** 12 - (S / R) ** 6)
+    tot = 0.0
+    for i in range(len(pos) - 1):
+        d = pos[i + 1:] - pos[i]
+        d -= np.round(d / box) * box
+        r2 = (d ** 2).sum(1)
+        c = R[types[i], types[i + 1:]]
+        m = r2 <= c * c
+        tot += V[types[i], types[i + 1:]][m].sum()
+    return tot

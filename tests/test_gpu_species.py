@@ -48,6 +48,10 @@ def test_species_fp64_matches_oracle(E, oracle, nc, amp, seed):
     assert (s.pairs_cand, s.pairs_cut) == (t.cand, t.cut)
     assert abs(s.pe - t.pe) < FP64_TOL * abs(t.pe)
     np.testing.assert_array_equal(got, hc)
+    # shifted energy: each pair's own row's V(rc), rounded once per row to
+    # 2^-32 and tallied as integers, so within pairs * 2^-33 of the sum
+    shift = H.species_shift_sum(pos, box, ty, *KA)
+    assert abs(s.pe_shifted - (t.pe - shift)) <= t.cut * 2.0 ** -33 + 1e-12 * abs(t.pe)
 
 
 def test_species_mixed_matches_oracle(E, oracle):
