@@ -23,6 +23,7 @@ namespace examd {
 namespace {
 
 constexpr int kBinThreads = 256;
+constexpr int kBinPer = 4;   // atoms per thread of the binning kernel
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -110,9 +111,9 @@ __global__ void k_fill_log(StepLog *p, int n) {
 }
 
 // ---------------------------------------------------------- binning ----
-// One thread per atom (input order).  mode kFinishKickDriftBin closes the
-// previous step (v += h F, kinetic energy) and opens the next one; all modes
-// zero the force accumulators and assign the block.
+// Atoms in input order, kBinPer per thread.  mode kFinishKickDriftBin closes
+// the previous step (v += h F, kinetic energy) and opens the next one; all
+// modes zero the force accumulators and assign the block.
 // System of atom i: the last k with atom_seg[k] <= i.
 __device__ __forceinline__ int system_of(const SysTable &t, int64_t i) {
   int lo = 0, hi = t.nsys - 1;
@@ -124,78 +125,102 @@ __device__ __forceinline__ int system_of(const SysTable &t, int64_t i) {
   return lo;
 }
 
+// Integrate atom i (mode permitting), zero its force accumulators and
+// return its global block id (flags for a position the block map rejects).
+template <bool MIXED, int MODE>
+__device__ __forceinline__ int step_bin_atom(const DevState &d,
+                                             const SysTable &t, double h,
+                                             double dt, int64_t i, double &ke,
+                                             unsigned &flags) {
+  const SysDesc &sd = t.sys[t.nsys > 1 ? system_of(t, i) : 0];
+  const Grid g = sd.g;
+  double X = d.x[i], Y = d.y[i], Z = d.z[i];
+  if constexpr (MODE != kBinOnly) {
+    double Fx, Fy, Fz;
+    if constexpr (MIXED) {
+      Fx = (double)(long long)d.qx[i] * kFixInv;
+      Fy = (double)(long long)d.qy[i] * kFixInv;
+      Fz = (double)(long long)d.qz[i] * kFixInv;
+    } else {
+      Fx = d.fx[i];
+      Fy = d.fy[i];
+      Fz = d.fz[i];
+    }
+    double Vx = d.vx[i], Vy = d.vy[i], Vz = d.vz[i];
+    if constexpr (MODE == kFinishKickDriftBin) {
+      Vx = Vx + h * Fx;
+      Vy = Vy + h * Fy;
+      Vz = Vz + h * Fz;
+      ke += Vx * Vx + Vy * Vy + Vz * Vz;
+    }
+    Vx = Vx + h * Fx;
+    Vy = Vy + h * Fy;
+    Vz = Vz + h * Fz;
+    X = X + dt * Vx;
+    Y = Y + dt * Vy;
+    Z = Z + dt * Vz;
+    X = wrap_coord(X, g.L[0]);
+    Y = wrap_coord(Y, g.L[1]);
+    Z = wrap_coord(Z, g.L[2]);
+    d.vx[i] = Vx;
+    d.vy[i] = Vy;
+    d.vz[i] = Vz;
+    d.x[i] = X;
+    d.y[i] = Y;
+    d.z[i] = Z;
+  }
+  if constexpr (MIXED) {
+    d.qx[i] = 0ull;
+    d.qy[i] = 0ull;
+    d.qz[i] = 0ull;
+  } else {
+    d.fx[i] = 0.0;
+    d.fy[i] = 0.0;
+    d.fz[i] = 0.0;
+  }
+  int cell = 0;
+  if (!(__builtin_isfinite(X) && __builtin_isfinite(Y) &&
+        __builtin_isfinite(Z)) ||
+      !(X / g.bl < 2.0e9 && Y / g.bl < 2.0e9 && Z / g.bl < 2.0e9)) {
+    flags |= kFlagNonFinite;
+  } else {
+    int bx = block_coord(X, g.bl, g.nb[0]);
+    int by = block_coord(Y, g.bl, g.nb[1]);
+    int bz = block_coord(Z, g.bl, g.nb[2]);
+    long long num = (long long)bx + (long long)(by - 1) * g.nb[0] +
+                    (long long)(bz - 1) * ((long long)g.nb[0] * g.nb[1]);
+    long long nblk = (long long)g.nb[0] * g.nb[1] * g.nb[2];
+    if (num < 1 || num > nblk) flags |= kFlagOutOfBox;
+    else cell = sd.block0 + (int)(num - 1);
+  }
+  d.cell[i] = cell;
+  return cell;
+}
+
+// kBinPer atoms per thread (coalesced, kBinThreads apart): their counting
+// atomics are issued together, so their round trips overlap.
 template <bool MIXED, int MODE>
 __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     DevState d, SysTable t, double h, double dt, int want_ke,
     StepLog *flag_log) {
-  const int64_t i = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
+  const int64_t base =
+      (int64_t)blockIdx.x * (kBinThreads * kBinPer) + threadIdx.x;
   double ke = 0.0;
   unsigned flags = 0;
-  if (i < d.n) {
-    const SysDesc &sd = t.sys[t.nsys > 1 ? system_of(t, i) : 0];
-    const Grid g = sd.g;
-    double X = d.x[i], Y = d.y[i], Z = d.z[i];
-    if constexpr (MODE != kBinOnly) {
-      double Fx, Fy, Fz;
-      if constexpr (MIXED) {
-        Fx = (double)(long long)d.qx[i] * kFixInv;
-        Fy = (double)(long long)d.qy[i] * kFixInv;
-        Fz = (double)(long long)d.qz[i] * kFixInv;
-      } else {
-        Fx = d.fx[i];
-        Fy = d.fy[i];
-        Fz = d.fz[i];
-      }
-      double Vx = d.vx[i], Vy = d.vy[i], Vz = d.vz[i];
-      if constexpr (MODE == kFinishKickDriftBin) {
-        Vx = Vx + h * Fx;
-        Vy = Vy + h * Fy;
-        Vz = Vz + h * Fz;
-        ke = Vx * Vx + Vy * Vy + Vz * Vz;
-      }
-      Vx = Vx + h * Fx;
-      Vy = Vy + h * Fy;
-      Vz = Vz + h * Fz;
-      X = X + dt * Vx;
-      Y = Y + dt * Vy;
-      Z = Z + dt * Vz;
-      X = wrap_coord(X, g.L[0]);
-      Y = wrap_coord(Y, g.L[1]);
-      Z = wrap_coord(Z, g.L[2]);
-      d.vx[i] = Vx;
-      d.vy[i] = Vy;
-      d.vz[i] = Vz;
-      d.x[i] = X;
-      d.y[i] = Y;
-      d.z[i] = Z;
-    }
-    if constexpr (MIXED) {
-      d.qx[i] = 0ull;
-      d.qy[i] = 0ull;
-      d.qz[i] = 0ull;
-    } else {
-      d.fx[i] = 0.0;
-      d.fy[i] = 0.0;
-      d.fz[i] = 0.0;
-    }
-    int cell = 0;
-    if (!(__builtin_isfinite(X) && __builtin_isfinite(Y) &&
-          __builtin_isfinite(Z)) ||
-        !(X / g.bl < 2.0e9 && Y / g.bl < 2.0e9 && Z / g.bl < 2.0e9)) {
-      flags |= kFlagNonFinite;
-    } else {
-      int bx = block_coord(X, g.bl, g.nb[0]);
-      int by = block_coord(Y, g.bl, g.nb[1]);
-      int bz = block_coord(Z, g.bl, g.nb[2]);
-      long long num = (long long)bx + (long long)(by - 1) * g.nb[0] +
-                      (long long)(bz - 1) * ((long long)g.nb[0] * g.nb[1]);
-      long long nblk = (long long)g.nb[0] * g.nb[1] * g.nb[2];
-      if (num < 1 || num > nblk) flags |= kFlagOutOfBox;
-      else cell = sd.block0 + (int)(num - 1);
-    }
-    d.cell[i] = cell;
-    d.rank[i] = atomicAdd(&d.count[cell], 1);
+  int cell[kBinPer];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = base + (int64_t)k * kBinThreads;
+    cell[k] = i < d.n ? step_bin_atom<MIXED, MODE>(d, t, h, dt, i, ke, flags)
+                      : -1;
   }
+  int rank[kBinPer];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k)
+    if (cell[k] >= 0) rank[k] = atomicAdd(&d.count[cell[k]], 1);
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k)
+    if (cell[k] >= 0) d.rank[base + (int64_t)k * kBinThreads] = rank[k];
   if constexpr (MODE == kFinishKickDriftBin) {
     if (want_ke) write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull, 0ull},
                                             d.part, blockIdx.x);
@@ -1198,7 +1223,7 @@ int64_t partial_slots(int64_t n, int64_t tiles) {
 void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
                      int mode, double h, double dt, StepLog *ke_log,
                      StepLog *flag_log, hipStream_t s) {
-  unsigned nb = nblk(d.n, kBinThreads);
+  unsigned nb = nblk(d.n, kBinThreads * kBinPer);
   if (nb == 0) return;
   const int want = (mode == kFinishKickDriftBin && ke_log) ? 1 : 0;
 #define EXAMD_BIN(M, MD)                                               \
