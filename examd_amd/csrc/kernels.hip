@@ -323,24 +323,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(
   if (threadIdx.x == 0) chunk[blockIdx.x] = total;
 }
 
-// Exclusive scan of the chunk totals in one workgroup (any number of
-// chunks, processed kScanThreads at a time).
-__global__ __launch_bounds__(kScanThreads) void k_scan_chunks(int *chunk,
-                                                              int nchunk) {
-  __shared__ int s_wave[kScanThreads / 64 + 1];
-  int carry = 0;
-  for (int b = 0; b < nchunk; b += kScanThreads) {
-    int i = b + threadIdx.x;
-    int v = i < nchunk ? chunk[i] : 0;
-    int total;
-    int inc = block_inclusive_scan<kScanThreads>(v, s_wave, &total);
-    if (i < nchunk) chunk[i] = carry + inc - v;
-    carry += total;
-  }
-}
-
-// clear: also zero the counts for the next binning pass (else
-// k_block_order does).
+// Exclusive scan of the block counts: k_scan_reduce writes each chunk's
+// total, k_scan_down adds the totals of the chunks before its own (a few
+// hundred cached reads per workgroup) and scans inside the chunk.
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(
     int *count, int64_t n, const int *chunk, int *start, int clear) {
   __shared__ int s_wave[kScanThreads / 64 + 1];
@@ -357,9 +342,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(
     for (int k = 0; k < kScanPer; ++k)
       if (base + k < n) count[base + k] = 0;
   }
+  // offset of this chunk: the totals of the chunks before it
+  int pre = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) pre += chunk[b];
+  int offset;
+  block_inclusive_scan<kScanThreads>(pre, s_wave, &offset);
   int total;
   int inc = block_inclusive_scan<kScanThreads>(s, s_wave, &total);
-  int run = chunk[blockIdx.x] + inc - s;
+  int run = offset + inc - s;
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) {
     if (base + k < n) start[base + k] = run;
@@ -368,9 +358,24 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(
   if (base + kScanPer >= n && base < n) start[n] = run;
 }
 
-__global__ void k_scatter(DevState d) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < d.n) d.sorted[d.start[d.cell[i]] + d.rank[i]] = (int)i;
+// kBinPer atoms per thread, kBinThreads apart: the gathers of start[]
+// are issued together.
+__global__ __launch_bounds__(kBinThreads) void k_scatter(DevState d) {
+  const int64_t base =
+      (int64_t)blockIdx.x * (kBinThreads * kBinPer) + threadIdx.x;
+  int c[kBinPer], r[kBinPer];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = base + (int64_t)k * kBinThreads;
+    c[k] = i < d.n ? d.cell[i] : -1;
+    r[k] = i < d.n ? d.rank[i] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k)
+    if (c[k] >= 0) r[k] += d.start[c[k]];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k)
+    if (c[k] >= 0) d.sorted[r[k]] = (int)(base + (int64_t)k * kBinThreads);
 }
 
 // Input order inside each block (the order the reference's linked list
@@ -1264,10 +1269,10 @@ void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
                  hipStream_t s) {
   int nchunk = (int)((nblocks + kScanChunk - 1) / kScanChunk);
   k_scan_reduce<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk);
-  k_scan_chunks<<<1, kScanThreads, 0, s>>>(d.chunk, nchunk);
   k_scan_down<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk,
                                               d.start, block_order ? 0 : 1);
-  if (d.n > 0) k_scatter<<<nblk(d.n, 256), 256, 0, s>>>(d);
+  if (d.n > 0)
+    k_scatter<<<nblk(d.n, kBinThreads * kBinPer), kBinThreads, 0, s>>>(d);
   if (block_order)
     k_block_order<<<nblk(nblocks, 256), 256, 0, s>>>(d, nblocks);
 }
