@@ -1027,40 +1027,52 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
   }
   HIPCHK(hipSetDevice(h->dev));
   HIPCHK(hipStreamSynchronize(h->stream));
-  const DevState saved = d;
-  if (ntypes > 0) {
-    if (!d.type && dalloc(&d.type, h->n)) return EXAMD_ERR_NOMEM;
-    if (!d.pair32 && dalloc(&d.pair32, kMaxSpecies * kMaxSpecies))
-      return EXAMD_ERR_NOMEM;
-    if (!d.pair64 && dalloc(&d.pair64, kMaxSpecies * kMaxSpecies))
-      return EXAMD_ERR_NOMEM;
-    if (!d.pair_shift && dalloc(&d.pair_shift, kMaxSpecies * kMaxSpecies))
-      return EXAMD_ERR_NOMEM;
-    HIPCHK(hipMemcpy(d.type, ty.data(), (size_t)n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d.pair32, p32.data(), p32.size() * sizeof(float4),
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d.pair64, p64.data(), p64.size() * sizeof(Pair64),
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d.pair_shift, vrc.data(), vrc.size() * sizeof(long long),
-                     hipMemcpyHostToDevice));
-  }
-  d.ntypes = ntypes;
-  d.max_eps_sig12 = max_es12;
-  drop_graph(h);
-  h->forces_valid = -1;
-  if (h->state_set) {
-    // the species bytes and the pair table take LDS: re-check the maps
+  drop_graph(h);   // a captured chunk holds the old species state
+  // the species bytes and the pair table take LDS: with positions set,
+  // re-check the maps first, so that a failure changes nothing
+  const int old_ntypes = d.ntypes;
+  if (h->state_set && (ntypes > 0) != (old_ntypes > 0)) {
     std::vector<double> pos;
     int st = download_positions(h, pos);
+    d.ntypes = ntypes;
     for (int prec = 0; st == EXAMD_OK && prec < 2; ++prec)
       st = choose_caps(h, prec, pos.data());
     if (st != EXAMD_OK) {
-      d.ntypes = saved.ntypes;
-      d.max_eps_sig12 = saved.max_eps_sig12;
+      d.ntypes = old_ntypes;
       for (int prec = 0; prec < 2; ++prec) (void)choose_caps(h, prec, pos.data());
       return st;
     }
   }
+  if (ntypes > 0) {
+    int st = EXAMD_OK;
+    if (!d.type) st |= dalloc(&d.type, h->n);
+    if (!d.pair32) st |= dalloc(&d.pair32, kMaxSpecies * kMaxSpecies);
+    if (!d.pair64) st |= dalloc(&d.pair64, kMaxSpecies * kMaxSpecies);
+    if (!d.pair_shift) st |= dalloc(&d.pair_shift, kMaxSpecies * kMaxSpecies);
+    hipError_t e = hipSuccess;
+    if (st == EXAMD_OK) {
+      e = hipMemcpy(d.type, ty.data(), (size_t)n, hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(d.pair32, p32.data(), p32.size() * sizeof(float4),
+                      hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(d.pair64, p64.data(), p64.size() * sizeof(Pair64),
+                      hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(d.pair_shift, vrc.data(), vrc.size() * sizeof(long long),
+                      hipMemcpyHostToDevice);
+    }
+    if (st != EXAMD_OK || e != hipSuccess) {
+      // the tables may be half written: fall back to one species
+      d.ntypes = 0;
+      d.max_eps_sig12 = 0.0;
+      h->forces_valid = -1;
+      return st != EXAMD_OK ? EXAMD_ERR_NOMEM : EXAMD_ERR_HIP;
+    }
+  }
+  d.ntypes = ntypes;
+  d.max_eps_sig12 = max_es12;
+  h->forces_valid = -1;
   return EXAMD_OK;
 }
 
