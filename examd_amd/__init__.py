@@ -33,6 +33,7 @@ ERR_NONFINITE = 7
 ERR_NOT_READY = 8
 ERR_FIXED_RANGE = 9
 ERR_ZERO_DISTANCE = 10
+ERR_TABLE_RANGE = 11
 ERR_OUT_OF_BOX = 109
 
 # every symbol include/examd_hip.h declares
@@ -45,7 +46,7 @@ EXPORTS = (
     "examd_hip_set_config", "examd_hip_set_graph_chunk",
     "examd_hip_batch_create", "examd_hip_batch_info",
     "examd_hip_generate_vacancy_fcc", "examd_hip_generate_slab",
-    "examd_hip_set_species",
+    "examd_hip_set_species", "examd_hip_set_pair_table",
 )
 
 MAX_SPECIES = 8
@@ -140,6 +141,8 @@ def lib():
             C.c_double, C.c_double, C.c_uint64, dp, dp, C.c_int64, ip64]
         L.examd_hip_set_species.argtypes = [C.c_void_p, C.c_int32, ip32,
                                             C.c_int64, dp, dp, dp]
+        L.examd_hip_set_pair_table.argtypes = [C.c_void_p, C.c_int32,
+                                               C.c_double, dp, dp]
         L.examd_hip_strerror.argtypes = [C.c_int]
         L.examd_hip_strerror.restype = C.c_char_p
         for name in EXPORTS:
@@ -298,6 +301,21 @@ class _Handle:
         _check(lib().examd_hip_set_species(
             self._h, k, types.ctypes.data_as(C.POINTER(C.c_int32)), self.n,
             _dp(tab[0]), _dp(tab[1]), _dp(tab[2])), "examd_hip_set_species")
+
+    def set_pair_table(self, r2_lo=None, g=None, v=None):
+        """A tabulated pair function: g (force factor, f12 = g d) and v at
+        len(g) points uniform in r^2 from r2_lo to the cutoff squared.  With
+        no arguments, back to the Lennard-Jones pair."""
+        if g is None:
+            _check(lib().examd_hip_set_pair_table(self._h, 0, 0.0, None, None),
+                   "examd_hip_set_pair_table")
+            return
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert g.shape == v.shape and g.ndim == 1
+        _check(lib().examd_hip_set_pair_table(self._h, len(g), float(r2_lo),
+                                              _dp(g), _dp(v)),
+               "examd_hip_set_pair_table")
 
     def get_state(self, forces=False):
         pos = np.zeros((self.n, 3))

@@ -112,6 +112,7 @@ int flags_to_status(unsigned long long f) {
   if (f & kFlagOverflow) return EXAMD_ERR_TILE_OVERFLOW;
   if (f & kFlagZero) return EXAMD_ERR_ZERO_DISTANCE;
   if (f & kFlagFixedRange) return EXAMD_ERR_FIXED_RANGE;
+  if (f & kFlagTableRange) return EXAMD_ERR_TABLE_RANGE;
   return EXAMD_OK;
 }
 
@@ -146,7 +147,8 @@ void free_state(examd_handle *h) {
   void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
                   d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.rank,
                   d.count, d.start, d.sorted, d.chunk, d.home, d.part, h->log,
-                  h->atom_seg, d.type, d.pair32, d.pair64, d.pair_shift};
+                  h->atom_seg, d.type, d.pair32, d.pair64, d.pair_shift,
+                  d.tab32, d.tab64};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DevState{};
@@ -664,6 +666,7 @@ void fill_stats(const examd_handle *h, const StepLog &L, int prec,
   s->ke = 0.5 * h->p.mass * L.ke;
   s->pe = pe;
   // several species: the V(rc) of each pair's own row, tallied on device
+  if (h->d.tab_n > 0) vrc = h->d.tab_v_last;   // the table's V(rc)
   s->pe_shifted = h->d.ntypes > 0 ? pe - (double)(long long)L.shift_fix * kFixInv
                                   : pe - (double)L.cut * vrc;
   s->etot = s->ke + pe;
@@ -892,6 +895,8 @@ const char *examd_hip_strerror(int s) {
     case EXAMD_ERR_FIXED_RANGE:
       return "pair force beyond the fixed-point range";
     case EXAMD_ERR_ZERO_DISTANCE: return "pair distance^2 below 0.001";
+    case EXAMD_ERR_TABLE_RANGE:
+      return "pair closer than the tabulated pair function's range";
     case EXAMD_ERR_OUT_OF_BOX: return "block number out of range";
     default: return "unknown status";
   }
@@ -987,6 +992,7 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
                           const double *sigma, const double *cutoff) {
   if (!h || ntypes < 0 || ntypes > EXAMD_MAX_SPECIES) return EXAMD_ERR_ARG;
   DevState &d = h->d;
+  if (ntypes > 0 && d.tab_n > 0) return EXAMD_ERR_ARG;   // not with a table
   std::vector<float4> p32;
   std::vector<Pair64> p64;
   std::vector<long long> vrc;
@@ -1073,6 +1079,68 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
   d.ntypes = ntypes;
   d.max_eps_sig12 = max_es12;
   h->forces_valid = -1;
+  return EXAMD_OK;
+}
+
+int examd_hip_set_pair_table(examd_handle *h, int32_t npoints, double r2_lo,
+                             const double *g, const double *v) {
+  if (!h || npoints < 0 || npoints == 1 || npoints > EXAMD_MAX_TABLE_POINTS)
+    return EXAMD_ERR_ARG;
+  DevState &d = h->d;
+  const double rc2 = h->p.cutoff * h->p.cutoff;
+  std::vector<float4> t32;
+  std::vector<double> t64;
+  double d2fix = 0.0;   // (mixed path) see launch_forces: 2^14 per pair
+  if (npoints > 0) {
+    if (d.ntypes > 0 || !g || !v || !(r2_lo >= 0.0 && r2_lo < rc2))
+      return EXAMD_ERR_ARG;
+    for (int k = 0; k < npoints; ++k) {
+      if (!std::isfinite(g[k]) || !std::isfinite(v[k])) return EXAMD_ERR_ARG;
+      const double r2 = r2_lo + (rc2 - r2_lo) * k / (npoints - 1);
+      // a point with |f| >= 2^14: pairs below the next point are flagged
+      if (std::fabs(g[k]) * std::sqrt(r2) >= 16384.0)
+        d2fix = std::max(d2fix, r2_lo + (rc2 - r2_lo) * (k + 1) / (npoints - 1));
+    }
+    const int ni = npoints - 1;
+    t32.resize(ni);
+    t64.resize(4 * (size_t)ni);
+    for (int k = 0; k < ni; ++k) {
+      const double dg = g[k + 1] - g[k], dv = v[k + 1] - v[k];
+      t64[4 * k + 0] = g[k];
+      t64[4 * k + 1] = dg;
+      t64[4 * k + 2] = v[k];
+      t64[4 * k + 3] = dv;
+      t32[k] = make_float4((float)(g[k] * 4294967296.0), (float)(dg * 4294967296.0),
+                           (float)(v[k] * 4294967296.0), (float)(dv * 4294967296.0));
+    }
+  }
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  drop_graph(h);   // a captured chunk holds the old pair function
+  if (d.tab32) (void)hipFree(d.tab32);
+  if (d.tab64) (void)hipFree(d.tab64);
+  d.tab32 = nullptr;
+  d.tab64 = nullptr;
+  d.tab_n = 0;
+  h->forces_valid = -1;
+  if (npoints == 0) return EXAMD_OK;
+  if (dalloc(&d.tab32, npoints - 1) || dalloc(&d.tab64, 4 * (npoints - 1))) {
+    if (d.tab32) (void)hipFree(d.tab32);
+    d.tab32 = nullptr;
+    return EXAMD_ERR_NOMEM;
+  }
+  if (hipMemcpy(d.tab32, t32.data(), t32.size() * sizeof(float4),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d.tab64, t64.data(), t64.size() * sizeof(double),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return EXAMD_ERR_HIP;
+  d.tab_n = npoints;
+  d.tab_lo = r2_lo;
+  d.tab_inv_h = (double)(npoints - 1) / (rc2 - r2_lo);
+  d.tab_lof = (float)r2_lo;
+  d.tab_inv_hf = (float)d.tab_inv_h;
+  d.tab_v_last = v[npoints - 1];
+  d.tab_d2fix = d2fix;
   return EXAMD_OK;
 }
 

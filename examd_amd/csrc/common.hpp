@@ -85,6 +85,7 @@ enum Flag : unsigned {
   kFlagZero = 4u,         // pair with d2 < 0.001 ('ZERO at')
   kFlagFixedRange = 8u,   // pair force beyond the fixed-point range
   kFlagNonFinite = 16u,   // NaN/Inf position or force
+  kFlagTableRange = 32u,  // pair closer than a tabulated function's range
 };
 
 // Per-step tallies written by device kernels.
@@ -150,6 +151,9 @@ __device__ __forceinline__ long long fix_sext51(long long v) {
 // Several species: at most kMaxSpecies, a pair (i, j) uses row
 // type[i] * ntypes + type[j] of a symmetric table.
 constexpr int kMaxSpecies = 8;
+// Pair kinds of the force kernel: one Lennard-Jones row, a row per species
+// pair, or a tabulated function of r^2.
+enum : int { kPairLJ = 0, kPairSpecies = 1, kPairTable = 2 };
 struct Pair64 {
   double sig2, eps24, eps4, rc2;
 };

@@ -590,9 +590,12 @@ __device__ __forceinline__ int bc_s(int bc, int axis) {
 }
 
 // One cell block (tile) of one system, evaluated by the whole workgroup.
-template <bool FP64, bool ENERGY, bool HOMECNT, bool TYPED, int NT>
+template <bool FP64, bool ENERGY, bool HOMECNT, int PK, int NT>
 __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
                                            unsigned char *smem) {
+  // pair kind: one Lennard-Jones row, a row per species pair, or a table
+  constexpr bool TYPED = PK == kPairSpecies;
+  constexpr bool TABLE = PK == kPairTable;
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x;
   const int sys = A.t.nsys > 1 ? A.t.tile_sys[tile] : 0;
@@ -911,6 +914,55 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       if constexpr (ENERGY) pefix += fix32_scaled(P.z * s6 * (s6 - 1.0f));
       d2min32 = min(d2min32, __float_as_uint(d2));
     };
+    // a tabulated pair function, linear in r^2 between its points (the
+    // pair force f12 = g d as the reference's user function returns it)
+    auto pair64tab = [&](int j, double dx, double dy, double dz, double d2) {
+      double u = (d2 - A.d.tab_lo) * A.d.tab_inv_h;
+      if (u < 0.0) {
+        u = 0.0;
+        flags |= kFlagTableRange;
+      }
+      int k = (int)u;
+      if (k > A.d.tab_n - 2) k = A.d.tab_n - 2;
+      const double t = u - (double)k;
+      const double *e = A.d.tab64 + 4 * k;
+      const double g = e[0] + t * e[1];
+      const double f1 = g * dx, f2 = g * dy, f3 = g * dz;
+      fax += f1;
+      fay += f2;
+      faz += f3;
+      double *fj = reinterpret_cast<double *>(
+          reinterpret_cast<unsigned char *>(s_f) + __umul24((unsigned)j, 24u));
+      lds_add(fj, -f1);
+      lds_add(fj + 1, -f2);
+      lds_add(fj + 2, -f3);
+      if constexpr (ENERGY) pe64 += e[2] + t * e[3];
+      if (d2 < A.d2zero) flags |= kFlagZero;
+    };
+    auto pair32tab = [&](int j, float dx, float dy, float dz, float d2) {
+      float u = (d2 - A.d.tab_lof) * A.d.tab_inv_hf;
+      if (u < 0.0f) {
+        u = 0.0f;
+        flags |= kFlagTableRange;
+      }
+      const int k = min((int)u, A.d.tab_n - 2);
+      const float t = u - (float)k;
+      const float4 e = A.d.tab32[k];   // g, dg, V, dV, times 2^32
+      const double gs = (double)__builtin_fmaf(t, e.y, e.x);
+      const long long q1 = fix32_prod_raw(gs, dx);
+      const long long q2 = fix32_prod_raw(gs, dy);
+      const long long q3 = fix32_prod_raw(gs, dz);
+      qax += q1;
+      qay += q2;
+      qaz += q3;
+      long long *qj = reinterpret_cast<long long *>(
+          reinterpret_cast<unsigned char *>(s_q) + __umul24((unsigned)j, 24u));
+      lds_sub(qj, q1);
+      lds_sub(qj + 1, q2);
+      lds_sub(qj + 2, q3);
+      if constexpr (ENERGY) pefix += fix32_scaled(__builtin_fmaf(t, e.w, e.z));
+      d2min32 = min(d2min32, __float_as_uint(d2));
+    };
     auto home_count = [&](int j) {
       if constexpr (HOMECNT) {
         const int bj = s_bc[j];
@@ -937,7 +989,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     auto hit64 = [&](int j, double dx, double dy, double dz, double d2) {
       const Pair64 P = params64(j);
       if (TYPED && d2 > P.rc2) return;
-      pair64(j, dx, dy, dz, d2, P);
+      if constexpr (TABLE) pair64tab(j, dx, dy, dz, d2);
+      else pair64(j, dx, dy, dz, d2, P);
       if constexpr (TYPED && ENERGY) shiftfix += s_vrc[trow + s_ty[j]];
       home_count(j);
       ++cut;
@@ -964,7 +1017,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         // own species cutoff with several species)
         const float4 P = params32(j);
         if (d2 > P.w) return;
-        pair32(j, dx, dy, dz, d2, P);
+        if constexpr (TABLE) pair32tab(j, dx, dy, dz, d2);
+        else pair32(j, dx, dy, dz, d2, P);
         if constexpr (TYPED && ENERGY) shiftfix += s_vrc[trow + s_ty[j]];
         home_count(j);
         ++cut;
@@ -1161,13 +1215,13 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
 // Tiles are taken in the XCD-aware order, so the tiles of one XCD's
 // workgroups are contiguous in space (the grid is a multiple of 8 when it
 // loops).
-template <bool FP64, bool ENERGY, bool HOMECNT, bool TYPED, int NT>
+template <bool FP64, bool ENERGY, bool HOMECNT, int PK, int NT>
 __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int ntiles = A.t.ntiles;
   for (int wg = blockIdx.x; wg < ntiles; wg += gridDim.x) {
     if (wg != (int)blockIdx.x) __syncthreads();  // LDS of the last tile
-    force_tile<FP64, ENERGY, HOMECNT, TYPED, NT>(A, xcd_remap(wg, ntiles), smem);
+    force_tile<FP64, ENERGY, HOMECNT, PK, NT>(A, xcd_remap(wg, ntiles), smem);
   }
 }
 
@@ -1187,18 +1241,18 @@ __global__ void k_forces_to_fp64(DevState d, double *out, bool mixed) {
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-template <bool FP64, bool ENERGY, bool HOMECNT, bool TYPED>
+template <bool FP64, bool ENERGY, bool HOMECNT, int PK>
 void launch_forces_t(const ForceArgs &A, int threads, size_t lds, int G,
                      hipStream_t s) {
   switch (threads) {
     case 512:
-      k_forces<FP64, ENERGY, HOMECNT, TYPED, 512><<<G, 512, lds, s>>>(A);
+      k_forces<FP64, ENERGY, HOMECNT, PK, 512><<<G, 512, lds, s>>>(A);
       break;
     case 1024:
-      k_forces<FP64, ENERGY, HOMECNT, TYPED, 1024><<<G, 1024, lds, s>>>(A);
+      k_forces<FP64, ENERGY, HOMECNT, PK, 1024><<<G, 1024, lds, s>>>(A);
       break;
     default:
-      k_forces<FP64, ENERGY, HOMECNT, TYPED, 256><<<G, 256, lds, s>>>(A);
+      k_forces<FP64, ENERGY, HOMECNT, PK, 256><<<G, 256, lds, s>>>(A);
       break;
   }
 }
@@ -1305,6 +1359,8 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                16384.0;
   double rmin = pow(r13, 1.0 / 13.0);
   A.d2fix = rmin * rmin;
+  // a table: the r^2 below which its pair force may reach 2^14
+  if (d.tab_n > 0) A.d2fix = d.tab_d2fix;
   A.rc2f = (float)A.rc2;
   A.eps24f = (float)A.eps24;
   A.eps4f = (float)A.eps4;
@@ -1323,8 +1379,9 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
   if (c.workgroups > 0 && c.workgroups < G) G = (c.workgroups + 7) / 8 * 8;
   if (G > t.ntiles) G = t.ntiles;
 #define EXAMD_F(F64, E, H)                                                  \
-  (typed ? launch_forces_t<F64, E, H, true>(A, c.threads, lds, G, s)        \
-         : launch_forces_t<F64, E, H, false>(A, c.threads, lds, G, s))
+  (typed ? launch_forces_t<F64, E, H, kPairSpecies>(A, c.threads, lds, G, s) \
+   : d.tab_n > 0 ? launch_forces_t<F64, E, H, kPairTable>(A, c.threads, lds, G, s) \
+                 : launch_forces_t<F64, E, H, kPairLJ>(A, c.threads, lds, G, s))
   if (homecount) {
     if (fp64) EXAMD_F(true, false, true);
     else EXAMD_F(false, false, true);

@@ -31,6 +31,15 @@ struct DevState {
   long long *pair_shift = nullptr;  // V(rc) of each row, 2^-32 fixed point
   int ntypes = 0;
   double max_eps_sig12 = 0.0; // largest eps sig^12 of the table
+  // tabulated pair function (tab_n > 0): tab_n - 1 intervals uniform in r^2
+  // from tab_lo; per interval (g, dg, V, dV) with f12 = g * d, the mixed
+  // copy scaled by 2^32
+  float4 *tab32 = nullptr;
+  double *tab64 = nullptr;
+  int tab_n = 0;
+  double tab_lo = 0.0, tab_inv_h = 0.0, tab_v_last = 0.0;
+  double tab_d2fix = 0.0;   // r^2 below which a pair force may reach 2^14
+  float tab_lof = 0.f, tab_inv_hf = 0.f;
 };
 
 constexpr int kProfSlots = 5;   // timestamps per cell block

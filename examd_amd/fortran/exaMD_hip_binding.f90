@@ -15,7 +15,7 @@ Module exaMD_hip_binding
        EXAMD_ERR_TILE_OVERFLOW = 5, EXAMD_ERR_NO_DEVICE = 6,              &
        EXAMD_ERR_NONFINITE = 7, EXAMD_ERR_NOT_READY = 8,                  &
        EXAMD_ERR_FIXED_RANGE = 9, EXAMD_ERR_ZERO_DISTANCE = 10,           &
-       EXAMD_ERR_OUT_OF_BOX = 109
+       EXAMD_ERR_TABLE_RANGE = 11, EXAMD_ERR_OUT_OF_BOX = 109
 
   Integer(c_int), Parameter :: EXAMD_PREC_FP64 = 0, EXAMD_PREC_MIXED = 1
 
@@ -100,6 +100,15 @@ Module exaMD_hip_binding
        Integer(c_int64_t), Value :: n
        Type(c_ptr), Value :: eps, sigma, cutoff
      End Function examd_hip_set_species
+
+     Integer(c_int) Function examd_hip_set_pair_table(handle, npoints, r2_lo, &
+          g, v) Bind(C, name="examd_hip_set_pair_table")
+       Import :: c_int, c_ptr, c_int32_t, c_double
+       Type(c_ptr), Value :: handle
+       Integer(c_int32_t), Value :: npoints
+       Real(c_double), Value :: r2_lo
+       Type(c_ptr), Value :: g, v
+     End Function examd_hip_set_pair_table
 
      Integer(c_int) Function examd_hip_get_state(handle, pos, vel, force, n) &
           Bind(C, name="examd_hip_get_state")

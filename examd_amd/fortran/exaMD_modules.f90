@@ -395,10 +395,16 @@ Module exaMD_forces_mod
   Integer(c_int) :: exaMD_precision = EXAMD_PREC_FP64
   !! Statistics of the last evaluation (pair counts, energy).
   Type(examd_stats) :: exaMD_last_stats
+  !! A user function that is not the Lennard-Jones pair is sampled into a
+  !! table of exaMD_table_points points uniform in r^2 from
+  !! (exaMD_table_rmin * cutoff)^2 to cutoff^2 and interpolated on the GPU.
+  Integer :: exaMD_table_points = 4096
+  Real(kind=PRC) :: exaMD_table_rmin = 0.2_PRC
 
   Type(c_ptr), Private :: handle = c_null_ptr
   Integer(c_int64_t), Private :: handle_n = -1
   Logical, Private :: handle_species = .False.
+  Logical, Private :: handle_table = .False.
 
 Contains
 
@@ -408,15 +414,17 @@ Contains
     handle = c_null_ptr
     handle_n = -1
     handle_species = .False.
+    handle_table = .False.
   End Subroutine exaMD_release_device
 
-  !! The device path evaluates the Lennard-Jones pair; check that user_func
-  !! is that pair with prmvdw's parameters at a few distances.
-  Subroutine check_user_func(user_func, cutoff)
+  !! Is user_func the Lennard-Jones pair with prmvdw(1,:)'s parameters?
+  !! (checked at a few distances; the device evaluates that pair directly)
+  Logical Function user_func_is_lj(user_func, cutoff)
     External :: user_func
     Real(kind=PRC), Intent(in) :: cutoff
     Real(kind=PRC) :: r(4), f12(3), e12, s6, want, d2
     Integer :: i, retval
+    user_func_is_lj = .False.
     r = (/ 0.95_PRC, 1.12_PRC, 1.7_PRC, 0.99_PRC * cutoff /) * prmvdw(1, 2)
     Do i = 1, 4
        If (r(i) > cutoff) Cycle
@@ -425,11 +433,7 @@ Contains
        s6 = (prmvdw(1, 2)**2 / d2)**3
        want = -twentyfour * prmvdw(1, 1) * (two * s6 * s6 - s6) / d2 * r(i)
        If (retval /= 0 .Or. Abs(f12(1) - want) > 1.0e-9_PRC * Max(one, &
-            Abs(want))) Then
-          Write (*, *) 'exaMD: user_func is not the Lennard-Jones pair ', &
-               'with prmvdw(1,:); the GPU path evaluates only that pair'
-          Call exaMD_abort(101)
-       End If
+            Abs(want))) Return
     End Do
     d2 = (1.01_PRC * cutoff)**2
     Call user_func(1.01_PRC * cutoff, zero, zero, d2, f12, e12, retval)
@@ -437,7 +441,51 @@ Contains
        Write (*, *) 'exaMD: user_func must skip pairs beyond the cutoff'
        Call exaMD_abort(101)
     End If
-  End Subroutine check_user_func
+    user_func_is_lj = .True.
+  End Function user_func_is_lj
+
+  !! The pair function on the device: the Lennard-Jones pair, or any other
+  !! user function sampled into a table (SRC/exaMD_forces.f90:532-549: g =
+  !! f12(1) / r along x, retval = 1 -> no update, retval < 0 -> error).
+  Subroutine set_device_pair(user_func, cutoff)
+    External :: user_func
+    Real(kind=PRC), Intent(in) :: cutoff
+    Real(c_double), Allocatable, Target :: g(:), v(:)
+    Real(kind=PRC) :: r2lo, r2, r, f12(3), e12
+    Integer :: k, n, retval
+    Integer(c_int) :: st
+    If (user_func_is_lj(user_func, cutoff)) Then
+       If (handle_table) Then
+          st = examd_hip_set_pair_table(handle, 0_c_int32_t, 0.0_c_double, &
+               c_null_ptr, c_null_ptr)
+          Call check_status(st, 'examd_hip_set_pair_table')
+          handle_table = .False.
+       End If
+       Return
+    End If
+    n = exaMD_table_points
+    r2lo = (exaMD_table_rmin * cutoff)**2
+    Allocate(g(n), v(n))
+    Do k = 1, n
+       r2 = r2lo + (cutoff * cutoff - r2lo) * Real(k - 1, PRC) / Real(n - 1, PRC)
+       r = Sqrt(r2)
+       Call user_func(r, zero, zero, r2, f12, e12, retval)
+       If (retval < 0) Then
+          Write (*, *) 'Error from user_func: ', retval
+          Call exaMD_abort(101)
+       Else If (retval > 0) Then
+          g(k) = 0.0_c_double
+          v(k) = 0.0_c_double
+       Else
+          g(k) = Real(f12(1) / r, c_double)
+          v(k) = Real(e12, c_double)
+       End If
+    End Do
+    st = examd_hip_set_pair_table(handle, Int(n, c_int32_t), Real(r2lo, c_double), &
+         c_loc(g), c_loc(v))
+    Call check_status(st, 'examd_hip_set_pair_table')
+    handle_table = .True.
+  End Subroutine set_device_pair
 
   !! Flatten grid%posn (locblocks order) into pos(3, n).
   Subroutine gather_positions(pos, n)
@@ -562,9 +610,9 @@ Contains
     Integer(c_int64_t) :: n
     Integer(c_int) :: st
     Integer :: b, m, i, j, k, na
-    Call check_user_func(user_func, cutoff)
     Call gather_positions(pos, n)
     Call ensure_handle(n, cutoff)
+    Call set_device_pair(user_func, cutoff)
     Call set_device_species(n)
     st = examd_hip_set_state(handle, pos, c_null_ptr, n)
     Call check_status(st, 'examd_hip_set_state')

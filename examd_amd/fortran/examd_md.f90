@@ -15,6 +15,8 @@
 !!   -seed s         generator seed (default 1)
 !!   -restart file   start from a checkpoint (positions, velocities, box, step)
 !!   -checkpoint file write the state after the MD steps as a checkpoint
+!!   -morse          the user function is a Morse pair (De 0.8, a 2.0,
+!!                   r0 1.1), which the library tabulates for the GPU
 !!   -binary         two species (Kob-Andersen): every fifth atom in input
 !!                   order is species 2; pair rows AA (1, 1, rc 2.5),
 !!                   AB (1.5, 0.8, 2.0), BB (0.5, 0.88, 2.2) as (eps, sigma,
@@ -45,7 +47,7 @@ Program examd_md
   Type(examd_stats), Allocatable :: stats(:)
   Character(len=256) :: arg, restart_file, checkpoint_file
   Integer(8) :: step0
-  Logical :: inputcase, overlapcase, writeout, binary
+  Logical :: inputcase, overlapcase, writeout, binary, morse
   Integer :: m, p
   Integer(8) :: c0, c1, crate
   Real(c_double) :: force_ms, e0, drift
@@ -63,6 +65,7 @@ Program examd_md
   overlapcase = .False.
   writeout = .False.
   binary = .False.
+  morse = .False.
   restart_file = ''
   checkpoint_file = ''
   step0 = 0
@@ -80,6 +83,8 @@ Program examd_md
         writeout = .True.
      Case ('-binary')
         binary = .True.
+     Case ('-morse')
+        morse = .True.
      Case ('-atoms')
         ia = ia + 1
         Call get_command_argument(ia, arg)
@@ -206,7 +211,9 @@ Program examd_md
   ! one force evaluation, timed like the reference (exaMD.f90:241-253)
   Call system_clock(c0, crate)
   Call exaMD_start_comms_posn()
-  If (overlapcase) Then
+  If (morse) Then
+     Call exaMD_forces_midpoint_all(morse_pair, cutoff, noprocs)
+  Else If (overlapcase) Then
      Call exaMD_forces_midpoint_overlap(lennardjones_12, cutoff, pid, noprocs)
   Else
      Call exaMD_forces_midpoint_all(lennardjones_12, cutoff, noprocs)
@@ -287,5 +294,30 @@ Program examd_md
 
   Deallocate(kktype, ltpvdw, prmvdw)
   Call exaMD_Finalize('Done')
+
+Contains
+
+  !! A user function in the form of user_func.f90:40-75 that is not the
+  !! Lennard-Jones pair: V = De (1 - exp(-a (r - r0)))^2 - De up to the
+  !! cutoff prmvdw(1,5).
+  Subroutine morse_pair(dx, dy, dz, d2, f12, e12, retval)
+    Real(kind=PRC), Intent(in) :: dx, dy, dz, d2
+    Real(kind=PRC), Intent(out) :: f12(3), e12
+    Integer, Intent(out) :: retval
+    Real(kind=PRC), Parameter :: de = 0.8_PRC, a = 2.0_PRC, r0 = 1.1_PRC
+    Real(kind=PRC) :: r, x, g
+    retval = 0
+    If (d2 > prmvdw(1, 5)**2) Then
+       retval = 1
+       Return
+    End If
+    r = Sqrt(d2)
+    x = Exp(-a * (r - r0))
+    g = -two * de * a * (one - x) * x / r
+    f12(1) = g * dx
+    f12(2) = g * dy
+    f12(3) = g * dz
+    e12 = de * (one - x)**2 - de
+  End Subroutine morse_pair
 
 End Program examd_md

@@ -42,6 +42,8 @@ enum {
                                    51-bit range */
     EXAMD_ERR_ZERO_DISTANCE = 10,/* a pair closer than sqrt(0.001): the
                                    reference prints 'ZERO at' */
+    EXAMD_ERR_TABLE_RANGE = 11, /* a pair closer than the range of the
+                                   tabulated pair function */
     EXAMD_ERR_OUT_OF_BOX = 109  /* block number out of range (ref. 109) */
 };
 
@@ -152,9 +154,25 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
  * and cutoff).  The species and the table hold for every later evaluation
  * and run of both precisions.  At most EXAMD_MAX_SPECIES species. */
 #define EXAMD_MAX_SPECIES 8
+#define EXAMD_MAX_TABLE_POINTS 65536
 int examd_hip_set_species(examd_handle *h, int32_t ntypes,
                           const int32_t *type, int64_t n, const double *eps,
                           const double *sigma, const double *cutoff);
+
+/* A tabulated pair function in place of the Lennard-Jones pair: the
+ * general user function of exaMD_forces_midpoint_all
+ * (SRC/exaMD_forces.f90:532, user_func.f90:40-75), which a GPU kernel cannot
+ * call, sampled by the host.  g[k] and v[k] are the force factor (the pair
+ * force is f12 = g * (dx, dy, dz), as user_func returns it) and the pair
+ * energy at r^2 = r2_lo + k h, k = 0 .. npoints - 1, h = (cutoff^2 - r2_lo) /
+ * (npoints - 1), with the handle's cutoff.  Between points both are linear
+ * in r^2; a pair closer than sqrt(r2_lo) uses the first point and the call
+ * returns EXAMD_ERR_TABLE_RANGE.  A user function's retval = 1 (skip) is
+ * g = v = 0.  2 <= npoints <= EXAMD_MAX_TABLE_POINTS, 0 <= r2_lo <
+ * cutoff^2; npoints = 0 returns to the Lennard-Jones pair.  Not combined
+ * with several species.  The energy shift of pe_shifted is v[npoints - 1]. */
+int examd_hip_set_pair_table(examd_handle *h, int32_t npoints, double r2_lo,
+                             const double *g, const double *v);
 
 /* Download the current state; any pointer may be NULL.  force is the last
  * evaluated force (FP64 or fixed point converted to FP64). */

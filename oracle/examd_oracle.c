@@ -127,10 +127,35 @@ int or_lennardjones_12(const or_lj *p, double dx, double dy, double dz,
     return 0;
 }
 
+/* A tabulated pair function at d2 (within the cutoff): the linear
+ * interpolation of or_table, the result in the form of user_func. */
+static int or_table_pair(const or_table *tab, double rc, double dx, double dy,
+                         double dz, double d2, double f12[3], double *e12,
+                         int64_t *below)
+{
+    if (d2 > rc * rc) return 1;
+    double u = (d2 - tab->r2_lo) * tab->inv_h;
+    if (u < 0.0) {
+        *below += 1;
+        u = 0.0;
+    }
+    int32_t k = (int32_t)u;
+    if (k > tab->n - 2) k = tab->n - 2;
+    double tt = u - (double)k;
+    double g = tab->g[k] + tt * tab->dg[k];
+    f12[0] = g * dx;
+    f12[1] = g * dy;
+    f12[2] = g * dz;
+    *e12 = tab->v[k] + tt * tab->dv[k];
+    return 0;
+}
+
 /* SRC/exaMD_forces.f90:492-551.  With species, the pair's parameters are
- * the row of the species pair (type[i1], type[i2]). */
+ * the row of the species pair (type[i1], type[i2]); with a table, the pair
+ * function is the tabulated one. */
 static inline void or_particle_interaction(const or_domain *d, const or_lj *p,
                                            const or_species *sp,
+                                           const or_table *tab,
                                            const double *pos, double *forces,
                                            int64_t i1, int64_t i2,
                                            int64_t home, int64_t *home_counts,
@@ -152,7 +177,9 @@ static inline void or_particle_interaction(const or_domain *d, const or_lj *p,
     t->cand += 1;
     if (fabs(d2) < 0.001) t->zero += 1;     /* 'ZERO at' guard */
     double f12[3], e12;
-    if (or_lennardjones_12(p, dx, dy, dz, d2, f12, &e12) == 0) {
+    int rv = tab ? or_table_pair(tab, p->rc, dx, dy, dz, d2, f12, &e12, &t->below)
+                 : or_lennardjones_12(p, dx, dy, dz, d2, f12, &e12);
+    if (rv == 0) {
         double *f1 = &forces[3 * i1], *f2 = &forces[3 * i2];
         f1[0] = f1[0] + f12[0];
         f1[1] = f1[1] + f12[1];
@@ -170,7 +197,7 @@ static inline void or_particle_interaction(const or_domain *d, const or_lj *p,
  * (SRC/exaMD_partitioning.f90:735-746) and interior_blks is filled with
  * k outermost, i fastest (:763-781). */
 static int forces_midpoint(const or_domain *d, const or_lj *p,
-                           const or_species *sp, int64_t n,
+                           const or_species *sp, const or_table *tab, int64_t n,
                            const double *pos, double *forces,
                            int64_t *home_counts, or_tally *t)
 {
@@ -219,7 +246,7 @@ static int forces_midpoint(const or_domain *d, const or_lj *p,
                 int64_t b0 = start[home], b1 = start[home + 1];
                 for (int64_t a1 = b0; a1 < b1 - 1; ++a1)
                     for (int64_t a2 = a1 + 1; a2 < b1; ++a2)
-                        or_particle_interaction(d, p, sp, pos, forces, order[a1],
+                        or_particle_interaction(d, p, sp, tab, pos, forces, order[a1],
                                                 order[a2], home, home_counts, t);
                 /* pairs 2..63 (:359-378) */
                 for (int r = 1; r < 63; ++r) {
@@ -229,7 +256,7 @@ static int forces_midpoint(const or_domain *d, const or_lj *p,
                     int64_t s2 = start[g2], e2 = start[g2 + 1];
                     for (int64_t a1 = s1; a1 < e1; ++a1)
                         for (int64_t a2 = s2; a2 < e2; ++a2)
-                            or_particle_interaction(d, p, sp, pos, forces,
+                            or_particle_interaction(d, p, sp, tab, pos, forces,
                                                     order[a1], order[a2], home,
                                                     home_counts, t);
                 }
@@ -244,7 +271,7 @@ int or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
                            const double *pos, double *forces,
                            int64_t *home_counts, or_tally *t)
 {
-    return forces_midpoint(d, p, NULL, n, pos, forces, home_counts, t);
+    return forces_midpoint(d, p, NULL, NULL, n, pos, forces, home_counts, t);
 }
 
 int or_forces_midpoint_species(const or_domain *d, const or_lj *p,
@@ -252,7 +279,16 @@ int or_forces_midpoint_species(const or_domain *d, const or_lj *p,
                                const double *pos, double *forces,
                                int64_t *home_counts, or_tally *t)
 {
-    return forces_midpoint(d, p, sp, n, pos, forces, home_counts, t);
+    return forces_midpoint(d, p, sp, NULL, n, pos, forces, home_counts, t);
+}
+
+int or_forces_midpoint_table(const or_domain *d, const or_lj *p,
+                             const or_table *tab, int64_t n,
+                             const double *pos, double *forces,
+                             int64_t *home_counts, or_tally *t)
+{
+    if (!tab || tab->n < 2) return 2;
+    return forces_midpoint(d, p, NULL, tab, n, pos, forces, home_counts, t);
 }
 
 double or_wrap(double x, double L)
@@ -285,7 +321,7 @@ int or_run_vv_species(const or_domain *d, const or_lj *p, const or_species *sp,
         }
         memset(forces, 0, (size_t)(3 * n) * sizeof(double));
         or_tally t;
-        int rc = forces_midpoint(d, p, sp, n, pos, forces, NULL, &t);
+        int rc = forces_midpoint(d, p, sp, NULL, n, pos, forces, NULL, &t);
         if (rc) return rc;
         double ke = 0.0;
         for (int64_t i = 0; i < 3 * n; ++i) {

@@ -46,6 +46,19 @@ typedef struct or_species {
     const or_lj *pair;
 } or_species;
 
+/* A tabulated pair function: g(r^2) and V(r^2) at n >= 2 points uniform in
+ * r^2 from r2_lo to r2_lo + (n - 1) h (the cutoff squared), g the factor
+ * of the pair force f12 = g * (dx, dy, dz) and V the pair energy, as the
+ * reference's user function returns them (user_func.f90:40-75).  Between
+ * points: u = (d2 - r2_lo) * inv_h, k = min(floor(u), n - 2), t = u - k,
+ * g = g[k] + t * dg[k] with dg[k] = g[k+1] - g[k] (V likewise).  A pair
+ * closer than r2_lo is counted in `below` and uses the first point. */
+typedef struct or_table {
+    int32_t n;
+    double r2_lo, inv_h;
+    const double *g, *dg, *v, *dv;   /* n - 1 intervals each (g, v: n) */
+} or_table;
+
 /* Per-evaluation tallies.  cand = candidate pairs visited by the midpoint
  * stencil, cut = pairs with d2 <= rc^2, zero = pairs flagged by the
  * "ZERO at" guard (|d2| < 0.001, SRC/exaMD_forces.f90:525-527), pe = sum
@@ -54,6 +67,7 @@ typedef struct or_species {
 typedef struct or_tally {
     int64_t cand, cut, zero;
     double pe;
+    int64_t below;      /* tabulated function: pairs closer than its range */
 } or_tally;
 
 int  or_domain_create(const double box[3], double block_length, or_domain *d);
@@ -97,6 +111,13 @@ int  or_forces_midpoint_species(const or_domain *d, const or_lj *p,
                                 const or_species *sp, int64_t n,
                                 const double *pos, double *forces,
                                 int64_t *home_counts, or_tally *t);
+
+/* or_forces_midpoint_all with a tabulated pair function (p->rc sizes the
+ * blocks and is the cutoff). */
+int  or_forces_midpoint_table(const or_domain *d, const or_lj *p,
+                              const or_table *tab, int64_t n,
+                              const double *pos, double *forces,
+                              int64_t *home_counts, or_tally *t);
 
 /* LJ pair of user_func.f90:40-75 plus the pair energy.  Returns retval
  * (0 = apply, 1 = beyond cutoff). */

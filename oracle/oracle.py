@@ -34,9 +34,15 @@ class _Species(C.Structure):
                 ("pair", C.POINTER(_LJ))]
 
 
+class _Table(C.Structure):
+    _fields_ = [("n", C.c_int32), ("r2_lo", C.c_double), ("inv_h", C.c_double),
+                ("g", C.POINTER(C.c_double)), ("dg", C.POINTER(C.c_double)),
+                ("v", C.POINTER(C.c_double)), ("dv", C.POINTER(C.c_double))]
+
+
 class _Tally(C.Structure):
     _fields_ = [("cand", C.c_int64), ("cut", C.c_int64), ("zero", C.c_int64),
-                ("pe", C.c_double)]
+                ("pe", C.c_double), ("below", C.c_int64)]
 
 
 _lib = None
@@ -67,6 +73,9 @@ def lib():
                                              P(C.c_int64), P(_Tally)]
         L.or_forces_midpoint_species.argtypes = [
             P(_Domain), P(_LJ), P(_Species), C.c_int64, P(C.c_double),
+            P(C.c_double), P(C.c_int64), P(_Tally)]
+        L.or_forces_midpoint_table.argtypes = [
+            P(_Domain), P(_LJ), P(_Table), C.c_int64, P(C.c_double),
             P(C.c_double), P(C.c_int64), P(_Tally)]
         L.or_run_vv_species.argtypes = [
             P(_Domain), P(_LJ), P(_Species), C.c_int64, C.c_double,
@@ -154,6 +163,7 @@ class Tally:
     cut: int
     zero: int
     pe: float
+    below: int = 0
 
 
 def _species(species, n):
@@ -170,8 +180,20 @@ def _species(species, n):
     return sp, (types, table)
 
 
+def table_arrays(table, cutoff):
+    """(r2_lo, g, v) sampled at len(g) points uniform in r^2 up to cutoff^2
+    -> (n, r2_lo, inv_h, g, dg, v, dv) as the interpolation uses them."""
+    r2_lo, g, v = table
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n = len(g)
+    inv_h = float(n - 1) / (cutoff * cutoff - float(r2_lo))
+    return n, float(r2_lo), inv_h, g, np.ascontiguousarray(g[1:] - g[:-1]), \
+        v, np.ascontiguousarray(v[1:] - v[:-1])
+
+
 def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
-           species=None):
+           species=None, table=None):
     """One exaMD_forces_midpoint_all evaluation from zeroed forces.  With
     species = (types, eps, sig, rc), every pair uses the parameters of its
     species pair (cutoff sizes the blocks).
@@ -184,7 +206,13 @@ def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
     hc = np.zeros(dom.nblocks, dtype=np.int64) if home_counts else None
     t = _Tally()
     p = _LJ(eps, sig, cutoff)
-    if species is None:
+    if table is not None:
+        tn, lo, ih, g, dg, v, dv = table_arrays(table, cutoff)
+        tab = _Table(tn, lo, ih, _dp(g), _dp(dg), _dp(v), _dp(dv))
+        rc = lib().or_forces_midpoint_table(
+            C.byref(dom._c), C.byref(p), C.byref(tab), n, _dp(pos), _dp(f),
+            _ip(hc) if hc is not None else None, C.byref(t))
+    elif species is None:
         rc = lib().or_forces_midpoint_all(
             C.byref(dom._c), C.byref(p), n, _dp(pos), _dp(f),
             _ip(hc) if hc is not None else None, C.byref(t))
@@ -195,7 +223,7 @@ def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
             _ip(hc) if hc is not None else None, C.byref(t))
     if rc:
         raise ValueError(f"or_forces_midpoint_all failed ({rc})")
-    return f, Tally(t.cand, t.cut, t.zero, t.pe), hc
+    return f, Tally(t.cand, t.cut, t.zero, t.pe, t.below), hc
 
 
 def lennardjones_12(dx, dy, dz, d2, cutoff, eps=1.0, sig=1.0):

@@ -187,3 +187,42 @@ def species_shift_sum(pos, box, types, eps, sig, rc):
         m = r2 <= c * c
         tot += V[types[i], types[i + 1:]][m].sum()
     return tot
+
+
+def lj_table(npoints, r_lo, rc, eps=1.0, sig=1.0):
+    """The LJ pair sampled for a tabulated pair function: (r2_lo, g, v) at
+    npoints uniform in r^2 from r_lo^2 to rc^2, g = f12 / d."""
+    r2 = np.linspace(r_lo * r_lo, rc * rc, npoints)
+    s6 = (sig * sig / r2) ** 3
+    g = -24.0 * eps * (2.0 * s6 * s6 - s6) / r2
+    v = 4.0 * eps * (s6 * s6 - s6)
+    return r_lo * r_lo, g, v
+
+
+def table_brute_force(pos, box, rc, table):
+    """All-pairs minimum-image sum of a tabulated pair function, linear in
+    r^2 between the points: (forces, pe, n_cut)."""
+    r2_lo, g, v = table
+    g, v = np.asarray(g), np.asarray(v)
+    n_pts = len(g)
+    inv_h = (n_pts - 1) / (rc * rc - r2_lo)
+    pos = np.asarray(pos, dtype=np.float64)
+    box = np.asarray(box, dtype=np.float64).reshape(3)
+    n = len(pos)
+    f = np.zeros((n, 3))
+    pe = 0.0
+    ncut = 0
+    for i in range(n):
+        d = pos - pos[i]
+        d -= np.round(d / box) * box
+        r2 = (d ** 2).sum(1)
+        r2[i] = np.inf
+        m = r2 <= rc * rc
+        u = np.clip((r2[m] - r2_lo) * inv_h, 0.0, None)
+        k = np.minimum(u.astype(np.int64), n_pts - 2)
+        t = u - k
+        gg = g[k] + t * (g[k + 1] - g[k])
+        f[i] += (gg[:, None] * d[m]).sum(0)
+        pe += 0.5 * (v[k] + t * (v[k + 1] - v[k])).sum()
+        ncut += int(m.sum())
+    return f, pe, ncut // 2

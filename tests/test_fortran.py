@@ -113,3 +113,29 @@ def test_driver_binary_mixture_pair_counts(tmp_path, oracle):
                             species=(types, H.KA_EPS, H.KA_SIG, H.KA_RC))
     assert int(m.group(1)) == t.cand
     assert abs(int(m.group(2)) - t.cut) <= 2
+
+
+@pytest.mark.gpu
+def test_driver_tabulates_a_general_user_function(tmp_path, oracle):
+    """-morse: the user function is not the LJ pair, so the library samples
+    it (4096 points from 0.2 rc) and the GPU interpolates the table; the
+    driver's pair counts and energy against the oracle's evaluation of the
+    same table."""
+    from tests import helpers as H
+    r = subprocess.run([_driver(), "-atoms", "4000", "-morse", "-steps", "20"],
+                       cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    out = r.stdout
+    print(out[-3000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"Pairs: candidate (\d+), in cutoff (\d+)", out)
+    pe = float(re.search(r"Potential energy:\s*(\S+)", out).group(1))
+    pos, L = H.fcc(10, 0.8442)
+    r2 = np.linspace((0.2 * 2.5) ** 2, 2.5 ** 2, 4096)
+    rr = np.sqrt(r2)
+    x = np.exp(-2.0 * (rr - 1.1))
+    g = -2 * 0.8 * 2.0 * (1 - x) * x / rr
+    v = 0.8 * (1 - x) ** 2 - 0.8
+    _, t, _ = oracle.forces(pos, [L, L, L], 2.5, table=(r2[0], g, v))
+    assert (int(m.group(1)), int(m.group(2))) == (t.cand, t.cut)
+    assert abs(pe - t.pe) < 1e-6 * abs(t.pe)

@@ -352,6 +352,52 @@ def test_species_bad_table_rejected(oracle):
         oracle.forces(pos, [L] * 3, 2.5, species=(bad, H.KA_EPS, H.KA_SIG, H.KA_RC))
 
 
+# ------------------------------------------------------------- table ----
+def test_table_forces_match_all_pairs(oracle):
+    """A tabulated pair function against an all-pairs sum that interpolates
+    the same table independently."""
+    pos, L = H.perturbed_fcc(6, 0.8442, 0.1, 31)
+    tab = H.lj_table(2001, 0.8, 2.5)
+    f, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=tab)
+    fb, pe, ncut = H.table_brute_force(pos, [L] * 3, 2.5, tab)
+    scale = np.sqrt((fb ** 2).sum(1).mean())
+    assert np.abs(f - fb).max() / scale < 1e-12
+    assert t.cut == ncut and t.below == 0
+    assert abs(t.pe - pe) < 1e-10 * abs(pe)
+
+
+def test_table_of_lj_approaches_lj(oracle):
+    """The LJ pair tabulated on a fine grid reproduces the analytic
+    evaluation within the linear-interpolation bound, which falls as h^2."""
+    pos, L = H.perturbed_fcc(6, 0.8442, 0.1, 32)
+    ref, t0, _ = oracle.forces(pos, [L] * 3, 2.5)
+    scale = np.sqrt((ref ** 2).sum(1).mean())
+    errs = []
+    for npts in (1001, 4001):
+        f, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=H.lj_table(npts, 0.8, 2.5))
+        errs.append(np.abs(f - ref).max() / scale)
+        assert t.cut == t0.cut
+    assert errs[1] < 1e-4
+    assert 10 < errs[0] / errs[1] < 22            # second order: about 16x
+
+
+def test_table_linear_in_r2_is_exact(oracle):
+    """g and V linear in r^2 are interpolated exactly: two atoms at r give
+    the closed form, and a pair closer than the table is counted."""
+    L = 13.0
+    r2 = np.linspace(0.25, 6.25, 7)
+    g, v = 3.0 - 0.5 * r2, 1.0 + 2.0 * r2
+    for r in (0.7, 1.3, 2.2):
+        pos = np.array([[4.0, 6.0, 6.0], [4.0 + r, 6.0, 6.0]])
+        f, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=(0.25, g, v))
+        want = (3.0 - 0.5 * r * r) * r
+        assert abs(f[0, 0] - want) < 1e-13 and abs(f[1, 0] + want) < 1e-13
+        assert abs(t.pe - (1.0 + 2.0 * r * r)) < 1e-13
+    pos = np.array([[4.0, 6.0, 6.0], [4.3, 6.0, 6.0]])    # r^2 = 0.09 < 0.25
+    _, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=(0.25, g, v))
+    assert t.below == 1
+
+
 def test_golden_kats(oracle):
     """Committed known-answer vectors (tests/golden/make_golden.py)."""
     path = os.path.join(GOLDEN, "kats.json")
