@@ -293,6 +293,49 @@ def species_bench(E, steps, warmup, tune_after=300):
     }
 
 
+def table_bench(E, steps, warmup, tune_after=300, npoints=4096, r_lo=0.6):
+    """A tabulated pair function: the headline 2,048,000-atom liquid with
+    the LJ pair given as a table (npoints uniform in r^2 from r_lo to rc,
+    linear between them), the path a general user function takes; mixed
+    atom-steps/s, force time against the analytic pair at the same state,
+    and the tabulated forces' error against the analytic ones."""
+    pos, vel, box = E.generate_fcc(NC, RHO, TEMP, seed=2048)
+    r2 = np.linspace(r_lo * r_lo, RC * RC, npoints)
+    s6 = (1.0 / r2) ** 3
+    g = -24.0 * (2.0 * s6 * s6 - s6) / r2
+    v = 4.0 * (s6 * s6 - s6)
+    n = len(pos)
+    with E.MidpointLJ(n, box, RC, graph_chunk=100) as m:
+        m.set_state(pos, vel)
+        m.run(tune_after, DT, "mixed", energy_every=0)
+        f_lj, _, _ = m.forces("fp64")
+        m.set_pair_table(r2[0], g, v)
+        f_tab, _, _ = m.forces("fp64")
+        tune = m.autotune("mixed", trials=2)
+        if warmup:
+            m.run(warmup, DT, "mixed", energy_every=0)
+        m.timing(enable=True, reset=True)
+        t0 = time.perf_counter()
+        st = m.run(steps, DT, "mixed", energy_every=100)
+        wall = time.perf_counter() - t0
+        force_ms, launches = m.timing(enable=False, reset=True)
+    rms = np.sqrt((f_lj ** 2).sum(1).mean())
+    e0 = st[0].ke + st[0].pe_shifted
+    e1 = st[-1].ke + st[-1].pe_shifted
+    return {
+        "workload": "LJ liquid 2,048,000 atoms (the headline system) with the "
+                    f"LJ pair as a table of {npoints} points uniform in r^2 "
+                    f"from {r_lo} sigma to rc 2.5, linear in r^2, mixed "
+                    "precision, dt 0.005",
+        "atoms": n, "steps": steps,
+        "atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "force_ms": force_ms / max(launches, 1),
+        "fp64_table_vs_analytic_max_rel": float(np.abs(f_tab - f_lj).max() / rms),
+        "shifted_energy_change_per_atom": (e1 - e0) / n,
+        "launch": tune,
+    }
+
+
 def config1(E, steps=100):
     """The reference's CPU-runnable case: 32,000 atoms (20^3 FCC, rho 0.8442,
     T* 0.72), rc 2.5, 4x4x4 cell blocks, FP64, 100 velocity-Verlet steps;
@@ -383,6 +426,7 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=200)
     ap.add_argument("--slab-steps", type=int, default=2000)
     ap.add_argument("--species-steps", type=int, default=2000)
+    ap.add_argument("--table-steps", type=int, default=1000)
     ap.add_argument("--graph-chunk", type=int, default=100,
                     help="MD steps per captured hipGraph chunk (0 = eager)")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -438,6 +482,9 @@ def main():
     species = None
     if args.species_steps > 0:
         species = species_bench(E, args.species_steps, args.warmup)
+    table = None
+    if args.table_steps > 0:
+        table = table_bench(E, args.table_steps, args.warmup)
 
     c1 = None
     if not args.no_cpu and args.config1_steps > 0:
@@ -497,6 +544,7 @@ def main():
             "batch": batch,
             "slab": slab,
             "species": species,
+            "table": table,
             "config1": c1,
             "parallelism": "1 GPU",
         },
