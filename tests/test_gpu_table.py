@@ -136,3 +136,24 @@ def test_table_md_conserves_energy(E):
     e = np.array([r.ke + r.pe_shifted for r in st])
     assert np.all(np.isfinite(e))
     assert np.abs(e - e[0]).max() < 1e-3 * np.abs(e).max()
+
+
+def test_table_batch_equals_standalone(E):
+    """A table in a batch handle: each system's mixed forces bit-identical
+    to the same system alone with the same table."""
+    print("[table batch]", flush=True)
+    tab = _morse_table(2049, 0.6, 2.5)
+    sy = [H.perturbed_fcc(nc, 0.8442, 0.1, s) for nc, s in ((6, 51), (8, 52))]
+    sy = [(p, np.array([L, L, L])) for p, L in sy]
+    with E.MidpointLJBatch([(len(p), b) for p, b in sy], 2.5) as bt:
+        bt.set_state(np.vstack([p for p, _ in sy]))
+        bt.set_pair_table(*tab)
+        fb, sb, _ = bt.forces("mixed")
+        off = bt.offsets
+    for k, (pos, box) in enumerate(sy):
+        with E.MidpointLJ(len(pos), box, 2.5) as m:
+            m.set_state(pos)
+            m.set_pair_table(*tab)
+            f1, s1, _ = m.forces("mixed")
+        assert np.array_equal(fb[off[k]:off[k + 1]], f1), k
+        assert (sb[k].pe, sb[k].pairs_cut) == (s1.pe, s1.pairs_cut), k
