@@ -696,7 +696,7 @@ int apply_config(examd_handle *h, int prec, const int *T, double edge,
                  bool balance, const double *pos) {
   TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
-  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 4 ? 4 : 8);
+  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 4 ? 4 : (lane_queue <= 8 ? 8 : 16));
   if (T) {
     for (int a = 0; a < 3; ++a)
       c.T[0][a] = std::max(1, std::min(T[a], h->sys[0].g.nb[a]));
@@ -1432,11 +1432,12 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
   // candidate cell-block sizes (home blocks per side) and launch shapes; a
   // single system takes fractional edges as explicit maps, a batch integer
   // edges for every system
-  static const double sizes1[] = {3.5, 4.0, 4.5, 5.0, 5.5, 6.0, 6.7, 7.5, 8.5};
+  static const double sizes1[] = {3.5, 4.0, 4.5, 5.0, 5.5, 6.0, 6.7, 7.5,
+                                  8.5, 9.0, 10.0};
   static const double sizesN[] = {3.0, 4.0, 5.0, 6.0, 7.0, 8.0};
   const bool single = h->nsys == 1;
   const double *sizes = single ? sizes1 : sizesN;
-  const int nsizes = single ? 9 : 6;
+  const int nsizes = single ? 11 : 6;
   static const int threads[] = {256, 512, 1024};
   const bool save_timing = h->timing;
   const double save_ms = h->force_ms;
@@ -1445,6 +1446,9 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
   const int ns = h->nsys;
   TileConfig bestc = start;
   double best_ms = 1e300;
+  // the fastest few shapes of the first pass, refined in the second
+  constexpr int kRefine = 3;
+  std::vector<std::pair<double, TileConfig>> top;
   // time `trials` evaluations of the current shape (1e300 if it overflowed)
   auto time_shape = [&](double *ms) -> int {
     launch_fill_log(h->log, ns, h->stream);
@@ -1482,6 +1486,12 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
       best_ms = ms;
       bestc = h->tc[precision];
     }
+    if (ms < 1e300) {
+      top.emplace_back(ms, h->tc[precision]);
+      std::sort(top.begin(), top.end(),
+                [](const auto &x, const auto &y) { return x.first < y.first; });
+      if ((int)top.size() > kRefine) top.resize(kRefine);
+    }
     return EXAMD_OK;
   };
   for (int si = 0; si < nsizes; ++si)
@@ -1500,31 +1510,26 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
             return st;
           }
         }
-  // for the best shape: balanced boundaries, then persistent grids of a
-  // few workgroups per CU
-  if (best_ms < 1e300) {
-    TileConfig b = bestc;
-    st = consider(b.edge == 0 ? b.T[0].data() : nullptr, (double)b.edge,
-                  b.threads, b.fixed_groups, b.lane_queue, 0, true);
-    if (st) {
-      h->timing = save_timing;
-      return st;
-    }
-    b = bestc;
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-                              h->dev) != hipSuccess || cus <= 0)
-      cus = 256;
-    for (int per = 1; per <= 4; per *= 2) {
-      const int wgs = cus * per;
-      if (wgs >= (int)tiles_of(b)) break;
-      st = consider(b.edge == 0 ? b.T[0].data() : nullptr, (double)b.edge,
-                    b.threads, b.fixed_groups, b.lane_queue, wgs, b.balance);
-      if (st) {
-        h->timing = save_timing;
-        return st;
+  // for the fastest few shapes: balanced boundaries, then persistent
+  // grids of a few workgroups per CU (a shape can win only in one of them)
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            h->dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  const std::vector<std::pair<double, TileConfig>> firsts = top;
+  for (const auto &cand : firsts) {
+    const TileConfig &b = cand.second;
+    for (int bal = 0; bal <= 1; ++bal)
+      for (int per = bal ? 0 : 1; per <= 4; per = per ? per * 2 : 1) {
+        const int wgs = cus * per;
+        if (per > 0 && wgs >= (int)tiles_of(b)) break;
+        st = consider(b.edge == 0 ? b.T[0].data() : nullptr, (double)b.edge,
+                      b.threads, b.fixed_groups, b.lane_queue, wgs, bal != 0);
+        if (st) {
+          h->timing = save_timing;
+          return st;
+        }
       }
-    }
   }
   h->timing = save_timing;
   h->force_ms = save_ms;
