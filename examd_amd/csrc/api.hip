@@ -696,7 +696,7 @@ int apply_config(examd_handle *h, int prec, const int *T, double edge,
                  bool balance, const double *pos) {
   TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
-  c.lane_queue = lane_queue <= 0 ? 0 : (lane_queue <= 4 ? 4 : (lane_queue <= 8 ? 8 : 16));
+  c.lane_queue = std::max(0, std::min(16, lane_queue));
   if (T) {
     for (int a = 0; a < 3; ++a)
       c.T[0][a] = std::max(1, std::min(T[a], h->sys[0].g.nb[a]));

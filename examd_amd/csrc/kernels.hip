@@ -433,8 +433,13 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   cap += kChunkPad;
   LdsLayout L;
   size_t o = (size_t)(maxrb + 1) * 4;  // s_off
+  // the global starts of the region blocks are read only by the gather, the
+  // lane queues only by the pair phase: they share space when the queues
+  // are large enough
+  const size_t qbytes = (size_t)nwaves * 64 * qdepth * 4;
+  const bool gst_in_q = qbytes >= (size_t)maxrb * 4;
   L.off_gst = o;
-  o += (size_t)maxrb * 4;
+  if (!gst_in_q) o += (size_t)maxrb * 4;
   L.off_wave = o;
   o += 32 * 4;
   o = align_up(o, 16);
@@ -455,7 +460,8 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   o += (kWorkBins + 1) * 4;
   o = align_up(o, 16);
   L.off_q = o;
-  o += (size_t)nwaves * 64 * qdepth * 4;
+  if (gst_in_q) L.off_gst = o;
+  o += qbytes;
   // several species: the species-pair table, then one species byte per atom
   o = align_up(o, 16);
   L.off_pair = o;

@@ -69,8 +69,8 @@ struct ForceConfig {
   int cap = 0;        // LDS atom capacity per cell block
   int maxrb = 0;      // largest region (home + shell) in blocks
   int groups = 1;     // run groups per atom (work items = groups x atoms)
-  int lane_queue = 0; // entries per lane queue of deferred pairs (0, 4, 8);
-                      // an entry holds the hits of four candidates
+  int lane_queue = 0; // entries per lane queue of deferred pairs (0..16);
+                      // an entry holds the hits of one candidate chunk
   int workgroups = 0; // persistent grid: workgroups that loop over the tiles
                       // (0 = one workgroup per tile)
 };

@@ -227,9 +227,8 @@ typedef struct examd_tune {
     int32_t lane_queue;   /* entries of the per-lane queues of deferred
                              in-cutoff pairs (an entry holds the hits among
                              one chunk of candidates, sixteen in the mixed
-                             path, four in FP64): 0 (evaluate at once), 4
-                             or 8;
-                             other values round up to 4 or down to 8 */
+                             path, four in FP64): 0 (evaluate at once) to
+                             16; the autotuner tries 0, 4 and 8 */
     int32_t tile_edge;    /* set_config: when tiles are 0, every system's map
                              is T = max(1, round(nblocks_xyz / tile_edge));
                              get_config: the edge the map came from (0 when
