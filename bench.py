@@ -103,17 +103,17 @@ def force_error(m):
 
 def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
               shape=None, graph_chunk=100, report_error=False, tune_after=300):
-    """Autotune the launch shape for this precision after `tune_after`
-    untimed steps (or apply the given shape), then warm up and time `steps`
-    MD steps, replayed from captured hipGraph chunks of `graph_chunk` steps.
-    With report_error, the mixed-vs-FP64 force error is taken at the final
-    positions (outside the timed region)."""
+    """Run `tune_after` untimed steps (melting the lattice), then autotune
+    the launch shape for this precision (or apply the given shape), warm up
+    and time `steps` MD steps, replayed from captured hipGraph chunks of
+    `graph_chunk` steps.  With report_error, the mixed-vs-FP64 force error is
+    taken at the final positions (outside the timed region)."""
     err = None
     with E.MidpointLJ(len(pos), box, RC, graph_chunk=graph_chunk) as m:
         m.set_state(pos, vel)
+        if tune_after:
+            m.run(tune_after, DT, prec, energy_every=0)
         if shape is None:
-            if tune_after:
-                m.run(tune_after, DT, prec, energy_every=0)
             tune = m.autotune(prec, trials=3)
         else:
             m.set_config(prec, **shape)
@@ -390,12 +390,12 @@ def pmc_probe(E, shape, melt=300):
     pos, vel, box = E.generate_fcc(NC, RHO, TEMP, seed=2048)
     with E.MidpointLJ(len(pos), box, RC) as m:
         m.set_state(pos, vel)
-        if shape is not None:
-            m.set_config("mixed", **shape)
         if melt:
             m.run(melt, DT, "mixed", energy_every=0)
         if shape is None:
             print("tune", m.autotune("mixed", trials=2), flush=True)
+        else:
+            m.set_config("mixed", **shape)
         for _ in range(3):
             m.forces("mixed")
     hip = ctypes.CDLL("libamdhip64.so")
