@@ -46,7 +46,7 @@ EXPORTS = (
     "examd_hip_set_config", "examd_hip_set_graph_chunk",
     "examd_hip_batch_create", "examd_hip_batch_info",
     "examd_hip_generate_vacancy_fcc", "examd_hip_generate_slab",
-    "examd_hip_set_species", "examd_hip_set_pair_table",
+    "examd_hip_set_species", "examd_hip_set_pair_table", "examd_hip_get_cuts",
 )
 
 MAX_SPECIES = 8
@@ -72,14 +72,16 @@ class Tune(C.Structure):
     _fields_ = [("tiles", C.c_int32 * 3), ("block_threads", C.c_int32),
                 ("run_groups", C.c_int32), ("lane_queue", C.c_int32),
                 ("tile_edge", C.c_int32), ("workgroups", C.c_int32),
-                ("balance", C.c_int32), ("reserved", C.c_int32),
+                ("balance", C.c_int32), ("rebalance_every", C.c_int32),
                 ("force_ms", C.c_double)]
 
     def as_dict(self):
         return {"tiles": tuple(self.tiles), "block_threads": self.block_threads,
                 "run_groups": self.run_groups, "lane_queue": self.lane_queue,
                 "tile_edge": self.tile_edge, "workgroups": self.workgroups,
-                "balance": self.balance, "force_ms": self.force_ms}
+                "balance": self.balance,
+                "rebalance_every": self.rebalance_every,
+                "force_ms": self.force_ms}
 
 
 class System(C.Structure):
@@ -380,7 +382,10 @@ class _Handle:
 
     def set_config(self, precision, tiles=(0, 0, 0), block_threads=512,
                    run_groups=0, lane_queue=0, tile_edge=0, workgroups=0,
-                   balance=0):
+                   balance=0, rebalance_every=0):
+        """Apply a launch shape; rebalance_every > 0 makes run() re-partition
+        the map from the current block counts that often (in steps, at
+        captured-chunk boundaries)."""
         t = Tune()
         t.tiles = (C.c_int32 * 3)(*[int(v) for v in tiles])
         t.block_threads = int(block_threads)
@@ -389,8 +394,22 @@ class _Handle:
         t.tile_edge = int(tile_edge)
         t.workgroups = int(workgroups)
         t.balance = int(balance)
+        t.rebalance_every = int(rebalance_every)
         _check(lib().examd_hip_set_config(self._h, _PREC[precision],
                                           C.byref(t)), "examd_hip_set_config")
+
+    def get_cuts(self, precision="mixed", system=0):
+        """Cell-block boundaries of one system's map, in blocks: a tuple of
+        three arrays (x, y, z), each from 0 to the axis's block count."""
+        buf = np.zeros(3 * 1024 + 3, dtype=np.int32)
+        n = C.c_int32(0)
+        _check(lib().examd_hip_get_cuts(
+            self._h, _PREC[precision], int(system),
+            buf.ctypes.data_as(C.POINTER(C.c_int32)), len(buf), C.byref(n)),
+            "examd_hip_get_cuts")
+        c = buf[:n.value].copy()
+        # each axis's boundaries start at 0 and increase
+        return tuple(np.split(c, np.flatnonzero(c == 0)[1:]))
 
     def phase_profile(self, precision="mixed"):
         """Mean cycles per cell block: scan, gather, pairs, return, total."""

@@ -44,6 +44,7 @@ struct TileConfig {
   int maxrb = 0;          // largest region in blocks
   bool fixed_tiles = false;
   int fixed_groups = 0;
+  int rebalance = 0;      // steps between re-partitions in a run (0 = never)
 };
 
 // What a captured chunk depends on.
@@ -96,6 +97,7 @@ struct examd_handle {
   size_t ev_used = 0;
   double force_ms = 0.0;
   int64_t force_launches = 0;
+  int64_t since_repartition[2] = {0, 0};  // steps run since, per precision
 };
 
 namespace {
@@ -304,16 +306,42 @@ inline int host_block(double x, double bl, int nb) {
   return v - 1;
 }
 
+// Atoms per block of every system for host positions, in the block order
+// of the binning kernel (x fastest, systems one after another).
+std::vector<int> host_counts(const examd_handle *h, const double *pos) {
+  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
+  for (const SysHost &S : h->sys) {
+    const Grid &g = S.g;
+    const double *p = pos + 3 * S.atom0;
+    for (int64_t i = 0; i < S.n; ++i) {
+      int b[3];
+      for (int a = 0; a < 3; ++a) b[a] = host_block(p[3 * i + a], g.bl, g.nb[a]);
+      cnt[(size_t)S.block0 + b[0] +
+          (size_t)g.nb[0] * (b[1] + (size_t)g.nb[1] * b[2])]++;
+    }
+  }
+  return cnt;
+}
+
 // Balanced map: along each axis, boundaries at equal shares of the atoms
 // counted per block slab (the weights METIS gets from atom counts,
 // SRC/exaMD_partitioning.f90:199-203, restated for a rectilinear map), each
-// tile at least one block wide.
-Cuts balanced_cuts(const Grid &g, const double *pos, int64_t n) {
+// tile at least one block wide.  cnt: the system's atoms per block.
+Cuts balanced_cuts(const Grid &g, const int *cnt, int64_t n) {
+  std::vector<double> slab[3];
+  for (int a = 0; a < 3; ++a) slab[a].assign((size_t)g.nb[a], 0.0);
+  for (int z = 0, i = 0; z < g.nb[2]; ++z)
+    for (int y = 0; y < g.nb[1]; ++y)
+      for (int x = 0; x < g.nb[0]; ++x, ++i) {
+        const double c = cnt[i];
+        slab[0][x] += c;
+        slab[1][y] += c;
+        slab[2][z] += c;
+      }
   Cuts b;
   for (int a = 0; a < 3; ++a) {
     const int nb = g.nb[a], T = g.T[a];
-    std::vector<double> w((size_t)nb, 0.0);
-    for (int64_t i = 0; i < n; ++i) w[host_block(pos[3 * i + a], g.bl, nb)] += 1.0;
+    std::vector<double> &w = slab[a];
     // a small floor so empty slabs still spread over tiles
     const double floor_w = std::max(1e-3, 0.02 * (double)n / nb);
     double total = 0.0;
@@ -347,16 +375,10 @@ int max_region_blocks(const Cuts &b) {
   return m[0] * m[1] * m[2];
 }
 
-// Largest number of atoms any tile's region holds for host positions.
-int64_t max_region_atoms(const Grid &g, const Cuts &cb, const double *pos,
-                         int64_t n) {
+// Largest number of atoms any tile's region holds.  cnt: the system's
+// atoms per block.
+int64_t max_region_atoms(const Grid &g, const Cuts &cb, const int *cnt) {
   const int nx = g.nb[0], ny = g.nb[1], nz = g.nb[2];
-  std::vector<int> cnt((size_t)nx * ny * nz, 0);
-  for (int64_t i = 0; i < n; ++i) {
-    int b[3];
-    for (int a = 0; a < 3; ++a) b[a] = host_block(pos[3 * i + a], g.bl, g.nb[a]);
-    cnt[(size_t)b[0] + (size_t)nx * (b[1] + (size_t)ny * b[2])]++;
-  }
   // per-axis tile boundaries
   std::vector<int> lo[3], hi[3];
   for (int a = 0; a < 3; ++a)
@@ -391,13 +413,20 @@ Tri tiles_from_edge(const Grid &g, double edge) {
   return T;
 }
 
+// Capacity a region of mra atoms needs at the least: 5% headroom.
+int64_t needed_cap(int64_t mra) {
+  return (int64_t)std::ceil((double)mra * 1.05) + 16;
+}
+
 // Pick the LDS capacity of a cell block for one precision from the largest
-// region of the given positions (over all systems), with cap_slack_pct
+// region of the given block counts (over all systems), with cap_slack_pct
 // headroom for density fluctuations during a run (at least 5%).  When the
 // maps were chosen automatically and a region does not fit the device's
 // LDS, refine the map of the system with the fullest region along its
-// coarsest axis.  Uploads the system table on success.
-int choose_caps(examd_handle *h, int prec, const double *pos) {
+// coarsest axis.  On success uploads the system table (when `upload`) and
+// reports the largest region in *mra_out.
+int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
+                       int64_t *mra_out, bool upload) {
   TileConfig &c = h->tc[prec];
   const bool fp64 = prec == EXAMD_PREC_FP64;
   const bool typed = h->d.ntypes > 0;
@@ -412,7 +441,7 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
     for (int k = 0; k < nsys; ++k) {
       const Grid g = grid_of(h, prec, k);
       const SysHost &S = h->sys[k];
-      if (c.balance) c.cuts[k] = balanced_cuts(g, pos + 3 * S.atom0, S.n);
+      if (c.balance) c.cuts[k] = balanced_cuts(g, cnt + S.block0, S.n);
       const Cuts cb = cuts_of(h, prec, k, g);
       c.maxrb = std::max(c.maxrb, max_region_blocks(cb));
       // region coordinates are packed 8 bits per axis on the device
@@ -420,7 +449,7 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
         for (int t = 0; t < g.T[a]; ++t)
           if (cb[a][t + 1] - cb[a][t] + 2 > 255) ok = false;
       if (!ok) break;
-      const int64_t m = max_region_atoms(g, cb, pos + 3 * S.atom0, S.n);
+      const int64_t m = max_region_atoms(g, cb, cnt + S.block0);
       if (m > mra) {
         mra = m;
         worst = k;
@@ -435,7 +464,7 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
     const double slack =
         (h->p.cap_slack_pct > 0 ? h->p.cap_slack_pct : 15) / 100.0;
     int64_t want = ((int64_t)std::ceil((double)mra * (1.0 + slack)) + 32 + 15) / 16 * 16;
-    int64_t need = (int64_t)std::ceil((double)mra * 1.05) + 16;
+    int64_t need = needed_cap(mra);
     int64_t cap = want;
     if (ok) {
       while (cap > need &&
@@ -455,7 +484,8 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
                    ? c.fixed_groups
                    : (int)std::lround(3.0 * c.threads / std::max(reg, 1.0));
       c.groups = std::max(1, std::min(4, ng));
-      return upload_tables(h, prec);
+      if (mra_out) *mra_out = mra;
+      return upload ? upload_tables(h, prec) : EXAMD_OK;
     }
     if (c.fixed_tiles) return EXAMD_ERR_TILE_OVERFLOW;
     Tri &T = c.T[worst];
@@ -473,6 +503,44 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
     T[ax] += 1;
   }
   return EXAMD_ERR_TILE_OVERFLOW;
+}
+
+int choose_caps(examd_handle *h, int prec, const double *pos) {
+  const std::vector<int> cnt = host_counts(h, pos);
+  return choose_caps_counts(h, prec, cnt.data(), nullptr, true);
+}
+
+// Re-partition the map of one precision from the block counts of the last
+// binning pass (start[] holds their exclusive scan): new balanced
+// boundaries (balanced maps) and a new LDS capacity.  The old map stays when
+// the new one would not fit, and when the new one has the same boundaries
+// and the old capacity still holds the fullest region with 5% to spare, so
+// that captured chunks are not captured again for nothing.  *changed: the
+// map was replaced.
+int repartition(examd_handle *h, int prec, bool *changed) {
+  *changed = false;
+  std::vector<int> start((size_t)h->nblocks + 1);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(start.data(), h->d.start, (h->nblocks + 1) * sizeof(int),
+                   hipMemcpyDeviceToHost));
+  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
+  for (int64_t b = 0; b < h->nblocks; ++b) cnt[b] = start[b + 1] - start[b];
+  const TileConfig saved = h->tc[prec];
+  TileConfig &c = h->tc[prec];
+  int64_t mra = 0;
+  if (choose_caps_counts(h, prec, cnt.data(), &mra, false) != EXAMD_OK) {
+    c = saved;
+    return EXAMD_OK;
+  }
+  if (c.T == saved.T && c.cuts == saved.cuts && c.maxrb == saved.maxrb &&
+      c.groups == saved.groups && saved.cap >= needed_cap(mra)) {
+    c = saved;
+    return EXAMD_OK;
+  }
+  int st = upload_tables(h, prec);
+  if (st == EXAMD_OK) st = ensure_partials(h);
+  *changed = st == EXAMD_OK;
+  return st;
 }
 
 ForceConfig force_config(const examd_handle *h, int prec) {
@@ -984,6 +1052,7 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
   }
   h->state_set = true;
   h->forces_valid = -1;
+  h->since_repartition[0] = h->since_repartition[1] = 0;
   return EXAMD_OK;
 }
 
@@ -1249,12 +1318,15 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
   // log[0..ns): flags of the evaluation at the starting positions, if one
   // runs
   launch_fill_log(h->log, ns, h->stream);
-  if (h->forces_valid != precision) {
-    const bool t = h->timing;   // only the step launches are timed
+  // forces at the current positions under the current map; only the step
+  // launches are timed
+  auto initial_forces = [&]() {
+    const bool t = h->timing;
     h->timing = false;
     evaluate(h, precision, true, false, h->log);
     h->timing = t;
-  }
+  };
+  if (h->forces_valid != precision) initial_forces();
   unsigned long long fl = 0;
   int64_t k = 0;
   std::vector<StepLog> L;
@@ -1278,25 +1350,52 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
     return EXAMD_OK;
   };
   const unsigned long long fatal = kFlagOverflow | kFlagOutOfBox | kFlagNonFinite;
-  if (nchunks > 0) {
+  // periodic re-partition of the map, between chunks; a new map
+  // re-evaluates the forces, so the run continues exactly as one that
+  // stopped here, applied the map and started again
+  const int64_t R = h->tc[precision].rebalance;
+  int64_t &since = h->since_repartition[precision];
+  auto after_steps = [&](int64_t count) -> int {
+    since += count;
+    if (R <= 0 || since < R || (fl & fatal)) return EXAMD_OK;
+    since = 0;
+    bool changed = false;
+    const int st = repartition(h, precision, &changed);
+    if (st == EXAMD_OK && changed) {
+      launch_fill_log(h->log, ns, h->stream);
+      initial_forces();
+      std::vector<StepLog> L0((size_t)ns);
+      HIPCHK(hipMemcpyAsync(L0.data(), h->log, ns * sizeof(StepLog),
+                            hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      for (const StepLog &q : L0) fl |= q.flags;
+    }
+    return st;
+  };
+  for (int64_t c = 0; c < nchunks; ++c) {
+    // re-captured only when a re-partition changed the map
     GraphKey key{precision, C, energy_every > 0 ? energy_every : 0, dt,
                  h->timing, h->tc[precision]};
     int st = ensure_graph(h, key);
     if (st) return st;
-    for (int64_t c = 0; c < nchunks; ++c) {
-      HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
-      h->ev_used = h->graph_events;   // the events the replay recorded
-      st = harvest(c * C, C);
-      if (st) return st;
-      if (fl & fatal) break;
-    }
-  }
-  if (rem > 0 && !(fl & fatal)) {
-    enqueue_steps(h, precision, rem, dt,
-                  [&](int64_t c) { return is_energy(done_by_graph + c); });
-    HIPCHK(hipGetLastError());
-    int st = harvest(done_by_graph, rem);
+    HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
+    h->ev_used = h->graph_events;   // the events the replay recorded
+    st = harvest(c * C, C);
     if (st) return st;
+    if (fl & fatal) break;
+    if ((st = after_steps(C)) != EXAMD_OK) return st;
+  }
+  // the rest eagerly, in pieces that end where a re-partition is due
+  for (int64_t base = done_by_graph; base < nsteps && !(fl & fatal);) {
+    int64_t count = nsteps - base;
+    if (R > 0) count = std::min(count, std::max<int64_t>(1, R - since));
+    enqueue_steps(h, precision, count, dt,
+                  [&](int64_t c) { return is_energy(base + c); });
+    HIPCHK(hipGetLastError());
+    int st = harvest(base, count);
+    if (st) return st;
+    base += count;
+    if ((st = after_steps(count)) != EXAMD_OK) return st;
   }
   h->forces_valid = precision;
   if (n_out) *n_out = k;
@@ -1384,7 +1483,7 @@ int examd_hip_get_config(examd_handle *h, int precision, examd_tune *out) {
   out->tile_edge = c.edge;
   out->workgroups = c.workgroups;
   out->balance = c.balance ? 1 : 0;
-  out->reserved = 0;
+  out->rebalance_every = c.rebalance;
   out->force_ms = 0.0;
   return EXAMD_OK;
 }
@@ -1399,7 +1498,8 @@ int examd_hip_set_config(examd_handle *h, int precision,
   if (thr != 256 && thr != 512 && thr != 1024) return EXAMD_ERR_ARG;
   if (cfg->run_groups < 0 || cfg->run_groups > 4) return EXAMD_ERR_ARG;
   if (cfg->lane_queue < 0 || cfg->lane_queue > 16) return EXAMD_ERR_ARG;
-  if (cfg->workgroups < 0 || cfg->balance < 0 || cfg->balance > 1)
+  if (cfg->workgroups < 0 || cfg->balance < 0 || cfg->balance > 1 ||
+      cfg->rebalance_every < 0)
     return EXAMD_ERR_ARG;
   const bool explicit_tiles =
       cfg->tiles[0] > 0 && cfg->tiles[1] > 0 && cfg->tiles[2] > 0;
@@ -1413,10 +1513,30 @@ int examd_hip_set_config(examd_handle *h, int precision,
   std::vector<double> pos;
   int st = download_positions(h, pos);
   if (st) return st;
-  return apply_config(h, precision, explicit_tiles ? cfg->tiles : nullptr,
-                      (double)cfg->tile_edge, thr, cfg->run_groups,
-                      cfg->lane_queue, cfg->workgroups, cfg->balance != 0,
-                      pos.data());
+  st = apply_config(h, precision, explicit_tiles ? cfg->tiles : nullptr,
+                    (double)cfg->tile_edge, thr, cfg->run_groups,
+                    cfg->lane_queue, cfg->workgroups, cfg->balance != 0,
+                    pos.data());
+  if (st == EXAMD_OK) {
+    h->tc[precision].rebalance = cfg->rebalance_every;
+    h->since_repartition[precision] = 0;
+  }
+  return st;
+}
+
+int examd_hip_get_cuts(examd_handle *h, int precision, int32_t system,
+                       int32_t *cuts, int32_t cap, int32_t *n_out) {
+  if (!h || !cuts || !n_out || system < 0 || system >= h->nsys ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  const Cuts cb = cuts_of(h, precision, system, grid_of(h, precision, system));
+  const int32_t n = (int32_t)(cb[0].size() + cb[1].size() + cb[2].size());
+  *n_out = n;
+  if (cap < n) return EXAMD_ERR_ARG;
+  int32_t k = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int v : cb[a]) cuts[k++] = v;
+  return EXAMD_OK;
 }
 
 int examd_hip_autotune(examd_handle *h, int precision, int trials,

@@ -43,7 +43,7 @@ Module exaMD_hip_binding
      Integer(c_int32_t) :: tile_edge
      Integer(c_int32_t) :: workgroups
      Integer(c_int32_t) :: balance
-     Integer(c_int32_t) :: reserved
+     Integer(c_int32_t) :: rebalance_every
      Real(c_double)     :: force_ms
   End Type examd_tune
 
@@ -209,6 +209,17 @@ Module exaMD_hip_binding
        Integer(c_int), Value :: precision
        Type(examd_tune), Intent(in) :: cfg
      End Function examd_hip_set_config
+
+     Integer(c_int) Function examd_hip_get_cuts(handle, precision, system, &
+          cuts, cap, n_out) Bind(C, name="examd_hip_get_cuts")
+       Import :: c_int, c_ptr, c_int32_t
+       Type(c_ptr), Value :: handle
+       Integer(c_int), Value :: precision
+       Integer(c_int32_t), Value :: system
+       Integer(c_int32_t), Intent(out) :: cuts(*)
+       Integer(c_int32_t), Value :: cap
+       Integer(c_int32_t), Intent(out) :: n_out
+     End Function examd_hip_get_cuts
 
      Integer(c_int) Function examd_hip_batch_create(common, systems, nsys, &
           handle) Bind(C, name="examd_hip_batch_create")

@@ -400,6 +400,11 @@ Module exaMD_forces_mod
   !! (exaMD_table_rmin * cutoff)^2 to cutoff^2 and interpolated on the GPU.
   Integer :: exaMD_table_points = 4096
   Real(kind=PRC) :: exaMD_table_rmin = 0.2_PRC
+  !! exaMD_md_run with exaMD_rebalance_every > 0: the cell blocks get
+  !! boundaries at equal shares of the atoms (the atom-count weights of
+  !! SRC/exaMD_partitioning.f90:199-203), re-partitioned from the current
+  !! positions at least every this many steps (DOC/exaMD.tex:284-291).
+  Integer :: exaMD_rebalance_every = 0
 
   Type(c_ptr), Private :: handle = c_null_ptr
   Integer(c_int64_t), Private :: handle_n = -1
@@ -656,11 +661,20 @@ Contains
     Real(c_double), Allocatable :: pos(:,:)
     Integer(c_int64_t) :: n, launches
     Integer(c_int) :: st
+    Type(examd_tune) :: cfg
     Call gather_positions(pos, n)
     Call ensure_handle(n, cutoff)
     Call set_device_species(n)
     st = examd_hip_set_state(handle, pos, c_loc(vel), n)
     Call check_status(st, 'examd_hip_set_state')
+    If (exaMD_rebalance_every > 0) Then
+       st = examd_hip_get_config(handle, exaMD_precision, cfg)
+       Call check_status(st, 'examd_hip_get_config')
+       cfg%balance = 1
+       cfg%rebalance_every = Int(exaMD_rebalance_every, c_int32_t)
+       st = examd_hip_set_config(handle, exaMD_precision, cfg)
+       Call check_status(st, 'examd_hip_set_config')
+    End If
     st = examd_hip_timing(handle, 1_c_int, force_ms, launches, 1_c_int)
     st = examd_hip_run(handle, exaMD_precision, nsteps, Real(dt, c_double), &
          energy_every, stats, Int(Size(stats), c_int64_t), nstats)

@@ -15,6 +15,8 @@
 !!   -seed s         generator seed (default 1)
 !!   -restart file   start from a checkpoint (positions, velocities, box, step)
 !!   -checkpoint file write the state after the MD steps as a checkpoint
+!!   -rebalance K    cell blocks at equal shares of the atoms, re-partitioned
+!!                   from the current positions at least every K MD steps
 !!   -morse          the user function is a Morse pair (De 0.8, a 2.0,
 !!                   r0 1.1), which the library tabulates for the GPU
 !!   -binary         two species (Kob-Andersen): every fifth atom in input
@@ -123,6 +125,10 @@ Program examd_md
      Case ('-checkpoint')
         ia = ia + 1
         Call get_command_argument(ia, checkpoint_file)
+     Case ('-rebalance')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) exaMD_rebalance_every
      End Select
      ia = ia + 1
   End Do

@@ -239,7 +239,15 @@ typedef struct examd_tune {
                              shares of the atoms (a load-balanced map for
                              uneven density, set from the current positions);
                              0: even split of the blocks */
-    int32_t reserved;
+    int32_t rebalance_every; /* examd_hip_run re-partitions the map of this
+                             precision from the current block counts once at
+                             least this many steps have run since the last
+                             time, at the next captured-chunk boundary (every
+                             this many steps when no chunks are captured):
+                             new balanced boundaries (balance = 1) and a new
+                             LDS capacity, the old map kept when the new one
+                             would not fit; 0 = never.  The autotuner keeps
+                             the value it finds. */
     double force_ms;      /* mean force-kernel time measured by autotune */
 } examd_tune;
 
@@ -254,6 +262,15 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
 int examd_hip_get_config(examd_handle *h, int precision, examd_tune *out);
 int examd_hip_set_config(examd_handle *h, int precision,
                          const examd_tune *cfg);
+
+/* Cell-block boundaries of one system's map in one precision, in blocks:
+ * Tx + 1 values along x, then Ty + 1 along y, then Tz + 1 along z (first 0,
+ * last nblocks_xyz of the axis), written to cuts[0 .. cap).  *n_out = the
+ * number of values; EXAMD_ERR_ARG when cap is smaller.  The boundaries of
+ * METIS's partition of the blocks over ranks in the reference
+ * (SRC/exaMD_partitioning.f90:65-455) restated as a rectilinear map. */
+int examd_hip_get_cuts(examd_handle *h, int precision, int32_t system,
+                       int32_t *cuts, int32_t cap, int32_t *n_out);
 
 /* Force-kernel timing over the calls since the last reset: enable, read
  * (total milliseconds of force-kernel launches measured with HIP events on

@@ -139,3 +139,27 @@ def test_driver_tabulates_a_general_user_function(tmp_path, oracle):
     _, t, _ = oracle.forces(pos, [L, L, L], 2.5, table=(r2[0], g, v))
     assert (int(m.group(1)), int(m.group(2))) == (t.cand, t.cut)
     assert abs(pe - t.pe) < 1e-6 * abs(t.pe)
+
+
+@pytest.mark.gpu
+def test_driver_rebalance(tmp_path):
+    """-rebalance 25: exaMD_md_run applies the load-balanced map and
+    re-partitions it every 25 steps of a 100-step mixed run; the run keeps
+    its energy and ends within rounding-driven divergence of the run on the
+    default map."""
+    drv = _driver()
+    common = ["-atoms", "4000", "-precision", "mixed", "-steps", "100"]
+    states = []
+    for extra, chk in ((["-rebalance", "25"], "a.chk"), ([], "b.chk")):
+        r = subprocess.run([drv, *common, *extra, "-checkpoint", chk],
+                           cwd=tmp_path, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+        drift = float(re.search(r"Relative shifted-energy change:\s*(\S+)",
+                                r.stdout).group(1))
+        assert abs(drift) < 1e-3
+        states.append(np.loadtxt(tmp_path / chk, skiprows=2))
+    d = states[0][:, :3] - states[1][:, :3]
+    L = (4000 / 0.8442) ** (1 / 3)
+    d -= np.round(d / L) * L
+    assert np.abs(d).max() < 1e-4
