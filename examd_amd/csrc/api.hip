@@ -910,7 +910,6 @@ int create_handle(const examd_params *prm, const examd_system *systems,
   DevState &d = h->d;
   d.n = h->n;
   const int64_t n = h->n;
-  const int64_t nchunk = (blocks + 2047) / 2048;
   int rc = EXAMD_OK;
   rc |= dalloc(&d.x, n) | dalloc(&d.y, n) | dalloc(&d.z, n);
   rc |= dalloc(&d.vx, n) | dalloc(&d.vy, n) | dalloc(&d.vz, n);
@@ -918,7 +917,7 @@ int create_handle(const examd_params *prm, const examd_system *systems,
   rc |= dalloc(&d.qx, n) | dalloc(&d.qy, n) | dalloc(&d.qz, n);
   rc |= dalloc(&d.cell, n) | dalloc(&d.rank, n) | dalloc(&d.sorted, n);
   rc |= dalloc(&d.count, blocks) | dalloc(&d.start, blocks + 1);
-  rc |= dalloc(&d.chunk, nchunk);
+  rc |= dalloc(&d.chunk, scan_chunk_ints(blocks));
   rc |= dalloc(&h->atom_seg, nsys + 1);
   if (rc == EXAMD_OK) rc = ensure_log(h, 1);
   if (rc != EXAMD_OK) {

@@ -281,6 +281,7 @@ __global__ __launch_bounds__(kBinThreads) void k_sys_ke(DevState d,
 constexpr int kScanThreads = 256;
 constexpr int kScanPer = 8;
 constexpr int kScanChunk = kScanThreads * kScanPer;  // 2048 counts
+constexpr int kScanDirect = 2048;  // chunks whose totals k_scan_down sums itself
 
 // Inclusive scan of one int per thread across the workgroup; returns the
 // inclusive prefix and writes the workgroup total to *total.
@@ -310,50 +311,110 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int *s_wave,
   return r;
 }
 
+// kScanPer consecutive counts of one thread: two 16-byte loads when the
+// segment lies inside the array (count is 256-byte aligned, segments start
+// at multiples of 8), else one at a time.
+__device__ __forceinline__ void load_counts(const int *count, int64_t base,
+                                            int64_t n, int (&c)[kScanPer]) {
+  if (base + kScanPer <= n) {
+    const int4 lo = *reinterpret_cast<const int4 *>(count + base);
+    const int4 hi = *reinterpret_cast<const int4 *>(count + base + 4);
+    c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w;
+    c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) c[k] = base + k < n ? count[base + k] : 0;
+  }
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(
     const int *count, int64_t n, int *chunk) {
-  __shared__ int s_wave[kScanThreads / 64 + 1];
-  int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  __shared__ int s_wave[kScanThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  int c[kScanPer];
+  load_counts(count, base, n, c);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanPer; ++k)
-    if (base + k < n) s += count[base + k];
-  int total;
-  block_inclusive_scan<kScanThreads>(s, s_wave, &total);
-  if (threadIdx.x == 0) chunk[blockIdx.x] = total;
+  for (int k = 0; k < kScanPer; ++k) s += c[k];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) t += s_wave[w];
+    chunk[blockIdx.x] = t;
+  }
 }
 
 // Exclusive scan of the block counts: k_scan_reduce writes each chunk's
-// total, k_scan_down adds the totals of the chunks before its own (a few
-// hundred cached reads per workgroup) and scans inside the chunk.
+// total; k_scan_down takes the offset of its chunk from chunk_off when
+// given (the scanned chunk totals of a large array), else adds the totals
+// of the chunks before its own (at most kScanDirect cached reads per
+// workgroup, summed in the same pass as the scan inside the chunk).
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(
-    int *count, int64_t n, const int *chunk, int *start, int clear) {
-  __shared__ int s_wave[kScanThreads / 64 + 1];
-  int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+    int *count, int64_t n, const int *chunk, const int *chunk_off, int *start,
+    int clear) {
+  __shared__ int s_wave[kScanThreads / 64];
+  __shared__ int s_pre[kScanThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  const bool full = base + kScanPer <= n;
   int c[kScanPer];
+  load_counts(count, base, n, c);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    c[k] = base + k < n ? count[base + k] : 0;
-    s += c[k];
-  }
+  for (int k = 0; k < kScanPer; ++k) s += c[k];
   if (clear) {
+    if (full) {
+      *reinterpret_cast<int4 *>(count + base) = make_int4(0, 0, 0, 0);
+      *reinterpret_cast<int4 *>(count + base + 4) = make_int4(0, 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanPer; ++k)
+        if (base + k < n) count[base + k] = 0;
+    }
+  }
+  // this thread's share of the totals of the chunks before this one
+  int pre = 0;
+  if (chunk_off) {
+    if (threadIdx.x == 0) pre = chunk_off[blockIdx.x];
+  } else {
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) pre += chunk[b];
+  }
+  // wave: inclusive scan of s, sum of pre
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
+  if (lane == 63) s_wave[wave] = inc;
+  if (lane == 0) s_pre[wave] = pre;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) acc += s_pre[w];
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+      const int t = s_wave[w];
+      s_wave[w] = acc;
+      acc += t;
+    }
+  }
+  __syncthreads();
+  int run = s_wave[wave] + inc - s;
+  int out[kScanPer];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    out[k] = run;
+    run += c[k];
+  }
+  if (full) {
+    *reinterpret_cast<int4 *>(start + base) = make_int4(out[0], out[1], out[2], out[3]);
+    *reinterpret_cast<int4 *>(start + base + 4) = make_int4(out[4], out[5], out[6], out[7]);
+  } else {
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k)
-      if (base + k < n) count[base + k] = 0;
-  }
-  // offset of this chunk: the totals of the chunks before it
-  int pre = 0;
-  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) pre += chunk[b];
-  int offset;
-  block_inclusive_scan<kScanThreads>(pre, s_wave, &offset);
-  int total;
-  int inc = block_inclusive_scan<kScanThreads>(s, s_wave, &total);
-  int run = offset + inc - s;
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    if (base + k < n) start[base + k] = run;
-    run += c[k];
+      if (base + k < n) start[base + k] = out[k];
   }
   if (base + kScanPer >= n && base < n) start[n] = run;
 }
@@ -1325,12 +1386,32 @@ void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
   if (t.nsys > 0) k_sys_ke<<<t.nsys, kBinThreads, 0, s>>>(d, t, log);
 }
 
+int64_t scan_chunk_ints(int64_t nblocks) {
+  const int64_t nchunk = (nblocks + kScanChunk - 1) / kScanChunk;
+  const int64_t n2 = (nchunk + kScanChunk - 1) / kScanChunk;
+  return 2 * ((nchunk + 1 + 7) / 8 * 8) + n2 + 8;
+}
+
 void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
                  hipStream_t s) {
-  int nchunk = (int)((nblocks + kScanChunk - 1) / kScanChunk);
+  const int nchunk = (int)((nblocks + kScanChunk - 1) / kScanChunk);
   k_scan_reduce<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk);
+  // beyond kScanDirect chunks the chunk totals are scanned first (chunk
+  // totals, then their offsets, then the second-level totals, each part
+  // 32-byte aligned for the vector loads)
+  const int *chunk_off = nullptr;
+  if (nchunk > kScanDirect) {
+    const int stride = (nchunk + 1 + 7) / 8 * 8;
+    int *off = d.chunk + stride, *tot2 = d.chunk + 2 * stride;
+    const int n2 = (nchunk + kScanChunk - 1) / kScanChunk;
+    k_scan_reduce<<<n2, kScanThreads, 0, s>>>(d.chunk, nchunk, tot2);
+    k_scan_down<<<n2, kScanThreads, 0, s>>>(d.chunk, nchunk, tot2, nullptr,
+                                            off, 0);
+    chunk_off = off;
+  }
   k_scan_down<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk,
-                                              d.start, block_order ? 0 : 1);
+                                              chunk_off, d.start,
+                                              block_order ? 0 : 1);
   if (d.n > 0)
     k_scatter<<<nblk(d.n, kBinThreads * kBinPer), kBinThreads, 0, s>>>(d);
   if (block_order)

@@ -95,6 +95,8 @@ void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
 // Kinetic energy of every system into log[sys].ke (sum of v^2).
 void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
                    hipStream_t s);
+// Ints of DevState::chunk the block-count scan needs for nblocks blocks.
+int64_t scan_chunk_ints(int64_t nblocks);
 // Counting sort of the atoms by block (scan of the counts, which it clears,
 // and scatter); block_order puts each block's atoms in input order.
 void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
