@@ -36,6 +36,11 @@ NC = 80                       # FCC cells per side -> 2,048,000 atoms
 RHO, TEMP, RC, DT = 0.8442, 0.72, 2.5, 0.005
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles",
                                "k_forces_mixed_2048000_traffic.txt")
+# The force kernel is timed with HIP events around the force launch of every
+# TIME_EVERY-th step (steps 5, 15, 25, ... of each chunk, none of them an
+# energy-tally step): each pair of event records costs the stream about
+# 5.6 us, which timing every launch would add to every step.
+TIME_EVERY = 10
 
 
 def algorithmic_flops(cand, cut, energy):
@@ -120,14 +125,14 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
             tune = m.get_config(prec)
         if warmup:
             m.run(warmup, DT, prec, energy_every=0)
-        m.timing(enable=True, reset=True)
+        m.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         stats = m.run(steps, DT, prec, energy_every=energy_every)
         t1 = time.perf_counter()
         force_ms, launches = m.timing(enable=False, reset=True)
         if report_error:
             err = force_error(m)
-    if launches != steps:
+    if launches < steps // TIME_EVERY:
         raise RuntimeError(f"{prec}: {launches} timed force launches for "
                            f"{steps} steps")
     return t1 - t0, stats, force_ms / max(launches, 1), launches, tune, err
@@ -166,7 +171,7 @@ def batch_bench(E, steps, warmup, loop_steps, nsys=256):
         tune = b.autotune("mixed", trials=2)
         if warmup:
             b.run(warmup, DT, "mixed")
-        b.timing(enable=True, reset=True)
+        b.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         rows = b.run(steps, DT, "mixed", energy_every=100)
         wall = time.perf_counter() - t0
@@ -230,7 +235,7 @@ def slab_bench(E, steps, warmup):
         tune = m.autotune("mixed", trials=2)
         if warmup:
             m.run(warmup, DT, "mixed")
-        m.timing(enable=True, reset=True)
+        m.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         st = m.run(steps, DT, "mixed", energy_every=100)
         wall = time.perf_counter() - t0
@@ -270,7 +275,7 @@ def species_bench(E, steps, warmup, tune_after=300):
         tune = m.autotune("mixed", trials=2)
         if warmup:
             m.run(warmup, dt, "mixed", energy_every=0)
-        m.timing(enable=True, reset=True)
+        m.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         st = m.run(steps, dt, "mixed", energy_every=100)
         wall = time.perf_counter() - t0
@@ -314,7 +319,7 @@ def table_bench(E, steps, warmup, tune_after=300, npoints=4096, r_lo=0.6):
         tune = m.autotune("mixed", trials=2)
         if warmup:
             m.run(warmup, DT, "mixed", energy_every=0)
-        m.timing(enable=True, reset=True)
+        m.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         st = m.run(steps, DT, "mixed", energy_every=100)
         wall = time.perf_counter() - t0
@@ -463,14 +468,13 @@ def main():
 
     rate_mx = n * steps / wall_mx
     rate_64 = n * fsteps / wall_64
-    # algorithmic work of one force launch: mean P_cand, P_cut over the
-    # energy steps tallied; about 1 in energy_every launches also tallies
-    # the energy (+3 P_cut)
+    # algorithmic work of one timed force launch: mean P_cand, P_cut over
+    # the energy steps tallied; the timed launches tally no energy
     cand = float(np.mean([s.pairs_cand for s in st_mx]))
     cut = float(np.mean([s.pairs_cut for s in st_mx]))
-    w = algorithmic_flops(cand, cut, False) + 3.0 * cut / energy_every
+    w = algorithmic_flops(cand, cut, False)
     achieved = w / (force_ms_mx * 1e-3) / 1e12
-    w64 = algorithmic_flops(cand, cut, False) + 3.0 * cut / energy_every
+    w64 = algorithmic_flops(cand, cut, False)
     achieved64 = w64 / (force_ms_64 * 1e-3) / 1e12
 
     batch = None

@@ -419,11 +419,13 @@ class _Handle:
                "examd_hip_phase_profile")
         return out
 
-    def timing(self, enable=True, reset=False):
+    def timing(self, enable=True, reset=False, every=1):
+        """Force-kernel time from HIP events: (total ms, launches) so far.
+        every=k > 1 times only every k-th step's force launch in run()."""
         ms = C.c_double()
         n = C.c_int64()
-        _check(lib().examd_hip_timing(self._h, int(enable), C.byref(ms),
-                                      C.byref(n), int(reset)),
+        _check(lib().examd_hip_timing(self._h, int(every) if enable else 0,
+                                      C.byref(ms), C.byref(n), int(reset)),
                "examd_hip_timing")
         return ms.value, n.value
 

@@ -53,6 +53,7 @@ struct GraphKey {
   int64_t chunk = 0, every = 0;
   double dt = 0.0;
   bool timing = false;
+  int timing_every = 1;
   TileConfig cfg;
 };
 
@@ -93,6 +94,7 @@ struct examd_handle {
   int forces_valid = -1;          // precision whose forces match the positions
   // timing
   bool timing = false;
+  int timing_every = 1;           // a run times every this many steps' force
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double force_ms = 0.0;
@@ -592,9 +594,9 @@ void record_event(examd_handle *h, hipEvent_t e) {
 // Force launch (tallies of system k into log[k]), bracketed by events when
 // timing is on.
 void forces_timed(examd_handle *h, int prec, bool energy, bool homecount,
-                  StepLog *log) {
+                  StepLog *log, bool sample = true) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (h->timing) {
+  if (h->timing && sample) {
     e0 = next_event(h);
     e1 = next_event(h);
     if (e0) record_event(h, e0);
@@ -665,7 +667,10 @@ void enqueue_steps(examd_handle *h, int prec, int64_t count, double dt,
                       h->stream);
     }
     launch_sort(h->d, h->nblocks, !mixed, h->stream);
-    forces_timed(h, prec, is_energy(c), false, Lc);
+    // with timing on, every timing_every-th step's force launch is
+    // bracketed by events (mid-stride, away from the tally steps)
+    const int k = h->timing_every;
+    forces_timed(h, prec, is_energy(c), false, Lc, c % k == (k / 2) % k);
   }
   if (count > 0) {
     StepLog *Ln = h->log + count * ns;
@@ -680,7 +685,8 @@ void enqueue_steps(examd_handle *h, int prec, int64_t count, double dt,
 
 bool same_key(const GraphKey &a, const GraphKey &b) {
   if (a.prec != b.prec || a.chunk != b.chunk || a.every != b.every ||
-      a.dt != b.dt || a.timing != b.timing)
+      a.dt != b.dt || a.timing != b.timing ||
+      a.timing_every != b.timing_every)
     return false;
   const TileConfig &x = a.cfg, &y = b.cfg;
   return x.T == y.T && x.threads == y.threads && x.groups == y.groups &&
@@ -1374,7 +1380,7 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
   for (int64_t c = 0; c < nchunks; ++c) {
     // re-captured only when a re-partition changed the map
     GraphKey key{precision, C, energy_every > 0 ? energy_every : 0, dt,
-                 h->timing, h->tc[precision]};
+                 h->timing, h->timing_every, h->tc[precision]};
     int st = ensure_graph(h, key);
     if (st) return st;
     HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
@@ -1715,7 +1721,9 @@ int examd_hip_timing(examd_handle *h, int enable, double *force_ms,
     h->force_ms = 0.0;
     h->force_launches = 0;
   }
+  if (enable < 0) return EXAMD_ERR_ARG;
   h->timing = enable != 0;
+  h->timing_every = std::max(1, enable);
   return EXAMD_OK;
 }
 

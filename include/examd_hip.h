@@ -274,7 +274,10 @@ int examd_hip_get_cuts(examd_handle *h, int precision, int32_t system,
 
 /* Force-kernel timing over the calls since the last reset: enable, read
  * (total milliseconds of force-kernel launches measured with HIP events on
- * the handle's stream, and the number of launches), reset. */
+ * the handle's stream, and the number of launches), reset.  enable: 0 off,
+ * 1 every force launch, k > 1 in examd_hip_run only the force launch of every
+ * k-th step (steps k/2, k/2 + k, ... of each captured chunk or eager piece),
+ * since each pair of event records costs the stream a few microseconds. */
 int examd_hip_timing(examd_handle *h, int enable, double *force_ms,
                      int64_t *force_launches, int reset);
 

@@ -287,6 +287,24 @@ def test_graph_chunk_timing_counts_launches(E):
     assert ms > 0.0
 
 
+def test_sampled_timing_counts_launches(E):
+    """Timing every 10th step: steps 5, 15, ... of each captured chunk or
+    eager piece; the sampled steps' trajectory is the untimed one, bit for
+    bit (the events only bracket the launches)."""
+    pos, vel, box = E.generate_fcc(10, RHO, 0.72, seed=23)
+    out = []
+    for every in (0, 10):
+        with E.MidpointLJ(len(pos), box, 2.5, graph_chunk=20) as m:
+            m.set_state(pos, vel)
+            m.timing(enable=every > 0, reset=True, every=max(every, 1))
+            m.run(65, 0.005, "mixed", energy_every=20)  # 3 chunks + 5 eager
+            ms, launches = m.timing(enable=False, reset=True)
+            out.append((m.get_state(), launches, ms))
+    (s0, n0, _), (s1, n1, ms1) = out
+    assert n0 == 0 and n1 == 3 * 2 + 1 and ms1 > 0.0
+    assert np.array_equal(s0[0], s1[0]) and np.array_equal(s0[1], s1[1])
+
+
 def test_checkpoint_restart_continues_exactly(E, tmp_path):
     """Mixed precision: a run continued from a checkpoint in a new handle
     matches the uninterrupted run bit for bit."""
