@@ -382,10 +382,14 @@ class _Handle:
 
     def set_config(self, precision, tiles=(0, 0, 0), block_threads=512,
                    run_groups=0, lane_queue=0, tile_edge=0, workgroups=0,
-                   balance=0, rebalance_every=0):
+                   balance=0, rebalance_every=0, force_ms=None):
         """Apply a launch shape; rebalance_every > 0 makes run() re-partition
         the map from the current block counts that often (in steps, at
-        captured-chunk boundaries)."""
+        captured-chunk boundaries).  Takes the dict get_config() and
+        autotune() return: force_ms is ignored, and a map that came from a
+        tile edge is derived from the edge again."""
+        if int(tile_edge) > 0:
+            tiles = (0, 0, 0)
         t = Tune()
         t.tiles = (C.c_int32 * 3)(*[int(v) for v in tiles])
         t.block_threads = int(block_threads)

@@ -170,6 +170,17 @@ def test_batch_autotune(E, oracle):
         assert best["tile_edge"] > 0 and best["force_ms"] > 0
         f, _, _ = b.forces("mixed")
         off = b.offsets
+        # the tuned shape goes back in as it came out
+        cuts = [b.get_cuts("mixed", k) for k in range(len(sy))]
+        b.set_config("mixed", **best)
+        again = b.get_config("mixed")
+        assert {k: v for k, v in again.items() if k != "force_ms"} == \
+            {k: v for k, v in best.items() if k != "force_ms"}
+        for k in range(len(sy)):
+            for a in range(3):
+                np.testing.assert_array_equal(b.get_cuts("mixed", k)[a], cuts[k][a])
+        f2, _, _ = b.forces("mixed")
+    assert np.array_equal(f, f2)
     pos, _, box = sy[3]
     ref, _, _ = oracle.forces(pos, box, 2.5)
     assert _rel(f[off[3]:off[4]], ref) < 2e-5
