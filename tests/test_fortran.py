@@ -32,6 +32,51 @@ def test_bindings_cover_the_header():
     assert names == bound
 
 
+CALLER = """
+Program caller
+  Use exaMD_mod
+  Use exaMD_forces_mod
+  Use exaMD_utils_mod
+  Use exaMD_partitioning
+  Use exaMD_comms
+  Use exaMD_comms_overlap
+  Use exaMD_input_formats
+  Use user_func_mod
+  Implicit None
+  Integer, Allocatable :: arrived(:), justin(:), sendlist(:)
+  Integer :: n
+  Allocate(arrived(1), justin(1), sendlist(1))
+  Call exaMD_start_comms_posn()
+  Call exaMD_recv_comms_posn(1)
+  Call exaMD_wait_comms_posn(1)
+  Call exaMD_start_comms_forces(1)
+  Call exaMD_wait_comms_forces(1)
+  n = exaMD_testsome_comms_posn(arrived, justin, 1)
+  Call exaMD_recvsome_comms_posn(justin, n, 1)
+  Call exaMD_waitrest_comms_posn(arrived, justin, 1)
+  Call exaMD_start_comms_forcessome(0, sendlist, 1)
+End Program caller
+"""
+
+
+def test_reference_module_surface_compiles(tmp_path):
+    """A caller that uses every module of the reference's force path
+    (exaMD_mod, exaMD_forces_mod, exaMD_utils_mod, exaMD_partitioning,
+    exaMD_comms, exaMD_comms_overlap, exaMD_input_formats, user_func_mod)
+    and the halo routines with the reference's argument lists compiles
+    against the drop-in modules."""
+    if not os.path.exists(FLANG):
+        pytest.skip("amdflang absent")
+    _driver()   # builds the modules when needed
+    src = tmp_path / "caller.f90"
+    src.write_text(CALLER)
+    mods = os.path.join(ROOT, "examd_amd", "fortran", "_build")
+    r = subprocess.run([FLANG, "-c", str(src), "-I", mods, "-module-dir",
+                        str(tmp_path), "-o", str(tmp_path / "caller.o")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 def test_driver_reports_missing_device(tmp_path):
     """Without a GPU the driver sets up the grid, then stops with the
     status of examd_hip_create instead of crashing."""
