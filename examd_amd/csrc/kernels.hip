@@ -641,12 +641,6 @@ __device__ __forceinline__ void lds_sub(long long *p, long long v) {
   __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Order LDS traffic between the lanes of one wave.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 // Packed region coordinates of an LDS atom: 8 bits per axis, plus the
 // periodic image code (s + 1, s in {-1, 0, 1}) per axis in bits 24..29.
 __device__ __forceinline__ int bc_x(int bc) { return bc & 255; }
