@@ -384,6 +384,20 @@ def cpu_baseline(pos, vel, box, nsteps):
     return len(pos) * nsteps / dt
 
 
+def host_cpu():
+    """Logical CPUs the process sees and the CPU model (/proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
 def pmc_probe(E, shape, melt=300):
     """Short program for rocprofv3 --pmc runs: `melt` MD steps of the bench
     system (kernels without tallies), then three mixed force evaluations
@@ -500,7 +514,8 @@ def main():
         cpu = {"value": v, "unit": "atom-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} velocity-Verlet steps (rebin + "
                          f"force) of the same 2,048,000-atom system, FP64, "
-                         f"oracle C restatement of the reference path"}
+                         f"oracle C restatement of the reference path",
+               "host": host_cpu()}
 
     traffic = read_traffic()
     line = {
