@@ -4,6 +4,10 @@ last block of an axis is enlarged), random densities 0.5-1.0 (uniform random
 positions with a 0.85σ minimum separation), and random cell-block maps,
 workgroup sizes, queue depths, persistent grids and balanced boundaries.
 
+The cases cycle through the pair functions: the one-species LJ pair, three
+species with random symmetric (ε, σ, cutoff) tables, and the LJ pair as a
+4096-point table.
+
 Bars as in the other parity tests: FP64 forces within 1e-10 relative,
 energy within 1e-10, pair counts and per-home-block ownership exact; mixed
 within 2e-5 (max) of the oracle with the candidate count exact and Σf
@@ -12,9 +16,12 @@ exactly zero in fixed point.
 import numpy as np
 import pytest
 
+from tests import helpers as H
+
 pytestmark = pytest.mark.gpu
 
-CASES = 8
+CASES = 12
+KINDS = ("lj", "species", "table")
 
 
 @pytest.fixture(scope="module")
@@ -40,15 +47,36 @@ def _case(E, k):
     return pos, box, shape
 
 
+def _species(n, k):
+    """Three species at random, symmetric random (eps, sigma, cutoff)."""
+    rng = np.random.default_rng(3000 + k)
+    types = rng.integers(0, 3, n).astype(np.int32)
+    tabs = []
+    for lo, hi in ((0.5, 1.5), (0.8, 1.1), (2.0, 2.5)):
+        a = rng.uniform(lo, hi, (3, 3))
+        tabs.append(np.triu(a) + np.triu(a, 1).T)
+    return (types, *tabs)
+
+
 @pytest.mark.parametrize("k", range(CASES))
 def test_sweep_matches_oracle(E, oracle, k):
     pos, box, shape = _case(E, k)
-    print(f"[sweep {k}] n={len(pos)} box={np.round(box, 2).tolist()} {shape}",
-          flush=True)
-    ref, t, hc = oracle.forces(pos, box, 2.5, home_counts=True)
+    kind = KINDS[k % 3]
+    print(f"[sweep {k}] {kind} n={len(pos)} box={np.round(box, 2).tolist()} "
+          f"{shape}", flush=True)
+    extra = {}
+    if kind == "species":
+        extra["species"] = _species(len(pos), k)
+    elif kind == "table":
+        extra["table"] = H.lj_table(4096, 0.6, 2.5)
+    ref, t, hc = oracle.forces(pos, box, 2.5, home_counts=True, **extra)
     scale = np.sqrt((ref ** 2).sum(1).mean())
     with E.MidpointLJ(len(pos), box, 2.5) as m:
         m.set_state(pos)
+        if kind == "species":
+            m.set_species(*extra["species"])
+        elif kind == "table":
+            m.set_pair_table(*extra["table"])
         for prec in ("fp64", "mixed"):
             m.set_config(prec, **shape)
             f, s, _ = m.forces(prec)
