@@ -100,6 +100,7 @@ struct examd_handle {
   double force_ms = 0.0;
   int64_t force_launches = 0;
   int64_t since_repartition[2] = {0, 0};  // steps run since, per precision
+  int *host_start = nullptr;      // pinned copy of start[] for re-partitions
 };
 
 namespace {
@@ -162,6 +163,8 @@ void free_state(examd_handle *h) {
   h->part_cap = 0;
   free_tables(h->tab[0]);
   free_tables(h->tab[1]);
+  if (h->host_start) (void)hipHostFree(h->host_start);
+  h->host_start = nullptr;
 }
 
 int64_t tiles_of(const TileConfig &c) {
@@ -377,33 +380,55 @@ int max_region_blocks(const Cuts &b) {
   return m[0] * m[1] * m[2];
 }
 
-// Largest number of atoms any tile's region holds.  cnt: the system's
-// atoms per block.
+// Sums of n values v(0..n) over the periodic ranges [c[t] - 1, c[t+1] + 1)
+// of the T tiles of one axis (the tile and one block on each side), from a
+// prefix sum over the line with one periodic block added at each end.
+template <class V>
+void region_line(int n, const std::vector<int> &c, int T, V v,
+                 std::vector<int64_t> &pre, int64_t *out, size_t ostride) {
+  pre.resize((size_t)n + 3);
+  int64_t *q = pre.data();
+  q[0] = 0;
+  q[1] = v(n - 1);
+  for (int i = 0; i < n; ++i) q[i + 2] = q[i + 1] + v(i);
+  q[n + 2] = q[n + 1] + v(0);
+  for (int t = 0; t < T; ++t) out[(size_t)t * ostride] = q[c[t + 1] + 2] - q[c[t]];
+}
+
+// Largest number of atoms any tile's region (the tile plus one block on each
+// side, periodic) holds.  cnt: the system's atoms per block, x fastest.  On
+// a rectilinear map the region sums separate: along x to one sum per (tile
+// column, y, z), then along y, then along z.
 int64_t max_region_atoms(const Grid &g, const Cuts &cb, const int *cnt) {
   const int nx = g.nb[0], ny = g.nb[1], nz = g.nb[2];
-  // per-axis tile boundaries
-  std::vector<int> lo[3], hi[3];
-  for (int a = 0; a < 3; ++a)
-    for (int t = 0; t < g.T[a]; ++t) {
-      lo[a].push_back(cb[a][t]);
-      hi[a].push_back(cb[a][t + 1]);
+  const int Tx = g.T[0], Ty = g.T[1], Tz = g.T[2];
+  std::vector<int64_t> pre;
+  // A[(z * ny + y) * Tx + tx]: x-region sums of row (y, z)
+  std::vector<int64_t> A((size_t)Tx * ny * nz);
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y) {
+      const int *row = cnt + (size_t)nx * (y + (size_t)ny * z);
+      region_line(nx, cb[0], Tx, [&](int x) { return (int64_t)row[x]; }, pre,
+                  &A[((size_t)z * ny + y) * Tx], 1);
     }
+  // B[(z * Ty + ty) * Tx + tx]: then over y
+  std::vector<int64_t> B((size_t)Tx * Ty * nz);
+  for (int z = 0; z < nz; ++z)
+    for (int tx = 0; tx < Tx; ++tx) {
+      const int64_t *col = &A[(size_t)z * ny * Tx + tx];
+      region_line(ny, cb[1], Ty, [&](int y) { return col[(size_t)y * Tx]; }, pre,
+                  &B[(size_t)z * Ty * Tx + tx], (size_t)Tx);
+    }
+  // then over z
   int64_t best = 0;
-  for (int tz = 0; tz < g.T[2]; ++tz)
-    for (int ty = 0; ty < g.T[1]; ++ty)
-      for (int tx = 0; tx < g.T[0]; ++tx) {
-        int64_t s = 0;
-        for (int cz = lo[2][tz] - 1; cz <= hi[2][tz]; ++cz) {
-          int gz = (cz + nz) % nz;
-          for (int cy = lo[1][ty] - 1; cy <= hi[1][ty]; ++cy) {
-            int gy = (cy + ny) % ny;
-            const int *row = &cnt[(size_t)nx * (gy + (size_t)ny * gz)];
-            for (int cx = lo[0][tx] - 1; cx <= hi[0][tx]; ++cx)
-              s += row[(cx + nx) % nx];
-          }
-        }
-        best = std::max(best, s);
-      }
+  std::vector<int64_t> C((size_t)Tz);
+  for (int ty = 0; ty < Ty; ++ty)
+    for (int tx = 0; tx < Tx; ++tx) {
+      const int64_t *col = &B[(size_t)ty * Tx + tx];
+      region_line(nz, cb[2], Tz, [&](int z) { return col[(size_t)z * Ty * Tx]; },
+                  pre, C.data(), 1);
+      for (int64_t v : C) best = std::max(best, v);
+    }
   return best;
 }
 
@@ -521,10 +546,16 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
 // map was replaced.
 int repartition(examd_handle *h, int prec, bool *changed) {
   *changed = false;
-  std::vector<int> start((size_t)h->nblocks + 1);
+  if (!h->host_start &&
+      hipHostMalloc(reinterpret_cast<void **>(&h->host_start),
+                    (h->nblocks + 1) * sizeof(int)) != hipSuccess) {
+    h->host_start = nullptr;
+    return EXAMD_ERR_NOMEM;
+  }
+  const int *start = h->host_start;
+  HIPCHK(hipMemcpyAsync(h->host_start, h->d.start, (h->nblocks + 1) * sizeof(int),
+                        hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(start.data(), h->d.start, (h->nblocks + 1) * sizeof(int),
-                   hipMemcpyDeviceToHost));
   std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
   for (int64_t b = 0; b < h->nblocks; ++b) cnt[b] = start[b + 1] - start[b];
   const TileConfig saved = h->tc[prec];
