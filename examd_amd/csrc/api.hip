@@ -101,6 +101,7 @@ struct examd_handle {
   int64_t force_launches = 0;
   int64_t since_repartition[2] = {0, 0};  // steps run since, per precision
   int *host_start = nullptr;      // pinned copy of start[] for re-partitions
+  double *dev_stage = nullptr;    // 3 n doubles: x y z per atom, host layout
 };
 
 namespace {
@@ -165,6 +166,16 @@ void free_state(examd_handle *h) {
   free_tables(h->tab[1]);
   if (h->host_start) (void)hipHostFree(h->host_start);
   h->host_start = nullptr;
+  if (h->dev_stage) (void)hipFree(h->dev_stage);
+  h->dev_stage = nullptr;
+}
+
+// Device buffer of x y z per atom for transfers in the host layout (one
+// contiguous copy, transposed on the device).
+int ensure_stage(examd_handle *h) {
+  if (h->dev_stage) return EXAMD_OK;
+  return dalloc(&h->dev_stage, 3 * std::max<int64_t>(h->n, 1)) ? EXAMD_ERR_NOMEM
+                                                                : EXAMD_OK;
 }
 
 int64_t tiles_of(const TileConfig &c) {
@@ -782,14 +793,11 @@ void fill_stats(const examd_handle *h, const StepLog &L, int prec,
 int download_positions(examd_handle *h, std::vector<double> &aos) {
   const int64_t n = h->n;
   aos.assign((size_t)std::max<int64_t>(3 * n, 1), 0.0);
-  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
-  const double *src[3] = {h->d.x, h->d.y, h->d.z};
+  if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
+  launch_soa_to_aos(h->d.x, h->d.y, h->d.z, h->dev_stage, n, h->stream);
+  HIPCHK(hipMemcpyAsync(aos.data(), h->dev_stage, 3 * n * sizeof(double),
+                        hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  for (int a = 0; a < 3; ++a) {
-    HIPCHK(hipMemcpy(buf.data(), src[a], n * sizeof(double),
-                     hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i) aos[3 * i + a] = buf[i];
-  }
   return EXAMD_OK;
 }
 
@@ -1068,24 +1076,32 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
   HIPCHK(hipSetDevice(h->dev));
   for (int64_t i = 0; i < 3 * n; ++i)
     if (!std::isfinite(pos[i])) return EXAMD_ERR_NONFINITE;
-  for (int prec = 0; prec < 2; ++prec) {
-    int st = choose_caps(h, prec, pos);
-    if (st) return st;
+  {
+    const std::vector<int> cnt = host_counts(h, pos);
+    for (int prec = 0; prec < 2; ++prec) {
+      int st = choose_caps_counts(h, prec, cnt.data(), nullptr, true);
+      if (st) return st;
+    }
   }
   if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
+  if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
   drop_graph(h);
   HIPCHK(hipStreamSynchronize(h->stream));
-  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
-  double *dst[3] = {h->d.x, h->d.y, h->d.z};
-  double *vdst[3] = {h->d.vx, h->d.vy, h->d.vz};
-  for (int a = 0; a < 3; ++a) {
-    for (int64_t i = 0; i < n; ++i) buf[i] = pos[3 * i + a];
-    HIPCHK(hipMemcpy(dst[a], buf.data(), n * sizeof(double),
-                     hipMemcpyHostToDevice));
-    for (int64_t i = 0; i < n; ++i) buf[i] = vel ? vel[3 * i + a] : 0.0;
-    HIPCHK(hipMemcpy(vdst[a], buf.data(), n * sizeof(double),
-                     hipMemcpyHostToDevice));
+  // one copy per array in the host layout, transposed on the device
+  HIPCHK(hipMemcpyAsync(h->dev_stage, pos, 3 * n * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
+  launch_aos_to_soa(h->dev_stage, h->d.x, h->d.y, h->d.z, n, h->stream);
+  if (vel) {
+    HIPCHK(hipMemcpyAsync(h->dev_stage, vel, 3 * n * sizeof(double),
+                          hipMemcpyHostToDevice, h->stream));
+    launch_aos_to_soa(h->dev_stage, h->d.vx, h->d.vy, h->d.vz, n, h->stream);
+  } else {
+    HIPCHK(hipMemsetAsync(h->d.vx, 0, n * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(h->d.vy, 0, n * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(h->d.vz, 0, n * sizeof(double), h->stream));
   }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
   h->state_set = true;
   h->forces_valid = -1;
   h->since_repartition[0] = h->since_repartition[1] = 0;
@@ -1254,34 +1270,27 @@ int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
   if (!h || n != h->n) return EXAMD_ERR_ARG;
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   HIPCHK(hipSetDevice(h->dev));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  std::vector<double> buf((size_t)std::max<int64_t>(n, 1));
-  const double *src[3] = {h->d.x, h->d.y, h->d.z};
-  const double *vsrc[3] = {h->d.vx, h->d.vy, h->d.vz};
-  for (int a = 0; a < 3; ++a) {
-    if (pos) {
-      HIPCHK(hipMemcpy(buf.data(), src[a], n * sizeof(double),
-                       hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < n; ++i) pos[3 * i + a] = buf[i];
-    }
-    if (vel) {
-      HIPCHK(hipMemcpy(buf.data(), vsrc[a], n * sizeof(double),
-                       hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < n; ++i) vel[3 * i + a] = buf[i];
-    }
+  if (force && h->forces_valid < 0) return EXAMD_ERR_NOT_READY;
+  if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
+  // each array: transposed into the host layout on the device, one copy
+  if (pos) {
+    launch_soa_to_aos(h->d.x, h->d.y, h->d.z, h->dev_stage, n, h->stream);
+    HIPCHK(hipMemcpyAsync(pos, h->dev_stage, 3 * n * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
+  }
+  if (vel) {
+    launch_soa_to_aos(h->d.vx, h->d.vy, h->d.vz, h->dev_stage, n, h->stream);
+    HIPCHK(hipMemcpyAsync(vel, h->dev_stage, 3 * n * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
   }
   if (force) {
-    if (h->forces_valid < 0) return EXAMD_ERR_NOT_READY;
-    double *tmp = nullptr;
-    if (dalloc(&tmp, 3 * n)) return EXAMD_ERR_NOMEM;
-    launch_forces_to_fp64(h->d, tmp, h->forces_valid == EXAMD_PREC_MIXED,
+    launch_forces_to_fp64(h->d, h->dev_stage, h->forces_valid == EXAMD_PREC_MIXED,
                           h->stream);
-    hipError_t e = hipMemcpyAsync(force, tmp, 3 * n * sizeof(double),
-                                  hipMemcpyDeviceToHost, h->stream);
-    hipError_t e2 = hipStreamSynchronize(h->stream);
-    (void)hipFree(tmp);
-    if (e != hipSuccess || e2 != hipSuccess) return EXAMD_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(force, h->dev_stage, 3 * n * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
   }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
   return EXAMD_OK;
 }
 
@@ -1313,15 +1322,12 @@ int examd_hip_forces(examd_handle *h, int precision, double *forces,
     return st;
   }
   if (forces) {
-    double *tmp = nullptr;
-    if (dalloc(&tmp, 3 * h->n)) return EXAMD_ERR_NOMEM;
-    launch_forces_to_fp64(h->d, tmp, precision == EXAMD_PREC_MIXED,
+    if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
+    launch_forces_to_fp64(h->d, h->dev_stage, precision == EXAMD_PREC_MIXED,
                           h->stream);
-    hipError_t e = hipMemcpyAsync(forces, tmp, 3 * h->n * sizeof(double),
-                                  hipMemcpyDeviceToHost, h->stream);
-    hipError_t e2 = hipStreamSynchronize(h->stream);
-    (void)hipFree(tmp);
-    if (e != hipSuccess || e2 != hipSuccess) return EXAMD_ERR_HIP;
+    HIPCHK(hipMemcpyAsync(forces, h->dev_stage, 3 * h->n * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
   }
   return st;
 }

@@ -1300,6 +1300,24 @@ __global__ void k_forces_to_fp64(DevState d, double *out, bool mixed) {
   }
 }
 
+// Host interchange layout (x y z per atom) to and from the device SoA.
+__global__ void k_aos_to_soa(const double *aos, double *x, double *y, double *z,
+                             int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = aos[3 * i];
+  y[i] = aos[3 * i + 1];
+  z[i] = aos[3 * i + 2];
+}
+__global__ void k_soa_to_aos(const double *x, const double *y, const double *z,
+                             double *aos, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  aos[3 * i] = x[i];
+  aos[3 * i + 1] = y[i];
+  aos[3 * i + 2] = z[i];
+}
+
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 template <bool FP64, bool ENERGY, bool HOMECNT, int PK>
@@ -1483,6 +1501,16 @@ void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,
                            hipStream_t s) {
   if (d.n > 0)
     k_forces_to_fp64<<<nblk(d.n, 256), 256, 0, s>>>(d, out_aos, mixed);
+}
+
+void launch_aos_to_soa(const double *aos, double *x, double *y, double *z,
+                       int64_t n, hipStream_t s) {
+  if (n > 0) k_aos_to_soa<<<nblk(n, 256), 256, 0, s>>>(aos, x, y, z, n);
+}
+
+void launch_soa_to_aos(const double *x, const double *y, const double *z,
+                       double *aos, int64_t n, hipStream_t s) {
+  if (n > 0) k_soa_to_aos<<<nblk(n, 256), 256, 0, s>>>(x, y, z, aos, n);
 }
 
 }  // namespace examd

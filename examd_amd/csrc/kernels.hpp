@@ -105,6 +105,11 @@ void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
 void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                    const ForceConfig &c, bool mixed, bool energy,
                    bool homecount, StepLog *log, hipStream_t s);
+// x y z per atom (the host interchange layout) to and from three arrays.
+void launch_aos_to_soa(const double *aos, double *x, double *y, double *z,
+                       int64_t n, hipStream_t s);
+void launch_soa_to_aos(const double *x, const double *y, const double *z,
+                       double *aos, int64_t n, hipStream_t s);
 void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,
                            hipStream_t s);
 
