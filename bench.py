@@ -106,13 +106,28 @@ def force_error(m):
             "rms_rel_to_rms_force": float(np.sqrt(np.mean(d * d))) / frms}
 
 
+def boundary_ms(m, prec, reps=3):
+    """Milliseconds of one evaluation through the boundary with host
+    buffers, as the drop-in force call makes it: positions in (set_state),
+    bin + sort + force, forces out.  PCIe-inclusive; never the headline."""
+    p, _ = m.get_state()
+    m.set_state(p)
+    m.forces(prec)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.set_state(p)
+        m.forces(prec)
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
 def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
               shape=None, graph_chunk=100, report_error=False, tune_after=300):
     """Run `tune_after` untimed steps (melting the lattice), then autotune
     the launch shape for this precision (or apply the given shape), warm up
     and time `steps` MD steps, replayed from captured hipGraph chunks of
     `graph_chunk` steps.  With report_error, the mixed-vs-FP64 force error is
-    taken at the final positions (outside the timed region)."""
+    taken at the final positions (outside the timed region), with the
+    PCIe-inclusive time of one evaluation through the boundary."""
     err = None
     with E.MidpointLJ(len(pos), box, RC, graph_chunk=graph_chunk) as m:
         m.set_state(pos, vel)
@@ -137,6 +152,7 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
         force_ms, launches = m.timing(enable=False, reset=True)
         if report_error:
             err = force_error(m)
+            err["boundary_eval_ms_mixed"] = boundary_ms(m, prec)
     if launches < steps // TIME_EVERY:
         raise RuntimeError(f"{prec}: {launches} timed force launches for "
                            f"{steps} steps")
@@ -487,6 +503,7 @@ def main():
 
     rate_mx = n * steps / wall_mx
     rate_64 = n * fsteps / wall_64
+    boundary = ferr.pop("boundary_eval_ms_mixed", None) if ferr else None
     # algorithmic work of one timed force launch: mean P_cand, P_cut over
     # the energy steps tallied; the timed launches tally no energy
     cand = float(np.mean([s.pairs_cand for s in st_mx]))
@@ -562,6 +579,12 @@ def main():
             "energy_max_dev_rel_mixed": max_dev(st_mx),
             "energy_max_dev_rel_fp64": max_dev(st_64),
             "mixed_force_error_vs_fp64_at_end": ferr,
+            "host_boundary": {
+                "eval_ms_mixed": boundary,
+                "what": "one evaluation through the C ABI with host buffers, "
+                        "as the drop-in force call makes it: set_state "
+                        "(positions in), bin + sort + force, forces out; "
+                        "PCIe-inclusive, not the headline"},
             "launch_mixed": tune_mx,
             "launch_fp64": tune_64,
             "graph_chunk_steps": args.graph_chunk,
