@@ -57,7 +57,7 @@ def build_library(verbose: bool = False) -> str:
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp,
           *objs])
     os.replace(tmp, LIB)
     return LIB

@@ -9,10 +9,12 @@
 // one.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/examd_hip.h"
@@ -324,18 +326,36 @@ inline int host_block(double x, double bl, int nb) {
 
 // Atoms per block of every system for host positions, in the block order
 // of the binning kernel (x fastest, systems one after another).
+// f(i0, i1) over [0, n) in contiguous pieces on up to 16 host threads.
+template <class F>
+void host_parallel_for(int64_t n, F f) {
+  unsigned t = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < (int64_t)1 << 16) t = 1;
+  std::vector<std::thread> th;
+  for (unsigned k = 1; k < t; ++k)
+    th.emplace_back(f, n * k / t, n * (k + 1) / t);
+  f((int64_t)0, n / t);
+  for (std::thread &x : th) x.join();
+}
+
+// Atoms per block of every system for host positions, in the block order
+// of the binning kernel (x fastest, systems one after another).  Each atom's
+// block is found on several threads, then counted in one pass.
 std::vector<int> host_counts(const examd_handle *h, const double *pos) {
-  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
-  for (const SysHost &S : h->sys) {
-    const Grid &g = S.g;
-    const double *p = pos + 3 * S.atom0;
-    for (int64_t i = 0; i < S.n; ++i) {
-      int b[3];
-      for (int a = 0; a < 3; ++a) b[a] = host_block(p[3 * i + a], g.bl, g.nb[a]);
-      cnt[(size_t)S.block0 + b[0] +
-          (size_t)g.nb[0] * (b[1] + (size_t)g.nb[1] * b[2])]++;
+  std::vector<int> blk((size_t)std::max<int64_t>(h->n, 1));
+  host_parallel_for(h->n, [&](int64_t i0, int64_t i1) {
+    for (const SysHost &S : h->sys) {
+      const int64_t a0 = std::max(i0, S.atom0), a1 = std::min(i1, S.atom0 + S.n);
+      const Grid &g = S.g;
+      for (int64_t i = a0; i < a1; ++i) {
+        int b[3];
+        for (int a = 0; a < 3; ++a) b[a] = host_block(pos[3 * i + a], g.bl, g.nb[a]);
+        blk[i] = (int)(S.block0 + b[0] + (int64_t)g.nb[0] * (b[1] + (int64_t)g.nb[1] * b[2]));
+      }
     }
-  }
+  });
+  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
+  for (int64_t i = 0; i < h->n; ++i) cnt[blk[i]]++;
   return cnt;
 }
 
@@ -1074,8 +1094,17 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
                         int64_t n) {
   if (!h || !pos || n != h->n) return EXAMD_ERR_ARG;
   HIPCHK(hipSetDevice(h->dev));
-  for (int64_t i = 0; i < 3 * n; ++i)
-    if (!std::isfinite(pos[i])) return EXAMD_ERR_NONFINITE;
+  {
+    std::atomic<bool> nonfinite(false);
+    host_parallel_for(3 * n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i)
+        if (!std::isfinite(pos[i])) {
+          nonfinite.store(true, std::memory_order_relaxed);
+          return;
+        }
+    });
+    if (nonfinite.load()) return EXAMD_ERR_NONFINITE;
+  }
   {
     const std::vector<int> cnt = host_counts(h, pos);
     for (int prec = 0; prec < 2; ++prec) {
