@@ -104,6 +104,7 @@ struct examd_handle {
   int64_t since_repartition[2] = {0, 0};  // steps run since, per precision
   int *host_start = nullptr;      // pinned copy of start[] for re-partitions
   double *dev_stage = nullptr;    // 3 n doubles: x y z per atom, host layout
+  double *host_stage = nullptr;   // pinned 3 n doubles, the same layout
 };
 
 namespace {
@@ -170,14 +171,22 @@ void free_state(examd_handle *h) {
   h->host_start = nullptr;
   if (h->dev_stage) (void)hipFree(h->dev_stage);
   h->dev_stage = nullptr;
+  if (h->host_stage) (void)hipHostFree(h->host_stage);
+  h->host_stage = nullptr;
 }
 
 // Device buffer of x y z per atom for transfers in the host layout (one
 // contiguous copy, transposed on the device).
 int ensure_stage(examd_handle *h) {
-  if (h->dev_stage) return EXAMD_OK;
-  return dalloc(&h->dev_stage, 3 * std::max<int64_t>(h->n, 1)) ? EXAMD_ERR_NOMEM
-                                                                : EXAMD_OK;
+  const int64_t m = 3 * std::max<int64_t>(h->n, 1);
+  if (!h->dev_stage && dalloc(&h->dev_stage, m)) return EXAMD_ERR_NOMEM;
+  if (!h->host_stage &&
+      hipHostMalloc(reinterpret_cast<void **>(&h->host_stage),
+                    m * sizeof(double)) != hipSuccess) {
+    h->host_stage = nullptr;
+    return EXAMD_ERR_NOMEM;
+  }
+  return EXAMD_OK;
 }
 
 int64_t tiles_of(const TileConfig &c) {
@@ -336,6 +345,30 @@ void host_parallel_for(int64_t n, F f) {
     th.emplace_back(f, n * k / t, n * (k + 1) / t);
   f((int64_t)0, n / t);
   for (std::thread &x : th) x.join();
+}
+
+// x y z per atom between a caller's array and dev_stage, through the pinned
+// staging buffer; the caller's array is copied on several threads.  The
+// stream is synchronised before the staging buffer is refilled.
+int upload_aos(examd_handle *h, const double *src) {
+  const int64_t m = 3 * h->n;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  host_parallel_for(m, [&](int64_t i0, int64_t i1) {
+    std::memcpy(h->host_stage + i0, src + i0, (size_t)(i1 - i0) * sizeof(double));
+  });
+  HIPCHK(hipMemcpyAsync(h->dev_stage, h->host_stage, m * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
+  return EXAMD_OK;
+}
+int download_aos(examd_handle *h, double *dst) {
+  const int64_t m = 3 * h->n;
+  HIPCHK(hipMemcpyAsync(h->host_stage, h->dev_stage, m * sizeof(double),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  host_parallel_for(m, [&](int64_t i0, int64_t i1) {
+    std::memcpy(dst + i0, h->host_stage + i0, (size_t)(i1 - i0) * sizeof(double));
+  });
+  return EXAMD_OK;
 }
 
 // Atoms per block of every system for host positions, in the block order
@@ -815,10 +848,7 @@ int download_positions(examd_handle *h, std::vector<double> &aos) {
   aos.assign((size_t)std::max<int64_t>(3 * n, 1), 0.0);
   if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
   launch_soa_to_aos(h->d.x, h->d.y, h->d.z, h->dev_stage, n, h->stream);
-  HIPCHK(hipMemcpyAsync(aos.data(), h->dev_stage, 3 * n * sizeof(double),
-                        hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return EXAMD_OK;
+  return download_aos(h, aos.data());
 }
 
 // Apply a launch shape to one precision: the cell-block map (explicit T of
@@ -1117,12 +1147,11 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
   drop_graph(h);
   HIPCHK(hipStreamSynchronize(h->stream));
   // one copy per array in the host layout, transposed on the device
-  HIPCHK(hipMemcpyAsync(h->dev_stage, pos, 3 * n * sizeof(double),
-                        hipMemcpyHostToDevice, h->stream));
+  int st = upload_aos(h, pos);
+  if (st) return st;
   launch_aos_to_soa(h->dev_stage, h->d.x, h->d.y, h->d.z, n, h->stream);
   if (vel) {
-    HIPCHK(hipMemcpyAsync(h->dev_stage, vel, 3 * n * sizeof(double),
-                          hipMemcpyHostToDevice, h->stream));
+    if ((st = upload_aos(h, vel)) != EXAMD_OK) return st;
     launch_aos_to_soa(h->dev_stage, h->d.vx, h->d.vy, h->d.vz, n, h->stream);
   } else {
     HIPCHK(hipMemsetAsync(h->d.vx, 0, n * sizeof(double), h->stream));
@@ -1302,24 +1331,21 @@ int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
   if (force && h->forces_valid < 0) return EXAMD_ERR_NOT_READY;
   if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
   // each array: transposed into the host layout on the device, one copy
+  int st = EXAMD_OK;
   if (pos) {
     launch_soa_to_aos(h->d.x, h->d.y, h->d.z, h->dev_stage, n, h->stream);
-    HIPCHK(hipMemcpyAsync(pos, h->dev_stage, 3 * n * sizeof(double),
-                          hipMemcpyDeviceToHost, h->stream));
+    if ((st = download_aos(h, pos)) != EXAMD_OK) return st;
   }
   if (vel) {
     launch_soa_to_aos(h->d.vx, h->d.vy, h->d.vz, h->dev_stage, n, h->stream);
-    HIPCHK(hipMemcpyAsync(vel, h->dev_stage, 3 * n * sizeof(double),
-                          hipMemcpyDeviceToHost, h->stream));
+    if ((st = download_aos(h, vel)) != EXAMD_OK) return st;
   }
   if (force) {
     launch_forces_to_fp64(h->d, h->dev_stage, h->forces_valid == EXAMD_PREC_MIXED,
                           h->stream);
-    HIPCHK(hipMemcpyAsync(force, h->dev_stage, 3 * n * sizeof(double),
-                          hipMemcpyDeviceToHost, h->stream));
+    if ((st = download_aos(h, force)) != EXAMD_OK) return st;
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(h->stream));
   return EXAMD_OK;
 }
 
@@ -1354,9 +1380,8 @@ int examd_hip_forces(examd_handle *h, int precision, double *forces,
     if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
     launch_forces_to_fp64(h->d, h->dev_stage, precision == EXAMD_PREC_MIXED,
                           h->stream);
-    HIPCHK(hipMemcpyAsync(forces, h->dev_stage, 3 * h->n * sizeof(double),
-                          hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    const int s2 = download_aos(h, forces);
+    if (s2 != EXAMD_OK) return s2;
   }
   return st;
 }
