@@ -1665,15 +1665,15 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
   // the fastest few shapes of the first pass, refined in the second
   constexpr int kRefine = 3;
   std::vector<std::pair<double, TileConfig>> top;
-  // time `trials` evaluations of the current shape (1e300 if it overflowed)
-  auto time_shape = [&](double *ms) -> int {
+  // time `reps` evaluations of the current shape (1e300 if it overflowed)
+  auto time_shape = [&](double *ms, int reps) -> int {
     launch_fill_log(h->log, ns, h->stream);
     evaluate(h, precision, false, false, h->log);  // warm-up
     h->timing = true;
     h->ev_used = 0;
     h->force_ms = 0.0;
     h->force_launches = 0;
-    for (int t = 0; t < trials; ++t) {
+    for (int t = 0; t < reps; ++t) {
       launch_fill_log(h->log, ns, h->stream);
       evaluate(h, precision, false, false, h->log);
     }
@@ -1696,7 +1696,7 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
                      pos.data()) != EXAMD_OK)
       return EXAMD_OK;
     double ms;
-    int e = time_shape(&ms);
+    int e = time_shape(&ms, trials);
     if (e) return e;
     if (ms < best_ms) {
       best_ms = ms;
@@ -1746,6 +1746,35 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
           return st;
         }
       }
+  }
+  // the fastest shapes of both passes, timed again with four times the
+  // trials: the first timings are a few launches each, and the leaders are
+  // often within their noise of one another
+  const std::vector<std::pair<double, TileConfig>> finals = top;
+  if (finals.size() > 1) {
+    double final_ms = 1e300;
+    TileConfig final_c = bestc;
+    for (const auto &cand : finals) {
+      const TileConfig &b = cand.second;
+      if (apply_config(h, precision, b.edge == 0 ? b.T[0].data() : nullptr,
+                       (double)b.edge, b.threads, b.fixed_groups, b.lane_queue,
+                       b.workgroups, b.balance, pos.data()) != EXAMD_OK)
+        continue;
+      double ms;
+      st = time_shape(&ms, 4 * trials);
+      if (st) {
+        h->timing = save_timing;
+        return st;
+      }
+      if (ms < final_ms) {
+        final_ms = ms;
+        final_c = h->tc[precision];
+      }
+    }
+    if (final_ms < 1e300) {
+      best_ms = final_ms;
+      bestc = final_c;
+    }
   }
   h->timing = save_timing;
   h->force_ms = save_ms;
