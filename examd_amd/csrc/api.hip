@@ -1136,11 +1136,23 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
     if (nonfinite.load()) return EXAMD_ERR_NONFINITE;
   }
   {
+    // both precisions' maps planned on two threads (each writes only its
+    // own TileConfig), then their tables uploaded here
     const std::vector<int> cnt = host_counts(h, pos);
+    int st1 = EXAMD_OK;
+    std::thread other([&] {
+      st1 = choose_caps_counts(h, 1, cnt.data(), nullptr, false);
+    });
+    const int st0 = choose_caps_counts(h, 0, cnt.data(), nullptr, false);
+    other.join();
+    const int planned[2] = {st0, st1};
     for (int prec = 0; prec < 2; ++prec) {
-      int st = choose_caps_counts(h, prec, cnt.data(), nullptr, true);
+      if (planned[prec] != EXAMD_OK) continue;
+      const int st = upload_tables(h, prec);
       if (st) return st;
     }
+    if (st0) return st0;
+    if (st1) return st1;
   }
   if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
   if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
