@@ -325,3 +325,25 @@ def test_checkpoint_restart_continues_exactly(E, tmp_path):
         m.run(20, 0.005, "mixed")
         p40, v40 = m.get_state()
     assert np.array_equal(p40, p_ref) and np.array_equal(v40, v_ref)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_get_state_returns_forces_of_the_last_evaluation(E, prec):
+    """examd_hip_get_state with a force array: the forces of the last
+    evaluation (after a run, the forces at its final positions), in the host
+    layout; before any evaluation the call reports NOT_READY."""
+    pos, vel, box = E.generate_fcc(9, RHO, 0.9, seed=41)
+    with E.MidpointLJ(len(pos), box, 2.5, graph_chunk=10) as m:
+        m.set_state(pos, vel)
+        with pytest.raises(E.ExamdError):
+            m.get_state(forces=True)
+        m.run(25, 0.005, prec)
+        p, v, f_run = m.get_state(forces=True)
+        f_now, _, _ = m.forces(prec)
+        p2, v2, f_again = m.get_state(forces=True)
+    assert np.array_equal(p, p2) and np.array_equal(v, v2)
+    assert np.array_equal(f_again, f_now)
+    if prec == "mixed":
+        assert np.array_equal(f_run, f_now)
+    else:
+        assert _rel(f_run, f_now) < 1e-13
