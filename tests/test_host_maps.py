@@ -1,0 +1,171 @@
+"""Host-side arithmetic of the cell-block maps (examd_amd/csrc/host_maps.hpp),
+compiled with the host compiler alone:
+
+- the largest region of a map (a cell block plus one block on each side,
+  periodic), which sizes the force kernel's LDS, against a brute-force
+  numpy sum over every cell block of random maps and block counts;
+- balanced boundaries: ordered, every cell block at least one block wide,
+  the even split for uniform counts, and every cell block within one block
+  slab of an equal share of the atoms;
+- the block of a position (exaMD_identify_block: x/bl + 1 truncated, clamped
+  to the grid) on exact faces, one ulp below them, and past the last face.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+DRIVER = r"""
+#include <cstdio>
+#include <vector>
+#include "examd_amd/csrc/host_maps.hpp"
+struct G { int nb[3]; int T[3]; };
+int main() {
+  int cases;
+  if (std::scanf("%d", &cases) != 1) return 1;
+  for (int c = 0; c < cases; ++c) {
+    int mode;
+    G g;
+    std::scanf("%d %d %d %d %d %d %d", &mode, &g.nb[0], &g.nb[1], &g.nb[2],
+               &g.T[0], &g.T[1], &g.T[2]);
+    const int nblk = g.nb[0] * g.nb[1] * g.nb[2];
+    if (mode == 0) {            // largest region of a given map
+      examd::Cuts cb;
+      for (int a = 0; a < 3; ++a) {
+        cb[a].resize(g.T[a] + 1);
+        for (int &v : cb[a]) std::scanf("%d", &v);
+      }
+      std::vector<int> cnt(nblk);
+      for (int &v : cnt) std::scanf("%d", &v);
+      std::printf("%lld\n", (long long)examd::max_region_atoms(g, cb, cnt.data()));
+    } else if (mode == 1) {     // balanced boundaries
+      std::vector<int> cnt(nblk);
+      long long n = 0;
+      for (int &v : cnt) { std::scanf("%d", &v); n += v; }
+      const examd::Cuts cb = examd::balanced_cuts(g, cnt.data(), n);
+      for (int a = 0; a < 3; ++a) {
+        for (int v : cb[a]) std::printf("%d ", v);
+        std::printf("\n");
+      }
+    } else {                    // block of positions: bl, count, values
+      double bl;
+      int m;
+      std::scanf("%lf %d", &bl, &m);
+      for (int i = 0; i < m; ++i) {
+        double x;
+        std::scanf("%lf", &x);
+        std::printf("%d ", examd::host_block(x, bl, g.nb[0]));
+      }
+      std::printf("\n");
+    }
+  }
+  return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ absent")
+    d = tmp_path_factory.mktemp("host_maps")
+    src = d / "driver.cpp"
+    src.write_text(DRIVER)
+    exe = d / "driver"
+    subprocess.run([cxx, "-std=c++17", "-O2", "-Wall", "-I", ROOT, str(src),
+                    "-o", str(exe)], check=True, capture_output=True,
+                   timeout=120)
+
+    def run(text):
+        r = subprocess.run([str(exe)], input=text, capture_output=True,
+                           text=True, timeout=120, check=True)
+        return r.stdout.split("\n")
+    return run
+
+
+def _random_cuts(rng, nb, T):
+    inner = np.sort(rng.choice(np.arange(1, nb), T - 1, replace=False))
+    return [0, *inner.tolist(), nb]
+
+
+def _region_max(cnt, cuts):
+    """Brute force: every cell block's region, periodic, summed by numpy."""
+    nz, ny, nx = cnt.shape
+    best = 0
+    for tz in range(len(cuts[2]) - 1):
+        iz = np.arange(cuts[2][tz] - 1, cuts[2][tz + 1] + 1) % nz
+        for ty in range(len(cuts[1]) - 1):
+            iy = np.arange(cuts[1][ty] - 1, cuts[1][ty + 1] + 1) % ny
+            for tx in range(len(cuts[0]) - 1):
+                ix = np.arange(cuts[0][tx] - 1, cuts[0][tx + 1] + 1) % nx
+                best = max(best, int(cnt[np.ix_(iz, iy, ix)].sum()))
+    return best
+
+
+def test_largest_region_matches_brute_force(driver):
+    rng = np.random.default_rng(7)
+    cases, want = [], []
+    for _ in range(60):
+        nb = rng.integers(5, 19, 3)
+        T = [int(rng.integers(1, min(n, 6) + 1)) for n in nb]
+        cuts = [_random_cuts(rng, int(n), t) for n, t in zip(nb, T)]
+        cnt = rng.integers(0, 6, (nb[2], nb[1], nb[0]))
+        want.append(_region_max(cnt, cuts))
+        cases.append(" ".join(map(str, [0, *nb, *T])) + "\n" +
+                     "\n".join(" ".join(map(str, c)) for c in cuts) + "\n" +
+                     " ".join(map(str, cnt.ravel())))
+    out = driver(f"{len(cases)}\n" + "\n".join(cases) + "\n")
+    got = [int(v) for v in out[:len(cases)]]
+    assert got == want
+
+
+def test_balanced_cuts_properties(driver):
+    rng = np.random.default_rng(8)
+    specs = []
+    for k in range(40):
+        nb = rng.integers(5, 25, 3)
+        T = [int(rng.integers(1, min(n, 8) + 1)) for n in nb]
+        if k < 10:
+            cnt = np.full((nb[2], nb[1], nb[0]), 3)        # uniform
+        else:
+            cnt = rng.integers(0, 8, (nb[2], nb[1], nb[0]))
+            cnt[:, :, : nb[0] // 3] *= 5                   # a dense slab
+        specs.append((nb, T, cnt))
+    text = f"{len(specs)}\n" + "\n".join(
+        " ".join(map(str, [1, *nb, *T])) + "\n" + " ".join(map(str, cnt.ravel()))
+        for nb, T, cnt in specs) + "\n"
+    out = driver(text)
+    for k, (nb, T, cnt) in enumerate(specs):
+        for a in range(3):
+            c = [int(v) for v in out[3 * k + a].split()]
+            assert c[0] == 0 and c[-1] == nb[a] and len(c) == T[a] + 1
+            assert all(c[t + 1] > c[t] for t in range(T[a]))
+            slab = cnt.sum(axis=tuple(i for i in range(3) if i != 2 - a))
+            if k < 10:
+                even = [t * nb[a] / T[a] for t in range(T[a] + 1)]
+                assert max(abs(x - y) for x, y in zip(c, even)) <= 1
+            else:
+                # each cell block within one slab of an equal share (the
+                # floor weight the balancer adds to empty slabs included)
+                w = slab + max(1e-3, 0.02 * cnt.sum() / nb[a])
+                share = w.sum() / T[a]
+                for t in range(T[a]):
+                    tile = w[c[t]:c[t + 1]].sum()
+                    if c[t + 1] - c[t] > 1:
+                        assert tile <= share + w.max() + 1e-9
+
+
+def test_block_of_position_faces(driver):
+    nb, bl = 13, 1.25
+    L = 16.9                                  # the last block is enlarged
+    xs = [0.0, bl, np.nextafter(bl, 0.0), 5 * bl, np.nextafter(5 * bl, 0.0),
+          12 * bl, np.nextafter(L, 0.0), 14 * bl + 0.3, -1e-12]
+    want = [0, 1, 0, 5, 4, 12, 12, 12, 0]
+    out = driver(f"1\n2 {nb} {nb} {nb} 1 1 1\n{bl!r} {len(xs)} " +
+                 " ".join(repr(float(x)) for x in xs) + "\n")
+    assert [int(v) for v in out[0].split()] == want
