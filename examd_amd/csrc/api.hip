@@ -1644,33 +1644,36 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
         }
       }
   }
-  // the fastest shapes of both passes, timed again with four times the
-  // trials: the first timings are a few launches each, and the leaders are
-  // often within their noise of one another
+  // the fastest shapes of both passes, timed again in turn (A, B, C, A, B,
+  // C, ...) for six rounds of `trials` launches each: the first timings are
+  // a few launches each, the leaders are often within their noise of one
+  // another, and interleaving keeps slow drifts of the clock from favouring
+  // whichever shape is timed first
   const std::vector<std::pair<double, TileConfig>> finals = top;
   if (finals.size() > 1) {
-    double final_ms = 1e300;
-    TileConfig final_c = bestc;
-    for (const auto &cand : finals) {
-      const TileConfig &b = cand.second;
-      if (apply_config(h, precision, b.edge == 0 ? b.T[0].data() : nullptr,
-                       (double)b.edge, b.threads, b.fixed_groups, b.lane_queue,
-                       b.workgroups, b.balance, pos.data()) != EXAMD_OK)
-        continue;
-      double ms;
-      st = time_shape(&ms, 4 * trials);
-      if (st) {
-        h->timing = save_timing;
-        return st;
+    std::vector<double> total(finals.size(), 0.0);
+    for (int round = 0; round < 6; ++round)
+      for (size_t k = 0; k < finals.size(); ++k) {
+        if (total[k] >= 1e300) continue;
+        h->tc[precision] = finals[k].second;
+        if ((st = upload_tables(h, precision)) != EXAMD_OK ||
+            (st = ensure_partials(h)) != EXAMD_OK) {
+          h->timing = save_timing;
+          return st;
+        }
+        double ms;
+        if ((st = time_shape(&ms, trials)) != EXAMD_OK) {
+          h->timing = save_timing;
+          return st;
+        }
+        total[k] = ms >= 1e300 ? 1e300 : total[k] + ms;
       }
-      if (ms < final_ms) {
-        final_ms = ms;
-        final_c = h->tc[precision];
-      }
-    }
-    if (final_ms < 1e300) {
-      best_ms = final_ms;
-      bestc = final_c;
+    size_t kbest = 0;
+    for (size_t k = 1; k < finals.size(); ++k)
+      if (total[k] < total[kbest]) kbest = k;
+    if (total[kbest] < 1e300) {
+      best_ms = total[kbest] / 6.0;
+      bestc = finals[kbest].second;
     }
   }
   h->timing = save_timing;

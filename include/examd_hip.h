@@ -252,9 +252,9 @@ typedef struct examd_tune {
 } examd_tune;
 
 /* Autotuner: time `trials` force evaluations of the current state for each
- * candidate launch shape, time the three fastest again with 4 x trials, keep
- * the fastest for `precision` and report it in best (may be NULL).  The
- * state is unchanged; forces must be recomputed. */
+ * candidate launch shape, time the three fastest again in turn for six
+ * rounds of `trials`, keep the fastest for `precision` and report it in best
+ * (may be NULL).  The state is unchanged; forces must be recomputed. */
 int examd_hip_autotune(examd_handle *h, int precision, int trials,
                        examd_tune *best);
 
