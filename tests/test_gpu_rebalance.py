@@ -146,3 +146,38 @@ def test_rebalance_config_checks(E):
     for a in range(3):
         np.testing.assert_array_equal(cuts[a], after[a])
         assert cuts[a][0] == 0
+
+
+def test_rebalance_with_species_in_a_batch(E):
+    """Periodic re-partition with two species (Kob-Andersen rows) in a batch
+    of two moving slabs: equal to stopping and re-applying the balanced map,
+    mixed results bit for bit."""
+    from tests import helpers as H
+    print("[rebalance species batch]", flush=True)
+    sy = [_moving_slab(E, s) for s in (11, 12)]
+    systems = [(len(p), box) for p, _, box in sy]
+    pos = np.vstack([p for p, _, _ in sy])
+    vel = np.vstack([v for _, v, _ in sy])
+    types = H.binary_types(len(pos), 0.2, 13)
+    out = []
+    for manual in (False, True):
+        with E.MidpointLJBatch(systems, 2.5, graph_chunk=25) as b:
+            b.set_state(pos, vel)
+            b.set_species(types, H.KA_EPS, H.KA_SIG, H.KA_RC)
+            b.set_config("mixed", block_threads=256, tile_edge=5, balance=1,
+                         rebalance_every=0 if manual else R)
+            if manual:
+                for _ in range(STEPS // R):
+                    b.run(R, DT, "mixed")
+                    b.set_config("mixed", block_threads=256, tile_edge=5,
+                                 balance=1)
+            else:
+                b.run(STEPS, DT, "mixed")
+            cuts = [b.get_cuts("mixed", k) for k in range(2)]
+            p, v = b.get_state()
+        out.append((cuts, p, v))
+    (ca, pa, va), (cb, pb, vb) = out
+    for k in range(2):
+        for a in range(3):
+            np.testing.assert_array_equal(ca[k][a], cb[k][a])
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
