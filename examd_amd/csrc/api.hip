@@ -324,11 +324,13 @@ Cuts cuts_of(const examd_handle *h, int prec, int k, const Grid &g) {
   return b;
 }
 
-// f(i0, i1) over [0, n) in contiguous pieces on up to 16 host threads.
+// f(i0, i1) over [0, n) in contiguous pieces on up to 16 host threads, one
+// thread when n is below min_n.
 template <class F>
-void host_parallel_for(int64_t n, F f) {
+void host_parallel_for(int64_t n, F f, int64_t min_n = (int64_t)1 << 16) {
   unsigned t = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < (int64_t)1 << 16) t = 1;
+  if (n < min_n) t = 1;
+  if ((int64_t)t > n) t = (unsigned)std::max<int64_t>(n, 1);
   std::vector<std::thread> th;
   for (unsigned k = 1; k < t; ++k)
     th.emplace_back(f, n * k / t, n * (k + 1) / t);
@@ -412,7 +414,6 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
     int64_t mra = 0;
     int worst = 0;
     c.maxrb = 0;
-    double reg_atoms = 0.0, reg_tiles = 0.0;
     if (c.balance) c.cuts.resize((size_t)nsys);
     for (int k = 0; k < nsys; ++k) {
       const Grid g = grid_of(h, prec, k);
@@ -430,12 +431,6 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
         mra = m;
         worst = k;
       }
-      const double vol = g.L[0] * g.L[1] * g.L[2];
-      double reg = (double)S.n / vol;
-      for (int a = 0; a < 3; ++a) reg *= ((double)g.nb[a] / g.T[a] + 2.0) * g.bl;
-      const double nt = (double)g.T[0] * g.T[1] * g.T[2];
-      reg_atoms += reg * nt;
-      reg_tiles += nt;
     }
     const double slack =
         (h->p.cap_slack_pct > 0 ? h->p.cap_slack_pct : 15) / 100.0;
@@ -454,7 +449,38 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
     }
     if (ok) {
       c.cap = (int)cap;
+      // a batch's maps from a tile edge: every system's map coarsened as
+      // far as the launch's largest region (atoms and blocks) allows
+      if (c.edge > 0 && nsys > 1) {
+        host_parallel_for(nsys, [&](int64_t k0, int64_t k1) {
+          for (int64_t k = k0; k < k1; ++k) {
+            const SysHost &S = h->sys[k];
+            const int *sc = cnt + S.block0;
+            Grid g = grid_of(h, prec, (int)k);
+            coarsen_map(
+                g,
+                [&](const Grid &t) {
+                  if (c.balance) return balanced_cuts(t, sc, S.n);
+                  Cuts b;
+                  for (int a = 0; a < 3; ++a) b[a] = even_cuts(t.nb[a], t.T[a]);
+                  return b;
+                },
+                sc, mra, c.maxrb);
+            for (int a = 0; a < 3; ++a) c.T[k][a] = g.T[a];
+            if (c.balance) c.cuts[k] = balanced_cuts(g, sc, S.n);
+          }
+        }, 1);
+      }
       // about three work items per lane: run groups from the mean region
+      double reg_atoms = 0.0, reg_tiles = 0.0;
+      for (int k = 0; k < nsys; ++k) {
+        const Grid g = grid_of(h, prec, k);
+        double reg = (double)h->sys[k].n / (g.L[0] * g.L[1] * g.L[2]);
+        for (int a = 0; a < 3; ++a) reg *= ((double)g.nb[a] / g.T[a] + 2.0) * g.bl;
+        const double nt = (double)g.T[0] * g.T[1] * g.T[2];
+        reg_atoms += reg * nt;
+        reg_tiles += nt;
+      }
       const double reg = reg_tiles > 0.0 ? reg_atoms / reg_tiles : 1.0;
       int ng = c.fixed_groups > 0
                    ? c.fixed_groups
@@ -509,6 +535,11 @@ int repartition(examd_handle *h, int prec, bool *changed) {
   for (int64_t b = 0; b < h->nblocks; ++b) cnt[b] = start[b + 1] - start[b];
   const TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
+  // maps from a tile edge are planned from the edge again, as set_config
+  // plans them
+  if (c.edge > 0)
+    for (int k = 0; k < h->nsys; ++k)
+      c.T[k] = tiles_from_edge(h->sys[k].g, (double)c.edge);
   int64_t mra = 0;
   if (choose_caps_counts(h, prec, cnt.data(), &mra, false) != EXAMD_OK) {
     c = saved;

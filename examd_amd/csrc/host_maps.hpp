@@ -1,6 +1,7 @@
 // host_maps.hpp -- host-side arithmetic of the cell-block maps: the block
 // of a position, balanced boundaries from block counts, and the largest
-// region (a cell block plus one block on each side, periodic) of a map.
+// region (a cell block plus one block on each side, periodic) of a map, and
+// the coarsening of a map under limits on that region.
 // Plain C++ (no HIP), so that it compiles with a host compiler alone; the
 // grid type G needs int nb[3] (blocks per axis) and int T[3] (cell blocks
 // per axis).
@@ -128,6 +129,41 @@ int64_t max_region_atoms(const G &g, const Cuts &cb, const int *cnt) {
       for (int64_t v : C) best = std::max(best, v);
     }
   return best;
+}
+
+// Coarsen a map while its largest region stays within a launch's limits:
+// take one cell block off the axis whose cell blocks are the thinnest (ties
+// to the lower axis), else off the next thinnest, until no single axis can
+// give one up.  A larger cell block owns a larger share of its region.
+// cuts_for(g): the boundaries of the map g.T (even or balanced); the limits
+// are the atoms and the blocks of the largest region (a launch sizes its LDS
+// for both) and 255 blocks per axis of a region.
+template <class G, class F>
+void coarsen_map(G &g, F cuts_for, const int *cnt, int64_t max_atoms,
+                 int max_blocks) {
+  for (;;) {
+    int order[3] = {0, 1, 2};
+    std::stable_sort(order, order + 3, [&](int p, int q) {
+      return (int64_t)g.nb[p] * g.T[q] < (int64_t)g.nb[q] * g.T[p];
+    });
+    bool merged = false;
+    for (int a : order) {
+      if (g.T[a] <= 1) continue;
+      G t = g;
+      t.T[a] -= 1;
+      const Cuts cb = cuts_for(t);
+      bool ok = max_region_blocks(cb) <= max_blocks;
+      for (int b = 0; b < 3 && ok; ++b)
+        for (size_t i = 0; i + 1 < cb[b].size(); ++i)
+          if (cb[b][i + 1] - cb[b][i] + 2 > 255) ok = false;
+      if (ok && max_region_atoms(t, cb, cnt) <= max_atoms) {
+        g = t;
+        merged = true;
+        break;
+      }
+    }
+    if (!merged) return;
+  }
 }
 
 }  // namespace examd

@@ -231,6 +231,9 @@ typedef struct examd_tune {
                              16; the autotuner tries 0, 4 and 8 */
     int32_t tile_edge;    /* set_config: when tiles are 0, every system's map
                              is T = max(1, round(nblocks_xyz / tile_edge));
+                             in a batch each map is then coarsened while its
+                             largest region stays within the batch's largest
+                             (atoms and blocks);
                              get_config: the edge the map came from (0 when
                              tiles were given) */
     int32_t workgroups;   /* persistent force grid: workgroups that loop over

@@ -222,17 +222,30 @@ def test_balanced_map_matches_oracle(E, oracle, prec):
 
 
 def test_balanced_batch_equals_standalone(E):
-    """Balanced maps in a batch give each system the map it gets alone:
-    mixed forces bit-identical."""
+    """Balanced maps from a tile edge in a batch: each system's map is
+    coarsened as far as the batch's largest region allows (some systems get
+    fewer cell blocks than from the edge alone), and its mixed forces are
+    bit-identical to the system alone under the same map."""
     print("[balanced batch]", flush=True)
     sy = [_small_slab(E, seed=s) for s in (4, 5)] + _systems(E)[:2]
     with _batch(E, sy) as b:
         b.set_config("mixed", block_threads=256, tile_edge=4, balance=1)
         fb, _, _ = b.forces("mixed")
         off = b.offsets
+        cuts = [b.get_cuts("mixed", k) for k in range(len(sy))]
+    coarser = 0
     for k, (pos, vel, box) in enumerate(sy):
+        T = tuple(len(c) - 1 for c in cuts[k])
         with E.MidpointLJ(len(pos), box, 2.5) as m:
             m.set_state(pos, vel)
             m.set_config("mixed", block_threads=256, tile_edge=4, balance=1)
+            T_edge = tuple(len(c) - 1 for c in m.get_cuts("mixed"))
+            m.set_config("mixed", T, 256, balance=1)
+            for a in range(3):
+                np.testing.assert_array_equal(m.get_cuts("mixed")[a],
+                                              cuts[k][a])
             f1, _, _ = m.forces("mixed")
+        assert all(t <= te for t, te in zip(T, T_edge)), (k, T, T_edge)
+        coarser += T != T_edge
         assert np.array_equal(fb[off[k]:off[k + 1]], f1), k
+    assert coarser > 0

@@ -8,7 +8,9 @@ compiled with the host compiler alone:
   the even split for uniform counts, and every cell block within one block
   slab of an equal share of the atoms;
 - the block of a position (exaMD_identify_block: x/bl + 1 truncated, clamped
-  to the grid) on exact faces, one ulp below them, and past the last face.
+  to the grid) on exact faces, one ulp below them, and past the last face;
+- the coarsening of a batch system's map: within the limits on the largest
+  region (atoms, blocks), and no axis can give up one more cell block.
 """
 import os
 import shutil
@@ -51,6 +53,17 @@ int main() {
         for (int v : cb[a]) std::printf("%d ", v);
         std::printf("\n");
       }
+    } else if (mode == 3) {     // coarsened balanced map: limits, counts
+      long long lim_atoms;
+      int lim_blocks;
+      std::scanf("%lld %d", &lim_atoms, &lim_blocks);
+      std::vector<int> cnt(nblk);
+      long long n = 0;
+      for (int &v : cnt) { std::scanf("%d", &v); n += v; }
+      examd::coarsen_map(
+          g, [&](const G &t) { return examd::balanced_cuts(t, cnt.data(), n); },
+          cnt.data(), lim_atoms, lim_blocks);
+      std::printf("%d %d %d\n", g.T[0], g.T[1], g.T[2]);
     } else {                    // block of positions: bl, count, values
       double bl;
       int m;
@@ -169,3 +182,44 @@ def test_block_of_position_faces(driver):
     out = driver(f"1\n2 {nb} {nb} {nb} 1 1 1\n{bl!r} {len(xs)} " +
                  " ".join(repr(float(x)) for x in xs) + "\n")
     assert [int(v) for v in out[0].split()] == want
+
+
+def _balanced(driver, nb, T, cnt):
+    out = driver(f"1\n{' '.join(map(str, [1, *nb, *T]))}\n" +
+                 " ".join(map(str, cnt.ravel())) + "\n")
+    return [[int(v) for v in out[a].split()] for a in range(3)]
+
+
+def _region_blocks(cuts):
+    return int(np.prod([max(c[t + 1] - c[t] + 2 for t in range(len(c) - 1))
+                        for c in cuts]))
+
+
+def test_coarsened_map_within_limits_and_maximal(driver):
+    rng = np.random.default_rng(9)
+    coarsened = 0
+    for k in range(12):
+        nb = [int(v) for v in rng.integers(6, 16, 3)]
+        T0 = [int(rng.integers(2, n // 2 + 1)) for n in nb]
+        cnt = rng.integers(0, 4, (nb[2], nb[1], nb[0]))
+        if k % 2:
+            cnt[:, :, : nb[0] // 2] *= 3                      # a dense half
+        c0 = _balanced(driver, nb, T0, cnt)
+        # limits a little above the starting map's own largest region
+        lim_a = int(_region_max(cnt, c0) * rng.uniform(1.0, 1.6))
+        lim_b = int(_region_blocks(c0) * rng.uniform(1.0, 1.6))
+        out = driver(f"1\n{' '.join(map(str, [3, *nb, *T0]))}\n{lim_a} {lim_b}\n" +
+                     " ".join(map(str, cnt.ravel())) + "\n")
+        T = [int(v) for v in out[0].split()]
+        assert all(1 <= t <= t0 for t, t0 in zip(T, T0))
+        coarsened += T != T0
+        c = _balanced(driver, nb, T, cnt)
+        assert _region_max(cnt, c) <= lim_a and _region_blocks(c) <= lim_b
+        for a in range(3):
+            if T[a] > 1:
+                Tm = list(T)
+                Tm[a] -= 1
+                cm = _balanced(driver, nb, Tm, cnt)
+                assert (_region_max(cnt, cm) > lim_a or
+                        _region_blocks(cm) > lim_b), (k, a, T)
+    assert coarsened >= 6
