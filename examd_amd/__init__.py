@@ -387,7 +387,9 @@ class _Handle:
         the map from the current block counts that often (in steps, at
         captured-chunk boundaries).  Takes the dict get_config() and
         autotune() return: force_ms is ignored, and a map that came from a
-        tile edge is derived from the edge again."""
+        tile edge is derived from the edge again (in a batch, each system's
+        map is then coarsened while its largest region stays within the
+        batch's largest)."""
         if int(tile_edge) > 0:
             tiles = (0, 0, 0)
         t = Tune()
