@@ -221,18 +221,26 @@ def test_balanced_map_matches_oracle(E, oracle, prec):
         assert _rel(f, f0) < 2e-5
 
 
-def test_balanced_batch_equals_standalone(E):
+def test_balanced_batch_equals_standalone(E, oracle):
     """Balanced maps from a tile edge in a batch: each system's map is
     coarsened as far as the batch's largest region allows (some systems get
-    fewer cell blocks than from the edge alone), and its mixed forces are
-    bit-identical to the system alone under the same map."""
+    fewer cell blocks than from the edge alone), its mixed forces are
+    bit-identical to the system alone under the same map, and its FP64
+    forces under the coarsened map match the oracle (1e-10, pair counts
+    exact)."""
     print("[balanced batch]", flush=True)
     sy = [_small_slab(E, seed=s) for s in (4, 5)] + _systems(E)[:2]
     with _batch(E, sy) as b:
         b.set_config("mixed", block_threads=256, tile_edge=4, balance=1)
+        b.set_config("fp64", block_threads=256, tile_edge=4, balance=1)
         fb, _, _ = b.forces("mixed")
+        f64, s64, _ = b.forces("fp64")
         off = b.offsets
         cuts = [b.get_cuts("mixed", k) for k in range(len(sy))]
+    for k, (pos, _, box) in enumerate(sy):
+        ref, t, _ = oracle.forces(pos, box, 2.5)
+        assert _rel(f64[off[k]:off[k + 1]], ref) < 1e-10, k
+        assert (s64[k].pairs_cand, s64[k].pairs_cut) == (t.cand, t.cut), k
     coarser = 0
     for k, (pos, vel, box) in enumerate(sy):
         T = tuple(len(c) - 1 for c in cuts[k])
