@@ -119,8 +119,11 @@ int flags_to_status(unsigned long long f) {
   if (f & kFlagOutOfBox) return EXAMD_ERR_OUT_OF_BOX;
   if (f & kFlagNonFinite) return EXAMD_ERR_NONFINITE;
   if (f & kFlagOverflow) return EXAMD_ERR_TILE_OVERFLOW;
-  if (f & kFlagZero) return EXAMD_ERR_ZERO_DISTANCE;
+  // a pair beyond the fixed-point range makes the mixed forces invalid; a
+  // 'ZERO at' pair alone is reported and the forces kept, as the reference
+  // prints and goes on
   if (f & kFlagFixedRange) return EXAMD_ERR_FIXED_RANGE;
+  if (f & kFlagZero) return EXAMD_ERR_ZERO_DISTANCE;
   if (f & kFlagTableRange) return EXAMD_ERR_TABLE_RANGE;
   return EXAMD_OK;
 }
@@ -214,6 +217,8 @@ SysTable table(const examd_handle *h, int prec) {
   return t;
 }
 
+int ensure_partials(examd_handle *h);
+
 // Upload the system table of one precision (after its tile maps changed).
 int upload_tables(examd_handle *h, int prec) {
   drop_graph(h);   // a captured chunk holds the old tables
@@ -270,7 +275,9 @@ int upload_tables(examd_handle *h, int prec) {
                      hipMemcpyHostToDevice));
   }
   t.ntiles = tiles;
-  return EXAMD_OK;
+  // the force kernel writes one partial tally per tile: every path that
+  // changes the maps comes through here, so the slots are sized here
+  return ensure_partials(h);
 }
 
 // Partial-tally slots for the current tile maps.
@@ -409,6 +416,12 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
   const bool fp64 = prec == EXAMD_PREC_FP64;
   const bool typed = h->d.ntypes > 0;
   const int nsys = h->nsys;
+  // maps from a tile edge are planned from the edge every time (a batch's
+  // coarsened maps would otherwise only ever get coarser), so the result
+  // depends on the counts alone, not on the calls before
+  if (c.edge > 0)
+    for (int k = 0; k < nsys; ++k)
+      c.T[k] = tiles_from_edge(h->sys[k].g, (double)c.edge);
   for (int iter = 0; iter < 4096; ++iter) {
     bool ok = true;
     int64_t mra = 0;
@@ -535,11 +548,6 @@ int repartition(examd_handle *h, int prec, bool *changed) {
   for (int64_t b = 0; b < h->nblocks; ++b) cnt[b] = start[b + 1] - start[b];
   const TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
-  // maps from a tile edge are planned from the edge again, as set_config
-  // plans them
-  if (c.edge > 0)
-    for (int k = 0; k < h->nsys; ++k)
-      c.T[k] = tiles_from_edge(h->sys[k].g, (double)c.edge);
   int64_t mra = 0;
   if (choose_caps_counts(h, prec, cnt.data(), &mra, false) != EXAMD_OK) {
     c = saved;
@@ -1053,22 +1061,29 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
   }
   {
     // both precisions' maps planned on two threads (each writes only its
-    // own TileConfig), then their tables uploaded here
+    // own TileConfig), then their tables uploaded here.  A plan that fails
+    // leaves both maps as they were (the device tables and the state on the
+    // device still agree with them).
     const std::vector<int> cnt = host_counts(h, pos);
+    const TileConfig saved[2] = {h->tc[0], h->tc[1]};
     int st1 = EXAMD_OK;
     std::thread other([&] {
       st1 = choose_caps_counts(h, 1, cnt.data(), nullptr, false);
     });
     const int st0 = choose_caps_counts(h, 0, cnt.data(), nullptr, false);
     other.join();
-    const int planned[2] = {st0, st1};
-    for (int prec = 0; prec < 2; ++prec) {
-      if (planned[prec] != EXAMD_OK) continue;
-      const int st = upload_tables(h, prec);
-      if (st) return st;
+    if (st0 != EXAMD_OK || st1 != EXAMD_OK) {
+      h->tc[0] = saved[0];
+      h->tc[1] = saved[1];
+      return st0 != EXAMD_OK ? st0 : st1;
     }
-    if (st0) return st0;
-    if (st1) return st1;
+    for (int prec = 0; prec < 2; ++prec) {
+      const int st = upload_tables(h, prec);
+      if (st) {
+        h->state_set = false;
+        return st;
+      }
+    }
   }
   if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
   if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
@@ -1146,13 +1161,18 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
   const int old_ntypes = d.ntypes;
   if (h->state_set && (ntypes > 0) != (old_ntypes > 0)) {
     std::vector<double> pos;
+    const TileConfig saved[2] = {h->tc[0], h->tc[1]};
     int st = download_positions(h, pos);
     d.ntypes = ntypes;
     for (int prec = 0; st == EXAMD_OK && prec < 2; ++prec)
       st = choose_caps(h, prec, pos.data());
     if (st != EXAMD_OK) {
+      // the maps and device tables as they were before this call
       d.ntypes = old_ntypes;
-      for (int prec = 0; prec < 2; ++prec) (void)choose_caps(h, prec, pos.data());
+      for (int prec = 0; prec < 2; ++prec) {
+        h->tc[prec] = saved[prec];
+        (void)upload_tables(h, prec);
+      }
       return st;
     }
   }

@@ -687,6 +687,15 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   const int RX = tx + 2, RY = ty + 2, RZ = tz + 2;
   const int RXY = RX * RY, NRB = RXY * RZ;
   const int nbx = g.nb[0], nby = g.nb[1], nbz = g.nb[2];
+  // the LDS image holds maxrb region blocks: a map that does not match the
+  // launch's tables is refused here, before the region scan writes s_off
+  if (NRB > A.maxrb) {
+    if (tid == 0) {
+      atomicOr(&log->flags, (unsigned long long)kFlagOverflow);
+      if (ENERGY) A.d.part[tile] = Partial{0.0, 0.0, 0ull, 0ull, 0ull, 0ull};
+    }
+    return;
+  }
 
   const LdsLayout LL = lds_layout(FP64, A.cap, A.maxrb, NW, A.lane_queue, TYPED);
   // several species: species of each region atom, species-pair rows
