@@ -38,6 +38,19 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
   return v;
 }
 
+// Sum over the 64 lanes of a wave (every lane active), the same in every
+// lane: an inclusive scan by shifts of 1, 2, 4 and 8 inside each row of 16
+// lanes and row broadcasts of lanes 15 and 31, then lane 63's value.
+__device__ __forceinline__ int wave_total(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ __forceinline__ void add_partial(Partial &a, const Partial &b) {
   a.ke += b.ke;
   a.pe += b.pe;
@@ -483,6 +496,8 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
 // atom (masked off).
 constexpr int kWorkBins = 64;   // owned J blocks of an atom: 0..63
 constexpr int kMixedChunk = 16; // mixed candidates per step
+constexpr int kDrainMin = 2;    // partial drain once the wave's lanes hold
+                                // this many pending pairs each on average
 constexpr int kChunkPad = 16;   // atom entries past the last one a chunk may read
 struct LdsLayout {
   size_t off_gst, off_wave, off_pos, off_f, off_id, off_bc, off_perm, off_hist,
@@ -931,7 +946,13 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     const int sax = bc_s(bc, 0), say = bc_s(bc, 1), saz = bc_s(bc, 2);
     double fax = 0.0, fay = 0.0, faz = 0.0;
     long long qax = 0, qay = 0, qaz = 0;
-    int qn = 0;   // queued pairs of this lane
+    // lane queue: a ring of QD entries from qh, the entry being consumed
+    // in (cjb, cm), and the pairs pending in both
+    int qn = 0;
+    int qh = 0;
+    int cjb = 0;
+    unsigned cm = 0u;
+    int pend = 0;
     // several species: first row of atom a's species in the pair table
     const int trow = TYPED ? (int)s_ty[a] * A.d.ntypes : 0;
 
@@ -1097,36 +1118,47 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     // empty the queues of the wave's lanes together, one pair per lane per
     // step: a lane takes its next entry when the hit bits of the current
     // one are used up
-    auto drain = [&]() {
-      int e = 0, jb = 0;
-      unsigned m = 0;
-      while (__ballot(m != 0u || e < qn)) {
-        if (m == 0u && e < qn) {
-          const unsigned v = my_q[e * 64];
-          jb = (int)(v >> kChunk);
-          m = v & kChunkMask;
-          ++e;
-        }
-        if (m) {
-          const int k = __builtin_ctz(m);
-          m &= m - 1u;
-          eval_queued(jb + k);
-        }
+    auto drain_step = [&]() {
+      if (cm == 0u && qn > 0) {
+        const unsigned v = my_q[qh * 64];
+        if (++qh == QD) qh = 0;
+        --qn;
+        cjb = (int)(v >> kChunk);
+        cm = v & kChunkMask;
       }
-      qn = 0;
+      if (cm) {
+        const int k = __builtin_ctz(cm);
+        cm &= cm - 1u;
+        --pend;
+        eval_queued(cjb + k);
+      }
     };
-    // defer the hits among the chunk at j as one queue entry; a full queue
-    // (rare) is emptied by its own lane
+    // every pending pair of the wave's lanes
+    auto drain = [&]() {
+      while (__ballot(cm != 0u || qn > 0)) drain_step();
+    };
+    // K steps: about the mean pending count, so that nearly every lane
+    // has a pair in each step; what remains stays queued
+    auto drain_some = [&](int K) {
+      for (int it = 0; it < K; ++it) drain_step();
+    };
+    // defer the hits among the chunk at j as one queue entry; a full ring
+    // (rare) first has its oldest entry evaluated by its own lane
     auto push = [&](int j, unsigned hits) {
-      my_q[qn * 64] = ((unsigned)j << kChunk) | hits;
-      if (++qn == QD) {
-        for (int e = 0; e < qn; ++e) {
-          const unsigned v = my_q[e * 64];
-          for (unsigned m = v & kChunkMask; m; m &= m - 1u)
-            eval_queued((int)(v >> kChunk) + __builtin_ctz(m));
+      if (qn == QD) {
+        const unsigned v = my_q[qh * 64];
+        if (++qh == QD) qh = 0;
+        --qn;
+        for (unsigned m = v & kChunkMask; m; m &= m - 1u) {
+          --pend;
+          eval_queued((int)(v >> kChunk) + __builtin_ctz(m));
         }
-        qn = 0;
       }
+      int tl = qh + qn;
+      if (tl >= QD) tl -= QD;
+      my_q[tl * 64] = ((unsigned)j << kChunk) | hits;
+      ++qn;
+      pend += __builtin_popcount(hits);
     };
 
     const int r0 = kGroupStart[NG][grp], r1 = kGroupStart[NG][grp + 1];
@@ -1226,7 +1258,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         }
       }
       // runs end together: empty the queues when one is half full
-      if (LQ && __ballot(qn >= QD / 2)) drain();
+      if (LQ) {
+        const int P = wave_total(pend);
+        if (P >= 64 * kDrainMin) drain_some(P >> 6);
+      }
     }
     if (LQ) drain();
     if (active) {
