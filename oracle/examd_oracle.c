@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <pthread.h>
 
 /* SRC/exaMD_utils.f90:167-190: nblocks_xyz = Int(globalbox/block_length),
  * globalboxi = one/globalbox. */
@@ -196,12 +197,10 @@ static inline void or_particle_interaction(const or_domain *d, const or_lj *p,
 /* SRC/exaMD_forces.f90:308-390 on one rank: every block is interior
  * (SRC/exaMD_partitioning.f90:735-746) and interior_blks is filled with
  * k outermost, i fastest (:763-781). */
-static int forces_midpoint(const or_domain *d, const or_lj *p,
-                           const or_species *sp, const or_table *tab, int64_t n,
-                           const double *pos, double *forces,
-                           int64_t *home_counts, or_tally *t)
+/* Checks of the arguments shared by the serial and threaded forms. */
+static int forces_check(const or_domain *d, const or_lj *p,
+                        const or_species *sp, int64_t n)
 {
-    memset(t, 0, sizeof(*t));
     if (sp) {
         if (sp->ntypes < 1) return 2;
         for (int32_t a = 0; a < sp->ntypes * sp->ntypes; ++a)
@@ -211,24 +210,26 @@ static int forces_midpoint(const or_domain *d, const or_lj *p,
     }
     for (int a = 0; a < 3; ++a)
         if (d->nb[a] < 5 || !(p->rc < 0.5 * d->box[a])) return 4;
-    int64_t nbk = d->nblocks;
-    int64_t *start = (int64_t *)malloc((size_t)(nbk + 1) * sizeof(int64_t));
-    int64_t *order = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
-    int64_t *bid = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
-    if (!start || !order || !bid) {
-        free(start); free(order); free(bid);
-        return 3;
-    }
-    int rc = or_bin(d, n, pos, start, order, bid);
-    if (rc) {
-        free(start); free(order); free(bid);
-        return rc;
-    }
+    return 0;
+}
+
+/* SRC/exaMD_forces.f90:308-390 for the home blocks of the z layers
+ * [k0, k1) (1-based), in the reference's order: k outermost, i fastest
+ * (the interior_blks order of SRC/exaMD_partitioning.f90:763-781), the
+ * block's own triangle (:349-356), then table rows 2..63 (:359-378) over
+ * the periodic neighbour references (neighrefs, :786-798). */
+static void home_layers(const or_domain *d, const or_lj *p,
+                        const or_species *sp, const or_table *tab,
+                        const double *pos, double *forces,
+                        int64_t *home_counts, or_tally *t,
+                        const int64_t *start, const int64_t *order,
+                        int32_t k0, int32_t k1)
+{
     const int32_t nx = d->nb[0], ny = d->nb[1], nz = d->nb[2];
     int32_t offs[27][3];
     for (int s = 0; s < 27; ++s) or_stencil_offset(s + 1, offs[s]);
 
-    for (int32_t k = 1; k <= nz; ++k)
+    for (int32_t k = k0; k < k1; ++k)
         for (int32_t j = 1; j <= ny; ++j)
             for (int32_t i = 1; i <= nx; ++i) {
                 int64_t home = (int64_t)(i - 1) + (int64_t)(j - 1) * nx
@@ -261,6 +262,97 @@ static int forces_midpoint(const or_domain *d, const or_lj *p,
                                                     home_counts, t);
                 }
             }
+}
+
+/* One z-slab of home layers per task of a phase of the threaded form. */
+typedef struct slab_task {
+    const or_domain *d;
+    const or_lj *p;
+    const or_species *sp;
+    const or_table *tab;
+    const double *pos;
+    double *forces;
+    int64_t *home_counts;
+    const int64_t *start, *order;
+    int32_t k0, k1;
+    or_tally t;
+} slab_task;
+
+static void *slab_run(void *arg)
+{
+    slab_task *s = (slab_task *)arg;
+    memset(&s->t, 0, sizeof(s->t));
+    home_layers(s->d, s->p, s->sp, s->tab, s->pos, s->forces, s->home_counts,
+                &s->t, s->start, s->order, s->k0, s->k1);
+    return NULL;
+}
+
+/* The evaluation over nthreads threads.  The z layers are cut into an even
+ * number of slabs at least two layers thick, and the slabs are run in two
+ * phases (even, then odd slabs), every slab of a phase on its own thread.
+ * A home block touches atoms of its own and the adjacent layers only, so
+ * two slabs of a phase never touch the same atom and no force is written
+ * by two threads at once.  The pairs and the arithmetic of each are the
+ * serial form's; an atom near a slab face receives its sums in another
+ * order (FP64 differences of a few ulps). */
+static int forces_midpoint(const or_domain *d, const or_lj *p,
+                           const or_species *sp, const or_table *tab, int64_t n,
+                           const double *pos, double *forces,
+                           int64_t *home_counts, or_tally *t, int nthreads)
+{
+    memset(t, 0, sizeof(*t));
+    int rc = forces_check(d, p, sp, n);
+    if (rc) return rc;
+    int64_t nbk = d->nblocks;
+    int64_t *start = (int64_t *)malloc((size_t)(nbk + 1) * sizeof(int64_t));
+    int64_t *order = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    int64_t *bid = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    if (!start || !order || !bid) {
+        free(start); free(order); free(bid);
+        return 3;
+    }
+    rc = or_bin(d, n, pos, start, order, bid);
+    if (rc) {
+        free(start); free(order); free(bid);
+        return rc;
+    }
+    const int32_t nz = d->nb[2];
+    int nslab = 2 * (nthreads < nz / 4 ? nthreads : nz / 4);
+    if (nthreads <= 1 || nslab < 2) {
+        home_layers(d, p, sp, tab, pos, forces, home_counts, t, start, order,
+                    1, nz + 1);
+    } else {
+        slab_task *task = (slab_task *)calloc((size_t)nslab, sizeof(slab_task));
+        pthread_t *th = (pthread_t *)calloc((size_t)nslab, sizeof(pthread_t));
+        if (!task || !th) {
+            free(task); free(th); free(start); free(order); free(bid);
+            return 3;
+        }
+        for (int s = 0; s < nslab; ++s) {
+            slab_task *k = &task[s];
+            k->d = d; k->p = p; k->sp = sp; k->tab = tab; k->pos = pos;
+            k->forces = forces; k->home_counts = home_counts;
+            k->start = start; k->order = order;
+            k->k0 = 1 + (int32_t)((int64_t)s * nz / nslab);
+            k->k1 = 1 + (int32_t)((int64_t)(s + 1) * nz / nslab);
+        }
+        for (int phase = 0; phase < 2; ++phase) {
+            for (int s = phase; s < nslab; s += 2)
+                if (pthread_create(&th[s], NULL, slab_run, &task[s]) != 0)
+                    slab_run(&task[s]), th[s] = 0;
+            for (int s = phase; s < nslab; s += 2)
+                if (th[s]) pthread_join(th[s], NULL);
+        }
+        for (int s = 0; s < nslab; ++s) {
+            t->cand += task[s].t.cand;
+            t->cut += task[s].t.cut;
+            t->zero += task[s].t.zero;
+            t->pe += task[s].t.pe;
+            t->below += task[s].t.below;
+        }
+        free(task);
+        free(th);
+    }
     free(start);
     free(order);
     free(bid);
@@ -271,7 +363,7 @@ int or_forces_midpoint_all(const or_domain *d, const or_lj *p, int64_t n,
                            const double *pos, double *forces,
                            int64_t *home_counts, or_tally *t)
 {
-    return forces_midpoint(d, p, NULL, NULL, n, pos, forces, home_counts, t);
+    return forces_midpoint(d, p, NULL, NULL, n, pos, forces, home_counts, t, 1);
 }
 
 int or_forces_midpoint_species(const or_domain *d, const or_lj *p,
@@ -279,7 +371,7 @@ int or_forces_midpoint_species(const or_domain *d, const or_lj *p,
                                const double *pos, double *forces,
                                int64_t *home_counts, or_tally *t)
 {
-    return forces_midpoint(d, p, sp, NULL, n, pos, forces, home_counts, t);
+    return forces_midpoint(d, p, sp, NULL, n, pos, forces, home_counts, t, 1);
 }
 
 int or_forces_midpoint_table(const or_domain *d, const or_lj *p,
@@ -288,7 +380,7 @@ int or_forces_midpoint_table(const or_domain *d, const or_lj *p,
                              int64_t *home_counts, or_tally *t)
 {
     if (!tab || tab->n < 2) return 2;
-    return forces_midpoint(d, p, NULL, tab, n, pos, forces, home_counts, t);
+    return forces_midpoint(d, p, NULL, tab, n, pos, forces, home_counts, t, 1);
 }
 
 double or_wrap(double x, double L)
@@ -298,12 +390,89 @@ double or_wrap(double x, double L)
     return x;
 }
 
+/* The velocity-Verlet sweeps over atoms [i0, i1): the opening half kick
+ * and drift with the wrap (mode 0), or the closing half kick with the sum
+ * of v^2 (mode 1). */
+typedef struct vv_task {
+    const or_domain *d;
+    int64_t i0, i1;
+    double h, dt;
+    double *pos, *vel, *forces;
+    int mode;
+    double v2;
+} vv_task;
+
+static void *vv_run(void *arg)
+{
+    vv_task *k = (vv_task *)arg;
+    k->v2 = 0.0;
+    for (int64_t i = 3 * k->i0; i < 3 * k->i1; ++i) {
+        int a = (int)(i % 3);
+        if (k->mode == 0) {
+            k->vel[i] = k->vel[i] + k->h * k->forces[i];
+            k->pos[i] = k->pos[i] + k->dt * k->vel[i];
+            k->pos[i] = or_wrap(k->pos[i], k->d->box[a]);
+            k->forces[i] = 0.0;
+        } else {
+            k->vel[i] = k->vel[i] + k->h * k->forces[i];
+            k->v2 = k->v2 + k->vel[i] * k->vel[i];
+        }
+    }
+    return NULL;
+}
+
+/* One sweep on up to 64 threads; returns the sum of v^2 in thread order. */
+static double vv_sweep(const or_domain *d, int64_t n, double h, double dt,
+                       double *pos, double *vel, double *forces, int mode,
+                       int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    vv_task k[64];
+    pthread_t th[64];
+    for (int t = 0; t < nthreads; ++t) {
+        k[t] = (vv_task){d, n * t / nthreads, n * (t + 1) / nthreads, h, dt,
+                         pos, vel, forces, mode, 0.0};
+        th[t] = 0;
+        if (t > 0 && pthread_create(&th[t], NULL, vv_run, &k[t]) != 0) th[t] = 0;
+        if (t > 0 && !th[t]) vv_run(&k[t]);
+    }
+    vv_run(&k[0]);
+    double v2 = 0.0;
+    for (int t = 0; t < nthreads; ++t) {
+        if (th[t]) pthread_join(th[t], NULL);
+        v2 = v2 + k[t].v2;
+    }
+    return v2;
+}
+
+static int run_vv_impl(const or_domain *d, const or_lj *p, const or_species *sp,
+                       int64_t n, double mass, double dt, int64_t nsteps,
+                       double *pos, double *vel, double *forces, double *ke_out,
+                       double *pe_out, int64_t *cut_out, int nthreads)
+{
+    double h = (0.5 * dt) / mass;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        vv_sweep(d, n, h, dt, pos, vel, forces, 0, nthreads);
+        or_tally t;
+        int rc = forces_midpoint(d, p, sp, NULL, n, pos, forces, NULL, &t,
+                                 nthreads);
+        if (rc) return rc;
+        double ke = 0.5 * mass * vv_sweep(d, n, h, dt, pos, vel, forces, 1,
+                                          nthreads);
+        if (ke_out) ke_out[s] = ke;
+        if (pe_out) pe_out[s] = t.pe;
+        if (cut_out) cut_out[s] = t.cut;
+    }
+    return 0;
+}
+
 int or_run_vv(const or_domain *d, const or_lj *p, int64_t n, double mass,
               double dt, int64_t nsteps, double *pos, double *vel,
               double *forces, double *ke_out, double *pe_out, int64_t *cut_out)
 {
-    return or_run_vv_species(d, p, NULL, n, mass, dt, nsteps, pos, vel,
-                             forces, ke_out, pe_out, cut_out);
+    return run_vv_impl(d, p, NULL, n, mass, dt, nsteps, pos, vel, forces,
+                       ke_out, pe_out, cut_out, 1);
 }
 
 int or_run_vv_species(const or_domain *d, const or_lj *p, const or_species *sp,
@@ -311,29 +480,25 @@ int or_run_vv_species(const or_domain *d, const or_lj *p, const or_species *sp,
                       double *pos, double *vel, double *forces, double *ke_out,
                       double *pe_out, int64_t *cut_out)
 {
-    double h = (0.5 * dt) / mass;
-    for (int64_t s = 0; s < nsteps; ++s) {
-        for (int64_t i = 0; i < 3 * n; ++i) {
-            int a = (int)(i % 3);
-            vel[i] = vel[i] + h * forces[i];
-            pos[i] = pos[i] + dt * vel[i];
-            pos[i] = or_wrap(pos[i], d->box[a]);
-        }
-        memset(forces, 0, (size_t)(3 * n) * sizeof(double));
-        or_tally t;
-        int rc = forces_midpoint(d, p, sp, NULL, n, pos, forces, NULL, &t);
-        if (rc) return rc;
-        double ke = 0.0;
-        for (int64_t i = 0; i < 3 * n; ++i) {
-            vel[i] = vel[i] + h * forces[i];
-            ke = ke + vel[i] * vel[i];
-        }
-        ke = 0.5 * mass * ke;
-        if (ke_out) ke_out[s] = ke;
-        if (pe_out) pe_out[s] = t.pe;
-        if (cut_out) cut_out[s] = t.cut;
-    }
-    return 0;
+    return run_vv_impl(d, p, sp, n, mass, dt, nsteps, pos, vel, forces,
+                       ke_out, pe_out, cut_out, 1);
+}
+
+int or_forces_midpoint_mt(const or_domain *d, const or_lj *p, int64_t n,
+                          const double *pos, double *forces,
+                          int64_t *home_counts, or_tally *t, int32_t nthreads)
+{
+    return forces_midpoint(d, p, NULL, NULL, n, pos, forces, home_counts, t,
+                           nthreads);
+}
+
+int or_run_vv_mt(const or_domain *d, const or_lj *p, int64_t n, double mass,
+                 double dt, int64_t nsteps, double *pos, double *vel,
+                 double *forces, double *ke_out, double *pe_out,
+                 int64_t *cut_out, int32_t nthreads)
+{
+    return run_vv_impl(d, p, NULL, n, mass, dt, nsteps, pos, vel, forces,
+                       ke_out, pe_out, cut_out, nthreads);
 }
 
 /* SRC/exaMD_input_formats.f90:215-236: list-directed N, box, then x y z. */

@@ -134,6 +134,21 @@ int  or_run_vv(const or_domain *d, const or_lj *p, int64_t n, double mass,
                double *forces, double *ke_out, double *pe_out,
                int64_t *cut_out);
 
+/* The same evaluation and MD loop on nthreads threads (the CPU baseline
+ * on all cores).  The z layers of home blocks are cut into an even number
+ * of slabs, at least two layers each, run in two phases (even slabs, then
+ * odd) so that no two threads write one atom's force; each slab keeps the
+ * serial loop order, so the pairs and their arithmetic are the serial
+ * form's and only the order of an atom's sums near slab faces differs. */
+int  or_forces_midpoint_mt(const or_domain *d, const or_lj *p, int64_t n,
+                           const double *pos, double *forces,
+                           int64_t *home_counts, or_tally *t,
+                           int32_t nthreads);
+int  or_run_vv_mt(const or_domain *d, const or_lj *p, int64_t n, double mass,
+                  double dt, int64_t nsteps, double *pos, double *vel,
+                  double *forces, double *ke_out, double *pe_out,
+                  int64_t *cut_out, int32_t nthreads);
+
 int  or_run_vv_species(const or_domain *d, const or_lj *p,
                        const or_species *sp, int64_t n, double mass, double dt,
                        int64_t nsteps, double *pos, double *vel,

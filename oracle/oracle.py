@@ -88,6 +88,13 @@ def lib():
                                 C.c_double, C.c_int64, P(C.c_double),
                                 P(C.c_double), P(C.c_double), P(C.c_double),
                                 P(C.c_double), P(C.c_int64)]
+        L.or_forces_midpoint_mt.argtypes = [P(_Domain), P(_LJ), C.c_int64,
+                                            P(C.c_double), P(C.c_double),
+                                            P(C.c_int64), P(_Tally), C.c_int32]
+        L.or_run_vv_mt.argtypes = [P(_Domain), P(_LJ), C.c_int64, C.c_double,
+                                   C.c_double, C.c_int64, P(C.c_double),
+                                   P(C.c_double), P(C.c_double), P(C.c_double),
+                                   P(C.c_double), P(C.c_int64), C.c_int32]
         L.or_wrap.argtypes = [C.c_double, C.c_double]
         L.or_wrap.restype = C.c_double
         L.or_write_particles.argtypes = [C.c_char_p, C.c_int64,
@@ -193,10 +200,12 @@ def table_arrays(table, cutoff):
 
 
 def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
-           species=None, table=None):
+           species=None, table=None, threads=1):
     """One exaMD_forces_midpoint_all evaluation from zeroed forces.  With
     species = (types, eps, sig, rc), every pair uses the parameters of its
-    species pair (cutoff sizes the blocks).
+    species pair (cutoff sizes the blocks).  threads > 1 (one species, no
+    table): the slab-threaded form (same pairs and arithmetic; an atom's
+    sums near slab faces in another order).
 
     Returns (forces (n,3), Tally, home_counts or None)."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
@@ -212,6 +221,10 @@ def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
         rc = lib().or_forces_midpoint_table(
             C.byref(dom._c), C.byref(p), C.byref(tab), n, _dp(pos), _dp(f),
             _ip(hc) if hc is not None else None, C.byref(t))
+    elif species is None and threads > 1:
+        rc = lib().or_forces_midpoint_mt(
+            C.byref(dom._c), C.byref(p), n, _dp(pos), _dp(f),
+            _ip(hc) if hc is not None else None, C.byref(t), int(threads))
     elif species is None:
         rc = lib().or_forces_midpoint_all(
             C.byref(dom._c), C.byref(p), n, _dp(pos), _dp(f),
@@ -235,20 +248,26 @@ def lennardjones_12(dx, dy, dz, d2, cutoff, eps=1.0, sig=1.0):
 
 
 def run_vv(pos, vel, box, cutoff, dt, nsteps, mass=1.0, eps=1.0, sig=1.0,
-           forces0=None, species=None):
-    """Velocity Verlet from (pos, vel); returns (pos, vel, forces, ke, pe, cut)."""
+           forces0=None, species=None, threads=1):
+    """Velocity Verlet from (pos, vel); returns (pos, vel, forces, ke, pe, cut).
+    threads > 1 (one species): the slab-threaded form."""
     pos = np.array(pos, dtype=np.float64, order="C")
     vel = np.array(vel, dtype=np.float64, order="C")
     n = pos.shape[0]
     if forces0 is None:
-        forces0, _, _ = forces(pos, box, cutoff, eps, sig, species=species)
+        forces0, _, _ = forces(pos, box, cutoff, eps, sig, species=species,
+                               threads=threads)
     f = np.array(forces0, dtype=np.float64, order="C")
     dom = domain_create(box, 0.5 * cutoff)
     ke = np.zeros(nsteps)
     pe = np.zeros(nsteps)
     cut = np.zeros(nsteps, dtype=np.int64)
     p = _LJ(eps, sig, cutoff)
-    if species is None:
+    if species is None and threads > 1:
+        rc = lib().or_run_vv_mt(C.byref(dom._c), C.byref(p), n, mass, dt,
+                                nsteps, _dp(pos), _dp(vel), _dp(f), _dp(ke),
+                                _dp(pe), _ip(cut), int(threads))
+    elif species is None:
         rc = lib().or_run_vv(C.byref(dom._c), C.byref(p), n, mass, dt, nsteps,
                              _dp(pos), _dp(vel), _dp(f), _dp(ke), _dp(pe),
                              _ip(cut))

@@ -226,3 +226,34 @@ def table_brute_force(pos, box, rc, table):
         pe += 0.5 * (v[k] + t * (v[k + 1] - v[k])).sum()
         ncut += int(m.sum())
     return f, pe, ncut // 2
+
+
+def candidate_pairs(pos, box, rc):
+    """P_cand of the midpoint stencil from block occupancies alone:
+    sum_b n_b (n_b - 1) / 2 + sum over the 62 offset classes d of the table
+    of sum_b n_b n_{b+d} (periodic)."""
+    box = np.asarray(box, dtype=np.float64)
+    bl = 0.5 * rc
+    nb = (box / bl).astype(np.int64)
+    b = block_of(pos, bl, nb)
+    occ = np.zeros((nb[2], nb[1], nb[0]), dtype=np.int64)
+    np.add.at(occ, (b[:, 2], b[:, 1], b[:, 0]), 1)
+    cand = int((occ * (occ - 1) // 2).sum())
+    for n1, n2 in TABLE[1:]:
+        d = np.array(stencil(n2)) - np.array(stencil(n1))
+        cand += int((occ * np.roll(occ, shift=(-d[2], -d[1], -d[0]),
+                                   axis=(0, 1, 2))).sum())
+    return cand
+
+
+def slab_system(E, natoms, seed, min_sep=0.8, temperature=0.85):
+    """The non-uniform density workload at a given size: box 3:1:1, the
+    central third of x at density 0.80, the rest at 0.05, uniform with a
+    minimum separation (random sequential addition), Maxwell-Boltzmann
+    velocities at T*: (pos, vel, box)."""
+    vol = natoms / (0.80 / 3 + 0.05 * 2 / 3)
+    ly = (vol / 3.0) ** (1.0 / 3.0)
+    box = np.array([3 * ly, ly, ly])
+    pos, vel = E.generate_slab(box, ly, 2 * ly, 0.80, 0.05, min_sep=min_sep,
+                               temperature=temperature, seed=seed)
+    return pos, vel, box

@@ -414,3 +414,27 @@ def test_golden_kats(oracle):
             scale = 24 * k["args"][5] * (k["args"][6] ** 2 / k["args"][3]) ** 3
             assert np.allclose(f, k["f12"], rtol=1e-13, atol=1e-14 * scale)
             assert np.isclose(e, k["e12"], rtol=1e-13, atol=1e-14 * scale)
+
+
+@pytest.mark.parametrize("nc,threads", [(8, 4), (13, 8), (16, 16)])
+def test_threaded_form_matches_serial(oracle, nc, threads):
+    """The slab-threaded evaluation (the all-cores CPU baseline) visits the
+    same pairs with the same arithmetic: pair counts and per-home-block
+    ownership equal, forces and energy within FP64 summation order."""
+    pos, L = H.perturbed_fcc(nc, 0.8442, 0.1, nc)
+    stretch = np.array([1.0, 1.1, 0.95])
+    pos = np.mod(pos, L) * stretch
+    box = L * stretch
+    f1, t1, h1 = oracle.forces(pos, box, 2.5, home_counts=True)
+    fm, tm, hm = oracle.forces(pos, box, 2.5, home_counts=True, threads=threads)
+    assert (t1.cand, t1.cut, t1.zero) == (tm.cand, tm.cut, tm.zero)
+    np.testing.assert_array_equal(h1, hm)
+    scale = np.sqrt((f1 ** 2).sum(1).mean())
+    assert np.abs(f1 - fm).max() < 1e-13 * scale
+    assert abs(t1.pe - tm.pe) < 1e-13 * abs(t1.pe)
+    vel = np.random.default_rng(nc).normal(0.0, 0.8, pos.shape)
+    r1 = oracle.run_vv(pos, vel, box, 2.5, 0.005, 3)
+    rm = oracle.run_vv(pos, vel, box, 2.5, 0.005, 3, threads=threads)
+    assert np.abs(r1[0] - rm[0]).max() < 1e-12
+    assert np.abs(r1[1] - rm[1]).max() < 1e-12
+    np.testing.assert_array_equal(r1[5], rm[5])
