@@ -37,10 +37,14 @@ RHO, TEMP, RC, DT = 0.8442, 0.72, 2.5, 0.005
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles",
                                "k_forces_mixed_2048000_traffic.txt")
 # The force kernel is timed with HIP events around the force launch of every
-# TIME_EVERY-th step (steps 5, 15, 25, ... of each chunk, none of them an
+# TIME_EVERY-th step (steps 2, 7, 12, ... of each chunk, none of them an
 # energy-tally step): each pair of event records costs the stream about
 # 5.6 us, which timing every launch would add to every step.
-TIME_EVERY = 10
+TIME_EVERY = 5
+# Accuracy leg, whatever the timed length: mixed and FP64 steps continued
+# from the timed run's state, energies every 10 steps.
+DRIFT_STEPS = {"mixed": 10000, "fp64": 1000}
+DRIFT_EVERY = 10
 
 
 def algorithmic_flops(cand, cut, energy):
@@ -52,7 +56,7 @@ def drift_per_1k(stats):
     """Least-squares slope of the shifted total energy per 1,000 steps,
     relative to |E|."""
     if len(stats) < 3:
-        return None
+        raise RuntimeError(f"drift needs 3 or more tallies, got {len(stats)}")
     t = np.array([s.step for s in stats], dtype=float)
     e = np.array([s.ke + s.pe_shifted for s in stats])
     slope = np.polyfit(t, e, 1)[0]
@@ -93,23 +97,39 @@ def parse_shape(text):
             "balance": v[7] if len(v) > 7 else 0}
 
 
-def force_error(m):
-    """Per-atom force error of the mixed path against the FP64 path at the
-    handle's current positions: max and RMS of |f_mixed - f_fp64| (epsilon /
-    sigma units) and both relative to the RMS FP64 force."""
+def error_stats(f, ref):
+    """max and RMS of |f - ref| per atom (epsilon / sigma units), and both
+    relative to the RMS reference force."""
+    d = np.linalg.norm(f - ref, axis=1)
+    frms = float(np.sqrt(np.mean(np.sum(ref * ref, axis=1))))
+    return {"max_abs": float(d.max()), "rms_abs": float(np.sqrt(np.mean(d * d))),
+            "rms_force": frms, "max_rel_to_rms_force": float(d.max()) / frms,
+            "rms_rel_to_rms_force": float(np.sqrt(np.mean(d * d))) / frms}
+
+
+def force_errors(m, box, threads):
+    """Per-atom force errors at the handle's current positions: the mixed
+    path against the oracle (the reference's FP64 evaluation restated, run
+    on `threads` host threads), the FP64 path against the oracle, and the
+    mixed path against the FP64 path.  Outside every timed region."""
+    from oracle import oracle as O
+    pos, _ = m.get_state()
     f64, _, _ = m.forces("fp64")
     fmx, _, _ = m.forces("mixed")
-    d = np.linalg.norm(fmx - f64, axis=1)
-    frms = float(np.sqrt(np.mean(np.sum(f64 * f64, axis=1))))
-    return {"max_abs": float(d.max()), "rms_abs": float(np.sqrt(np.mean(d * d))),
-            "rms_force_fp64": frms, "max_rel_to_rms_force": float(d.max()) / frms,
-            "rms_rel_to_rms_force": float(np.sqrt(np.mean(d * d))) / frms}
+    ref, _, _ = O.forces(pos, box, RC, threads=threads)
+    return {"mixed_vs_oracle": error_stats(fmx, ref),
+            "fp64_vs_oracle": error_stats(f64, ref),
+            "mixed_vs_fp64": error_stats(fmx, f64),
+            "oracle": f"oracle C restatement of exaMD_forces_midpoint_all, "
+                      f"FP64, {threads} host threads"}
 
 
 def boundary_ms(m, prec, reps=3):
     """Milliseconds of one evaluation through the boundary with host
     buffers, as the drop-in force call makes it: positions in (set_state),
-    bin + sort + force, forces out.  PCIe-inclusive; never the headline."""
+    bin + sort + force, forces out; and the same with the positions already
+    resident (the drop-in call skips set_state when the positions have not
+    changed since the last call).  PCIe-inclusive; never the headline."""
     p, _ = m.get_state()
     m.set_state(p)
     m.forces(prec)
@@ -117,19 +137,40 @@ def boundary_ms(m, prec, reps=3):
     for _ in range(reps):
         m.set_state(p)
         m.forces(prec)
-    return (time.perf_counter() - t0) / reps * 1e3
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        m.forces(prec)
+    t2 = time.perf_counter()
+    return (t1 - t0) / reps * 1e3, (t2 - t1) / reps * 1e3
+
+
+def chunk_used(graph_chunk, steps, every):
+    """Steps per captured chunk that examd_hip_run replays for a run of
+    `steps` steps tallied every `every` steps (0 = launched kernel by
+    kernel): the chunk is made a multiple of the tally stride, and a run
+    shorter than the chunk is not captured."""
+    if graph_chunk <= 0 or steps <= 0:
+        return 0
+    c = graph_chunk
+    if every > 0:
+        c = max(every, c // every * every)
+    return c if c <= steps else 0
 
 
 def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
-              shape=None, graph_chunk=100, report_error=False, tune_after=300):
+              shape=None, graph_chunk=100, report_error=False, tune_after=300,
+              oracle_threads=1):
     """Run `tune_after` untimed steps (melting the lattice), then autotune
     the launch shape for this precision (or apply the given shape), warm up
-    and time `steps` MD steps, replayed from captured hipGraph chunks of
-    `graph_chunk` steps.  With report_error, the mixed-vs-FP64 force error is
-    taken at the final positions (outside the timed region), with the
-    PCIe-inclusive time of one evaluation through the boundary."""
+    and time `steps` MD steps, replayed from captured hipGraph chunks of up
+    to `graph_chunk` steps (a shorter run replays a chunk of its own length).
+    Then, outside the timed region: the accuracy leg (DRIFT_STEPS more steps
+    with energies every DRIFT_EVERY steps) and, with report_error, the
+    per-atom force errors at the final positions and the PCIe-inclusive
+    time of one evaluation through the boundary."""
     err = None
-    with E.MidpointLJ(len(pos), box, RC, graph_chunk=graph_chunk) as m:
+    chunk = min(graph_chunk, steps) if graph_chunk > 0 else 0
+    with E.MidpointLJ(len(pos), box, RC, graph_chunk=chunk) as m:
         m.set_state(pos, vel)
         if tune_after:
             m.run(tune_after, DT, prec, energy_every=0)
@@ -143,20 +184,27 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
         # one untimed chunk with the timed run's settings captures the graph
         # the timed steps replay (the handle keeps one captured chunk)
         m.timing(enable=True, reset=True, every=TIME_EVERY)
-        if graph_chunk and steps >= graph_chunk:
-            m.run(graph_chunk, DT, prec, energy_every=energy_every)
+        used = chunk_used(chunk, steps, energy_every)
+        if used:
+            m.run(used, DT, prec, energy_every=energy_every)
         m.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         stats = m.run(steps, DT, prec, energy_every=energy_every)
         t1 = time.perf_counter()
         force_ms, launches = m.timing(enable=False, reset=True)
+        # accuracy leg, in 100-step chunks
+        m.set_graph_chunk(graph_chunk if graph_chunk > 0 else 0)
+        drift = m.run(DRIFT_STEPS[prec], DT, prec, energy_every=DRIFT_EVERY)
         if report_error:
-            err = force_error(m)
-            err["boundary_eval_ms_mixed"] = boundary_ms(m, prec)
+            err = force_errors(m, box, oracle_threads)
+            ms, ms_res = boundary_ms(m, prec)
+            err["boundary_eval_ms"] = ms
+            err["boundary_eval_ms_resident"] = ms_res
     if launches < steps // TIME_EVERY:
         raise RuntimeError(f"{prec}: {launches} timed force launches for "
                            f"{steps} steps")
-    return t1 - t0, stats, force_ms / max(launches, 1), launches, tune, err
+    return (t1 - t0, stats, force_ms / max(launches, 1), launches, tune, err,
+            drift, used)
 
 
 def batch_systems(E, nsys=256, seed=5):
@@ -192,6 +240,10 @@ def batch_bench(E, steps, warmup, loop_steps, nsys=256):
         tune = b.autotune("mixed", trials=2)
         if warmup:
             b.run(warmup, DT, "mixed")
+        # one untimed chunk with the timed run's settings captures the graph
+        # the timed steps replay
+        b.timing(enable=True, reset=True, every=TIME_EVERY)
+        b.run(100, DT, "mixed", energy_every=100)
         b.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         rows = b.run(steps, DT, "mixed", energy_every=100)
@@ -253,15 +305,30 @@ def slab_bench(E, steps, warmup):
     n = len(pos)
     with E.MidpointLJ(n, box, rc, graph_chunk=100) as m:
         m.set_state(pos, vel)
+        # the workload's named map, 32 x 12 x 12 cell blocks at equal atom
+        # shares: refused when a region does not fit one workgroup's LDS
+        try:
+            m.set_config("mixed", (32, 12, 12), 1024, balance=1)
+            named = "fits"
+        except E.ExamdError as e:
+            named = f"refused: {E.strerror(e.status)}"
         tune = m.autotune("mixed", trials=2)
+        cuts = m.get_cuts("mixed")
         if warmup:
             m.run(warmup, DT, "mixed")
+        m.timing(enable=True, reset=True, every=TIME_EVERY)
+        m.run(100, DT, "mixed", energy_every=100)
         m.timing(enable=True, reset=True, every=TIME_EVERY)
         t0 = time.perf_counter()
         st = m.run(steps, DT, "mixed", energy_every=100)
         wall = time.perf_counter() - t0
         force_ms, launches = m.timing(enable=False, reset=True)
     return {
+        "map": f"autotuned: {len(cuts[0]) - 1} x {len(cuts[1]) - 1} x "
+               f"{len(cuts[2]) - 1} cell blocks"
+               + (", boundaries at equal atom shares" if tune["balance"] else
+                  ", even split"),
+        "named_map_32x12x12_balanced": named,
         "workload": "liquid-vapour slab, box 3:1:1, central third at rho 0.80, "
                     "vapour 0.05, 0.8 sigma minimum separation, T* 0.85, "
                     "rc 3.0, mixed precision, dt 0.005",
@@ -393,14 +460,15 @@ def config1(E, steps=100):
             "max_velocity_diff_vs_oracle": float(np.abs(gvel - ovel).max())}
 
 
-def cpu_baseline(pos, vel, box, nsteps):
-    """The oracle (a 1-core C restatement of the reference path, FP64) on a
-    bounded sample of the same workload: nsteps VV steps of the full system,
-    after one untimed force evaluation."""
+def cpu_baseline(pos, vel, box, nsteps, threads=1):
+    """The oracle (the C restatement of the reference path, FP64) on a
+    bounded sample of the same workload: nsteps VV steps (rebin + force) of
+    the full system, after one untimed force evaluation, on `threads` host
+    threads (the slab-threaded form when more than one)."""
     from oracle import oracle as O
-    f0, _, _ = O.forces(pos, box, RC)
+    f0, _, _ = O.forces(pos, box, RC, threads=threads)
     t0 = time.perf_counter()
-    O.run_vv(pos, vel, box, RC, DT, nsteps, forces0=f0)
+    O.run_vv(pos, vel, box, RC, DT, nsteps, forces0=f0, threads=threads)
     dt = time.perf_counter() - t0
     return len(pos) * nsteps / dt
 
@@ -459,7 +527,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="oracle MD steps of the one-core CPU baseline")
+    ap.add_argument("--cpu-mt-steps", type=int, default=4,
+                    help="oracle MD steps of the all-cores CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--config1-steps", type=int, default=100)
@@ -494,16 +565,20 @@ def main():
     fsteps = args.fp64_steps if args.fp64_steps is not None else steps
     energy_every = 10
 
-    wall_mx, st_mx, force_ms_mx, launches, tune_mx, ferr = timed_run(
-        E, pos, vel, box, "mixed", steps, args.warmup, energy_every,
-        parse_shape(args.mixed_shape), args.graph_chunk, report_error=True)
-    wall_64, st_64, force_ms_64, _, tune_64, _ = timed_run(
+    host_threads = max(1, min(16, os.cpu_count() or 1))
+    (wall_mx, st_mx, force_ms_mx, launches, tune_mx, ferr, drift_mx,
+     chunk_mx) = timed_run(E, pos, vel, box, "mixed", steps, args.warmup,
+                           energy_every, parse_shape(args.mixed_shape),
+                           args.graph_chunk, report_error=True,
+                           oracle_threads=host_threads)
+    wall_64, st_64, force_ms_64, _, tune_64, _, drift_64, chunk_64 = timed_run(
         E, pos, vel, box, "fp64", fsteps, args.warmup, energy_every,
         parse_shape(args.fp64_shape), args.graph_chunk)
 
     rate_mx = n * steps / wall_mx
     rate_64 = n * fsteps / wall_64
-    boundary = ferr.pop("boundary_eval_ms_mixed", None) if ferr else None
+    boundary = ferr.pop("boundary_eval_ms", None)
+    boundary_res = ferr.pop("boundary_eval_ms_resident", None)
     # algorithmic work of one timed force launch: mean P_cand, P_cut over
     # the energy steps tallied; the timed launches tally no energy
     cand = float(np.mean([s.pairs_cand for s in st_mx]))
@@ -531,13 +606,22 @@ def main():
         c1 = config1(E, args.config1_steps)
 
     cpu = None
-    if not args.no_cpu and args.cpu_steps > 0:
-        v = cpu_baseline(pos, vel, box, args.cpu_steps)
-        cpu = {"value": v, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} velocity-Verlet steps (rebin + "
+    if not args.no_cpu and args.cpu_mt_steps > 0:
+        v = cpu_baseline(pos, vel, box, args.cpu_mt_steps, host_threads)
+        cpu = {"value": v, "unit": "atom-steps/s", "cores": host_threads,
+               "kind": "port",
+               "sample": f"{args.cpu_mt_steps} velocity-Verlet steps (rebin + "
                          f"force) of the same 2,048,000-atom system, FP64, "
-                         f"oracle C restatement of the reference path",
+                         f"oracle C restatement of the reference path on "
+                         f"{host_threads} threads (z slabs in two phases)",
                "host": host_cpu()}
+        if args.cpu_steps > 0:
+            v1 = cpu_baseline(pos, vel, box, args.cpu_steps, 1)
+            cpu["one_core"] = {
+                "value": v1, "cores": 1,
+                "sample": f"{args.cpu_steps} velocity-Verlet steps of the "
+                          f"same system, the serial oracle (the reference's "
+                          f"one-rank mode)"}
 
     traffic = read_traffic()
     line = {
@@ -552,7 +636,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "mixed",
         "data": "synthetic",
         "config": {
             "workload": "LJ liquid 2,048,000 atoms (80^3 FCC, rho 0.8442, "
@@ -573,21 +657,34 @@ def main():
             "force_frac_of_step_mixed": force_ms_mx * steps / (1e3 * wall_mx),
             "fp64_force_tflops": achieved64,
             "fp64_force_frac_of_fp64_peak": achieved64 / FP64_PEAK_TFLOPS,
-            "energy_drift_per_1k_mixed": drift_per_1k(st_mx),
-            "energy_drift_per_1k_fp64": drift_per_1k(st_64),
-            "energy_drift_steps": [steps, fsteps],
-            "energy_max_dev_rel_mixed": max_dev(st_mx),
-            "energy_max_dev_rel_fp64": max_dev(st_64),
-            "mixed_force_error_vs_fp64_at_end": ferr,
+            "energy_drift_per_1k_mixed": drift_per_1k(drift_mx),
+            "energy_drift_per_1k_fp64": drift_per_1k(drift_64),
+            "energy_drift_leg": {
+                "steps": [DRIFT_STEPS["mixed"], DRIFT_STEPS["fp64"]],
+                "tally_every": DRIFT_EVERY,
+                "what": "shifted total energy KE + sum(V - V(rc)), "
+                        "least-squares slope per 1,000 steps relative to "
+                        "|E|, steps continued from the timed run (mixed, "
+                        "fp64), outside the timed region"},
+            "energy_max_dev_rel_mixed": max_dev(drift_mx),
+            "energy_max_dev_rel_fp64": max_dev(drift_64),
+            "force_error_at_end_of_drift_leg": ferr,
             "host_boundary": {
                 "eval_ms_mixed": boundary,
+                "eval_ms_mixed_positions_resident": boundary_res,
                 "what": "one evaluation through the C ABI with host buffers, "
                         "as the drop-in force call makes it: set_state "
                         "(positions in), bin + sort + force, forces out; "
-                        "PCIe-inclusive, not the headline"},
+                        "and the same when the positions are unchanged "
+                        "since the last call, which the drop-in call then "
+                        "does not send again; PCIe-inclusive, not the "
+                        "headline"},
             "launch_mixed": tune_mx,
             "launch_fp64": tune_64,
-            "graph_chunk_steps": args.graph_chunk,
+            "graph_chunk_steps": [chunk_mx, chunk_64],
+            "graph_chunk_what": "steps per captured hipGraph chunk the timed "
+                                "region replayed (mixed, fp64; 0 = launched "
+                                "kernel by kernel)",
             "batch": batch,
             "slab": slab,
             "species": species,
