@@ -439,8 +439,14 @@ Module exaMD_forces_mod
   !! positions at least every this many steps (DOC/exaMD.tex:284-291).
   Integer :: exaMD_rebalance_every = 0
 
+  !! Force calls that found grid%posn unchanged since the positions last
+  !! sent to the device, and so did not send them again.
+  Integer :: exaMD_positions_resent_skipped = 0
+
   Type(c_ptr), Private :: handle = c_null_ptr
   Integer(c_int64_t), Private :: handle_n = -1
+  !! The positions the device holds (locblocks order), while they are known.
+  Real(c_double), Allocatable, Private :: sent_pos(:,:)
   Logical, Private :: handle_species = .False.
   Logical, Private :: handle_table = .False.
 
@@ -453,7 +459,28 @@ Contains
     handle_n = -1
     handle_species = .False.
     handle_table = .False.
+    If (Allocated(sent_pos)) Deallocate(sent_pos)
   End Subroutine exaMD_release_device
+
+  !! Send the gathered positions unless the device already holds exactly
+  !! these (the same call repeated on unchanged grid%posn).
+  Subroutine send_positions(pos, n)
+    Real(c_double), Intent(in), Target :: pos(:,:)
+    Integer(c_int64_t), Intent(in) :: n
+    Integer(c_int) :: st
+    If (Allocated(sent_pos)) Then
+       If (Size(sent_pos, 2) == Size(pos, 2)) Then
+          If (All(sent_pos == pos)) Then
+             exaMD_positions_resent_skipped = exaMD_positions_resent_skipped + 1
+             Return
+          End If
+       End If
+       Deallocate(sent_pos)
+    End If
+    st = examd_hip_set_state(handle, pos, c_null_ptr, n)
+    Call check_status(st, 'examd_hip_set_state')
+    Allocate(sent_pos, source=pos)
+  End Subroutine send_positions
 
   !! Is user_func the Lennard-Jones pair with prmvdw(1,:)'s parameters?
   !! (checked at a few distances; the device evaluates that pair directly)
@@ -652,8 +679,7 @@ Contains
     Call ensure_handle(n, cutoff)
     Call set_device_pair(user_func, cutoff)
     Call set_device_species(n)
-    st = examd_hip_set_state(handle, pos, c_null_ptr, n)
-    Call check_status(st, 'examd_hip_set_state')
+    Call send_positions(pos, n)
     Allocate(f(3, Max(n, 1_c_int64_t)))
     st = examd_hip_forces(handle, exaMD_precision, f, exaMD_last_stats)
     Call check_status(st, 'examd_hip_forces')
@@ -698,6 +724,8 @@ Contains
     Call gather_positions(pos, n)
     Call ensure_handle(n, cutoff)
     Call set_device_species(n)
+    ! the run moves the device's positions
+    If (Allocated(sent_pos)) Deallocate(sent_pos)
     st = examd_hip_set_state(handle, pos, c_loc(vel), n)
     Call check_status(st, 'examd_hip_set_state')
     If (exaMD_rebalance_every > 0) Then

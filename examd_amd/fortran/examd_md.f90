@@ -19,6 +19,13 @@
 !!                   from the current positions at least every K MD steps
 !!   -morse          the user function is a Morse pair (De 0.8, a 2.0,
 !!                   r0 1.1), which the library tabulates for the GPU
+!!   -evals N        force evaluations (default 1); the forces accumulate
+!!                   in grid%forces over the calls, as the reference's
+!!                   routine adds into them, and each call's time is printed
+!!   -printforces    per-block sums of squared forces after the check, in the
+!!                   reference's PRINTALLFORCES format (exaMD.f90:340-357)
+!!   -dumpforces f   per-atom positions and forces in grid order (blocks of
+!!                   locblocks, then file order), 17 significant digits
 !!   -binary         two species (Kob-Andersen): every fifth atom in input
 !!                   order is species 2; pair rows AA (1, 1, rc 2.5),
 !!                   AB (1.5, 0.8, 2.0), BB (0.5, 0.88, 2.2) as (eps, sigma,
@@ -40,7 +47,9 @@ Program examd_md
 
   Real(kind=PRC) :: cutoff, block_length, rho, temp, dt, ftot(3), ftot2(3)
   Integer :: natms_tot, nblocks, nblocks_xyz(3), pid, noprocs, nc, i, j, k, b
-  Integer :: argc, ia, nnan
+  Integer :: argc, ia, nnan, nevals, ev
+  Logical :: printforces
+  Character(len=256) :: dump_file
   Integer(c_int64_t) :: nsteps, seed, nstats, s
   Integer, Allocatable :: ntot(:), first_particle(:), next_particle(:)
   Integer, Allocatable :: part(:,:,:)
@@ -70,6 +79,9 @@ Program examd_md
   morse = .False.
   restart_file = ''
   checkpoint_file = ''
+  dump_file = ''
+  nevals = 1
+  printforces = .False.
   step0 = 0
 
   argc = command_argument_count()
@@ -87,6 +99,15 @@ Program examd_md
         binary = .True.
      Case ('-morse')
         morse = .True.
+     Case ('-printforces')
+        printforces = .True.
+     Case ('-dumpforces')
+        ia = ia + 1
+        Call get_command_argument(ia, dump_file)
+     Case ('-evals')
+        ia = ia + 1
+        Call get_command_argument(ia, arg)
+        Read (arg, *) nevals
      Case ('-atoms')
         ia = ia + 1
         Call get_command_argument(ia, arg)
@@ -214,18 +235,24 @@ Program examd_md
   End If
   Call exaMD_setup_midpoint_interactions()
 
-  ! one force evaluation, timed like the reference (exaMD.f90:241-253)
-  Call system_clock(c0, crate)
-  Call exaMD_start_comms_posn()
-  If (morse) Then
-     Call exaMD_forces_midpoint_all(morse_pair, cutoff, noprocs)
-  Else If (overlapcase) Then
-     Call exaMD_forces_midpoint_overlap(lennardjones_12, cutoff, pid, noprocs)
-  Else
-     Call exaMD_forces_midpoint_all(lennardjones_12, cutoff, noprocs)
-  End If
-  Call exaMD_wait_comms_forces(noprocs)
-  Call system_clock(c1)
+  ! force evaluations, timed like the reference (exaMD.f90:241-253); the
+  ! calls after the first find the positions unchanged
+  Do ev = 1, Max(1, nevals)
+     Call system_clock(c0, crate)
+     Call exaMD_start_comms_posn()
+     If (morse) Then
+        Call exaMD_forces_midpoint_all(morse_pair, cutoff, noprocs)
+     Else If (overlapcase) Then
+        Call exaMD_forces_midpoint_overlap(lennardjones_12, cutoff, pid, noprocs)
+     Else
+        Call exaMD_forces_midpoint_all(lennardjones_12, cutoff, noprocs)
+     End If
+     Call exaMD_wait_comms_forces(noprocs)
+     Call system_clock(c1)
+     If (nevals > 1) Write (*, '(A,I0,A,F10.4,A,I0)') '  Evaluation ', ev, &
+          ' time (secs):', Real(c1 - c0, 8) / Real(crate, 8), &
+          ', positions not resent: ', exaMD_positions_resent_skipped
+  End Do
 
   ! checks of exaMD.f90:269-317
   nnan = 0
@@ -245,11 +272,39 @@ Program examd_md
   If (Abs(ftot(1)) > 1e-10_PRC) Write (*, *) 'WARNING! Suspected error'
   Write (*, '(A,3E10.1)') '  SUM(fxx, fyy, fzz):', ftot
   Write (*, '(A,3E10.1)') '  SUM(fxx^2, fyy^2, fzz^2):', ftot2
+  Write (*, '(A,3ES25.16)') '  SUM(fxx, fyy, fzz) full:', ftot
+  Write (*, '(A,3ES25.16)') '  SUM(fxx^2, fyy^2, fzz^2) full:', ftot2
   Write (*, '(A,I0,A,I0)') '  Pairs: candidate ', exaMD_last_stats%pairs_cand, &
        ', in cutoff ', exaMD_last_stats%pairs_cut
   Write (*, '(A,F16.6)') '  Potential energy:', exaMD_last_stats%pe
   Write (*, '(A,F10.4)') '  Computation time (secs):', &
        Real(c1 - c0, 8) / Real(crate, 8)
+
+  ! exaMD.f90:340-357 (PRINTALLFORCES): i, j, k and the block's sums of
+  ! squared force components, for every block with a non-zero sum
+  If (printforces) Then
+     Do b = 1, nblocks_loc
+        i = locblocks(1, b)
+        j = locblocks(2, b)
+        k = locblocks(3, b)
+        If (grid(i, j, k)%natms == 0) Cycle
+        ftot2 = Sum(grid(i, j, k)%forces**2, dim=2)
+        If (Sum(ftot2) > zero) Write (*, '(3I4.3,3(X,E20.12))') i, j, k, ftot2
+     End Do
+  End If
+  If (Len_Trim(dump_file) > 0) Then
+     Open(61, file=Trim(dump_file), status='replace')
+     Do b = 1, nblocks_loc
+        i = locblocks(1, b)
+        j = locblocks(2, b)
+        k = locblocks(3, b)
+        Do m = 1, grid(i, j, k)%natms
+           Write (61, '(6ES25.16)') grid(i, j, k)%posn(:, m), &
+                grid(i, j, k)%forces(:, m)
+        End Do
+     End Do
+     Close(61)
+  End If
 
   If (nsteps > 0) Then
      ! velocities in the grid's atom order (blocks, then input order)

@@ -208,3 +208,88 @@ def test_driver_rebalance(tmp_path):
     L = (4000 / 0.8442) ** (1 / 3)
     d -= np.round(d / L) * L
     assert np.abs(d).max() < 1e-4
+
+
+def _particle_file(tmp_path, oracle):
+    """exaMD_particles.dat of a non-cubic box whose sides are not multiples
+    of the block length (enlarged last blocks on every axis), uniform random
+    atoms with a 0.85 sigma minimum separation at density 0.75."""
+    import examd_amd as E
+    box = np.array([17.3, 15.1, 19.7])
+    pos, _ = E.generate_slab(box, 0.0, box[0], 0.75, 0.75, min_sep=0.85,
+                             seed=77)
+    oracle.write_particles(tmp_path / "exaMD_particles.dat", pos, box)
+    return oracle.read_particles(tmp_path / "exaMD_particles.dat")
+
+
+def _read_dump(path, pos_ref):
+    """The driver's per-atom dump (grid order) matched to the file's atom
+    order by position (both read the same decimal file)."""
+    d = np.loadtxt(path)
+    where = {tuple(p): i for i, p in enumerate(pos_ref)}
+    idx = np.array([where[tuple(p)] for p in d[:, :3]])
+    assert np.array_equal(np.sort(idx), np.arange(len(pos_ref)))
+    f = np.zeros_like(pos_ref)
+    f[idx] = d[:, 3:]
+    return f
+
+
+def _block_sums(pos, f, box, rc):
+    """Per-block sums of squared force components, keyed by the reference's
+    1-based (i, j, k)."""
+    from tests import helpers as H
+    nb = (np.asarray(box) / (0.5 * rc)).astype(np.int64)
+    b = H.block_of(pos, 0.5 * rc, nb)
+    out = {}
+    for bb, ff in zip(map(tuple, b), f):
+        key = (bb[0] + 1, bb[1] + 1, bb[2] + 1)
+        out[key] = out.get(key, 0.0) + ff ** 2
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("call", ["all", "overlap"])
+def test_driver_readfile_forces_per_atom(tmp_path, oracle, call):
+    """-readfile through exaMD_forces_midpoint_all (and -overlap through
+    exaMD_forces_midpoint_overlap), evaluated twice on unchanged positions:
+    every atom's accumulated grid%forces equals twice the oracle's forces of
+    the same file (FP64, 1e-10 normwise), the pair counts are exact, the
+    PRINTALLFORCES lines equal the oracle's per-block sums of squares, the
+    second call does not send the positions again, and the mixed path stays
+    within its FP32 bars."""
+    pos, box = _particle_file(tmp_path, oracle)
+    ref, t, _ = oracle.forces(pos, box, 2.5)
+    print(f"[fortran readfile {call}] n={len(pos)} box={box.tolist()}", flush=True)
+    extra = ["-overlap"] if call == "overlap" else []
+    r = subprocess.run([_driver(), "-readfile", "-evals", "2", "-printforces",
+                        "-dumpforces", "f64.txt", *extra], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    out = r.stdout
+    print(out[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    assert f"Number of atoms: {len(pos)}" in out
+    m = re.search(r"Pairs: candidate (\d+), in cutoff (\d+)", out)
+    assert (int(m.group(1)), int(m.group(2))) == (t.cand, t.cut)
+    assert "positions not resent: 1" in out
+    f = _read_dump(tmp_path / "f64.txt", pos)
+    scale = np.sqrt((ref ** 2).sum(1).mean())
+    assert np.abs(f - 2.0 * ref).max() < 1e-10 * 2.0 * scale
+    want = _block_sums(pos, 2.0 * ref, box, 2.5)
+    lines = re.findall(r"^\s*(\d{3,})\s*(\d{3,})\s*(\d{3,})\s+(\S+)\s+(\S+)\s+(\S+)\s*$",
+                       out, re.M)
+    assert len(lines) == sum(1 for v in want.values() if v.sum() > 0)
+    for i, j, k, a, b, c in lines:
+        got = np.array([float(a), float(b), float(c)])
+        w = want[(int(i), int(j), int(k))]
+        assert np.all(np.abs(got - w) <= 1e-10 * max(w.sum(), 1e-300)), (i, j, k)
+    full = re.search(r"SUM\(fxx, fyy, fzz\) full:\s*(\S+)\s+(\S+)\s+(\S+)", out)
+    assert np.abs([float(x) for x in full.groups()]).max() < 1e-10 * np.abs(ref).sum()
+    # the mixed path through the same call
+    r = subprocess.run([_driver(), "-readfile", "-precision", "mixed",
+                        "-dumpforces", "fmx.txt", *extra], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    fm = _read_dump(tmp_path / "fmx.txt", pos)
+    err = np.sqrt(((fm - ref) ** 2).sum(1))
+    assert err.max() / scale < 2e-5
+    assert np.all(np.round(fm * 2.0 ** 32).astype(np.int64).sum(0) == 0)
