@@ -34,6 +34,7 @@ ERR_NOT_READY = 8
 ERR_FIXED_RANGE = 9
 ERR_ZERO_DISTANCE = 10
 ERR_TABLE_RANGE = 11
+ERR_STATIC_LIMIT = 12
 ERR_OUT_OF_BOX = 109
 
 # every symbol include/examd_hip.h declares
@@ -47,6 +48,7 @@ EXPORTS = (
     "examd_hip_batch_create", "examd_hip_batch_info",
     "examd_hip_generate_vacancy_fcc", "examd_hip_generate_slab",
     "examd_hip_set_species", "examd_hip_set_pair_table", "examd_hip_get_cuts",
+    "examd_hip_forces_static",
 )
 
 MAX_SPECIES = 8
@@ -121,6 +123,8 @@ def lib():
         L.examd_hip_set_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
         L.examd_hip_get_state.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64]
         L.examd_hip_forces.argtypes = [C.c_void_p, C.c_int, dp, P(Stats)]
+        L.examd_hip_forces_static.argtypes = [P(Params), C.c_int, C.c_void_p,
+                                              C.c_void_p, P(Stats)]
         L.examd_hip_run.argtypes = [C.c_void_p, C.c_int, C.c_int64,
                                     C.c_double, C.c_int64, P(Stats),
                                     C.c_int64, ip64]
@@ -447,6 +451,25 @@ def _params(cutoff, epsilon, sigma, mass, device, block_threads,
     p.run_groups = int(run_groups)
     p.graph_chunk = int(graph_chunk)
     return p
+
+
+def forces_static(natoms, box, cutoff, pos_dev, forces_dev, precision="fp64",
+                  tiles=(0, 0, 0), epsilon=1.0, sigma=1.0, device=0,
+                  block_threads=0, run_groups=0, allow=()):
+    """examd_hip_forces_static: one evaluation that allocates nothing.
+    pos_dev and forces_dev are device addresses (ints) of (natoms, 3)
+    float64 arrays.  Returns (Stats, status)."""
+    p = _params(cutoff, epsilon, sigma, 1.0, device, block_threads, 0,
+                run_groups, 0)
+    p.natoms = int(natoms)
+    p.box = (C.c_double * 3)(*[float(v) for v in np.reshape(box, 3)])
+    p.tiles = (C.c_int32 * 3)(*[int(t) for t in tiles])
+    s = Stats()
+    st = _check(lib().examd_hip_forces_static(
+        C.byref(p), _PREC[precision], C.c_void_p(int(pos_dev)),
+        C.c_void_p(int(forces_dev)), C.byref(s)), "examd_hip_forces_static",
+        allow)
+    return s, st
 
 
 class MidpointLJ(_Handle):

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -961,6 +962,80 @@ int create_handle(const examd_params *prm, const examd_system *systems,
   return EXAMD_OK;
 }
 
+// ------------------------------------------------ allocation-free form ----
+// examd_hip_forces_static keeps everything beyond its inputs and outputs in
+// one static device array per GPU: the atoms' SoA positions, force sums,
+// block ids, ranks and sorted ids, the block counts and their scan, the
+// per-cell-block partial tallies, the step log and the system descriptor.
+constexpr int64_t kStaticAtoms = EXAMD_STATIC_MAX_ATOMS;
+constexpr int64_t kStaticBlocks = EXAMD_STATIC_MAX_BLOCKS;
+constexpr int64_t kStaticTiles = EXAMD_STATIC_MAX_CELL_BLOCKS;
+constexpr size_t scratch_align(size_t v) { return (v + 255) / 256 * 256; }
+struct StaticLayout {
+  size_t xyz, f, ints, count, start, chunk, part, log, sys, total;
+};
+constexpr StaticLayout static_layout() {
+  StaticLayout L{};
+  size_t o = 0;
+  L.xyz = o;
+  o = scratch_align(o + 3 * (size_t)kStaticAtoms * sizeof(double));
+  L.f = o;   // FP64 sums or fixed-point sums, one precision per call
+  o = scratch_align(o + 3 * (size_t)kStaticAtoms * sizeof(double));
+  L.ints = o;   // cell, rank, sorted
+  o = scratch_align(o + 3 * (size_t)kStaticAtoms * sizeof(int));
+  L.count = o;
+  o = scratch_align(o + (size_t)kStaticBlocks * sizeof(int));
+  L.start = o;
+  o = scratch_align(o + ((size_t)kStaticBlocks + 1) * sizeof(int));
+  L.chunk = o;
+  o = scratch_align(o + 4096 * sizeof(int));
+  L.part = o;
+  o = scratch_align(o + (size_t)std::max<int64_t>(kStaticTiles, kStaticAtoms / 256 + 1) *
+                            sizeof(Partial));
+  L.log = o;
+  o = scratch_align(o + sizeof(StepLog));
+  L.sys = o;
+  o = scratch_align(o + sizeof(SysDesc));
+  L.total = o;
+  return L;
+}
+constexpr StaticLayout kStatic = static_layout();
+static_assert(kStatic.total == EXAMD_STATIC_SCRATCH_BYTES,
+              "EXAMD_STATIC_SCRATCH_BYTES must match the scratch layout");
+
+// The array itself.  In host code its name stands for a host-side symbol,
+// not a device address: host code reaches it only through the per-device
+// addresses hipGetSymbolAddress returns (scratch_of).
+static __device__ __align__(256) unsigned char scratch[kStatic.total];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kScratchDevices] = {};
+size_t g_scratch_lds[kScratchDevices] = {};
+
+// The device address of `scratch` on device dev and that device's LDS
+// limit for the force kernel, looked up once per device with the device
+// current (each device holds its own copy of the array).
+int scratch_of(int dev, unsigned char **ptr, size_t *lds_limit) {
+  if (dev < 0 || dev >= kScratchDevices) return EXAMD_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    HIPCHK(hipSetDevice(dev));
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return EXAMD_ERR_HIP;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                              dev) != hipSuccess || lds <= 0)
+      lds = 65536;
+    g_scratch_lds[dev] = (size_t)lds > 4096 ? (size_t)lds - 4096 : (size_t)lds;
+    g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+  }
+  *ptr = g_scratch_dev[dev];
+  *lds_limit = g_scratch_lds[dev];
+  return EXAMD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -984,6 +1059,9 @@ const char *examd_hip_strerror(int s) {
     case EXAMD_ERR_TABLE_RANGE:
       return "pair closer than the tabulated pair function's range";
     case EXAMD_ERR_OUT_OF_BOX: return "block number out of range";
+    case EXAMD_ERR_STATIC_LIMIT:
+      return "system beyond the static scratch limits of "
+             "examd_hip_forces_static";
     default: return "unknown status";
   }
 }
@@ -1332,6 +1410,132 @@ int examd_hip_forces(examd_handle *h, int precision, double *forces,
     if (s2 != EXAMD_OK) return s2;
   }
   return st;
+}
+
+int examd_hip_forces_static(const examd_params *prm, int precision,
+                            const double *pos, double *forces,
+                            examd_stats *stats) {
+  if (!prm || !pos || !forces ||
+      (precision != EXAMD_PREC_FP64 && precision != EXAMD_PREC_MIXED))
+    return EXAMD_ERR_ARG;
+  const int64_t n = prm->natoms;
+  if (n < 1 || !(prm->cutoff > 0.0) || !(prm->sigma > 0.0) ||
+      !(prm->epsilon >= 0.0))
+    return EXAMD_ERR_ARG;
+  if (n > kStaticAtoms) return EXAMD_ERR_STATIC_LIMIT;
+  int32_t nb[3];
+  int64_t nblocks = 0;
+  int st = examd_hip_domain(prm->box, 0.5 * prm->cutoff, nb, &nblocks);
+  if (st) return st;
+  for (int a = 0; a < 3; ++a)
+    if (nb[a] < 5 || !(prm->cutoff < 0.5 * prm->box[a])) return EXAMD_ERR_GRID;
+  if (nblocks > kStaticBlocks) return EXAMD_ERR_STATIC_LIMIT;
+  // the cell-block map: the given tiles or the default, split evenly
+  Grid g{};
+  int64_t ntiles = 1;
+  int wmax[3];
+  for (int a = 0; a < 3; ++a) {
+    g.nb[a] = nb[a];
+    g.L[a] = prm->box[a];
+    const int T = prm->tiles[a] > 0 ? prm->tiles[a] : (nb[a] + 6) / 7;
+    g.T[a] = std::max(1, std::min(T, nb[a]));
+    ntiles *= g.T[a];
+    const std::vector<int> c = even_cuts(nb[a], g.T[a]);
+    wmax[a] = 0;
+    for (int t = 0; t < g.T[a]; ++t) wmax[a] = std::max(wmax[a], c[t + 1] - c[t]);
+    if (wmax[a] + 2 > 255) return EXAMD_ERR_TILE_OVERFLOW;
+  }
+  g.bl = 0.5 * prm->cutoff;
+  if (ntiles > kStaticTiles) return EXAMD_ERR_STATIC_LIMIT;
+  unsigned char *base = nullptr;
+  size_t lds_limit = 0;
+  if ((st = scratch_of(prm->device, &base, &lds_limit)) != EXAMD_OK) return st;
+  HIPCHK(hipSetDevice(prm->device));
+  const bool mixed = precision == EXAMD_PREC_MIXED;
+  // launch shape: the LDS holds the largest region the limit allows (a
+  // region beyond it is reported as TILE_OVERFLOW by the kernel); run
+  // groups as a handle chooses them
+  ForceConfig cfg;
+  const int thr = prm->block_threads;
+  cfg.threads = (thr == 256 || thr == 512 || thr == 1024) ? thr : 512;
+  cfg.maxrb = (wmax[0] + 2) * (wmax[1] + 2) * (wmax[2] + 2);
+  cfg.lane_queue = 0;
+  cfg.workgroups = 0;
+  int64_t cap = 65520;
+  while (cap >= 32 && force_lds_bytes(!mixed, (int)cap, cfg.maxrb, cfg.threads,
+                                      0, false) > lds_limit)
+    cap -= 16;
+  if (cap < 32) return EXAMD_ERR_TILE_OVERFLOW;
+  cfg.cap = (int)cap;
+  double reg = (double)n / (g.L[0] * g.L[1] * g.L[2]);
+  for (int a = 0; a < 3; ++a) reg *= ((double)g.nb[a] / g.T[a] + 2.0) * g.bl;
+  const int ng = prm->run_groups > 0
+                     ? prm->run_groups
+                     : (int)std::lround(3.0 * cfg.threads / std::max(reg, 1.0));
+  cfg.groups = std::max(1, std::min(4, ng));
+  // the device state, every array in the scratch
+  DevState d{};
+  d.n = n;
+  double *xyz = reinterpret_cast<double *>(base + kStatic.xyz);
+  d.x = xyz;
+  d.y = xyz + n;
+  d.z = xyz + 2 * n;
+  double *f = reinterpret_cast<double *>(base + kStatic.f);
+  d.fx = f;
+  d.fy = f + n;
+  d.fz = f + 2 * n;
+  d.qx = reinterpret_cast<unsigned long long *>(f);
+  d.qy = d.qx + n;
+  d.qz = d.qx + 2 * n;
+  int *ints = reinterpret_cast<int *>(base + kStatic.ints);
+  d.cell = ints;
+  d.rank = ints + n;
+  d.sorted = ints + 2 * n;
+  d.count = reinterpret_cast<int *>(base + kStatic.count);
+  d.start = reinterpret_cast<int *>(base + kStatic.start);
+  d.chunk = reinterpret_cast<int *>(base + kStatic.chunk);
+  d.part = reinterpret_cast<Partial *>(base + kStatic.part);
+  StepLog *log = reinterpret_cast<StepLog *>(base + kStatic.log);
+  SysDesc *sysd = reinterpret_cast<SysDesc *>(base + kStatic.sys);
+  SysDesc sd{};
+  sd.g = g;
+  sd.atom0 = 0;
+  sd.block0 = 0;
+  sd.tile0 = 0;
+  sd.cut[0] = sd.cut[1] = sd.cut[2] = -1;
+  SysTable t;
+  t.sys = sysd;
+  t.nsys = 1;
+  t.ntiles = (int)ntiles;
+  const LJParams p{prm->epsilon, prm->sigma, prm->cutoff, prm->mass};
+  hipStream_t s = nullptr;
+  // every call starts from cleared counts and tallies (fill kernels: the
+  // array keeps the last call's contents)
+  launch_store_sys(sd, sysd, s);
+  launch_aos_to_soa(pos, d.x, d.y, d.z, n, s);
+  launch_fill_i32(d.count, 0, nblocks, s);
+  launch_fill_log(log, 1, s);
+  launch_step_bin(d, t, mixed, kBinOnly, 0.0, 0.0, nullptr, log, s);
+  launch_sort(d, nblocks, !mixed, s);
+  launch_forces(d, t, p, cfg, mixed, true, false, log, s);
+  launch_forces_to_fp64(d, forces, mixed, s);
+  HIPCHK(hipGetLastError());
+  StepLog L{};
+  HIPCHK(hipMemcpy(&L, log, sizeof(StepLog), hipMemcpyDeviceToHost));
+  if (stats) {
+    const double pe = mixed ? (double)(long long)L.pe_fix * kFixInv : L.pe;
+    const double sr2 = prm->sigma * prm->sigma / (prm->cutoff * prm->cutoff);
+    const double sr6 = sr2 * sr2 * sr2;
+    const double vrc = 4.0 * prm->epsilon * (sr6 * sr6 - sr6);
+    stats->step = 0;
+    stats->ke = 0.0;
+    stats->pe = pe;
+    stats->pe_shifted = pe - (double)L.cut * vrc;
+    stats->etot = pe;
+    stats->pairs_cand = (int64_t)L.cand;
+    stats->pairs_cut = (int64_t)L.cut;
+  }
+  return flags_to_status(L.flags);
 }
 
 int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,

@@ -1330,6 +1330,12 @@ __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   }
 }
 
+// One system descriptor into device memory, from a kernel argument (no
+// copy from host memory on the stream).
+__global__ void k_store_sys(SysDesc v, SysDesc *dst) {
+  if (threadIdx.x == 0) *dst = v;
+}
+
 __global__ void k_forces_to_fp64(DevState d, double *out, bool mixed) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.n) return;
@@ -1539,6 +1545,10 @@ void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
   if (energy && !homecount)
     k_reduce_partials<<<t.nsys, kReduceThreads, 0, s>>>(
         d.part, t.nsys > 1 ? t.tile_seg : nullptr, t.ntiles, log, kSumForce);
+}
+
+void launch_store_sys(const SysDesc &v, SysDesc *dst, hipStream_t s) {
+  k_store_sys<<<1, 64, 0, s>>>(v, dst);
 }
 
 void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,

@@ -112,5 +112,7 @@ void launch_soa_to_aos(const double *x, const double *y, const double *z,
                        double *aos, int64_t n, hipStream_t s);
 void launch_forces_to_fp64(const DevState &d, double *out_aos, bool mixed,
                            hipStream_t s);
+// A system descriptor written by a kernel from its argument.
+void launch_store_sys(const SysDesc &v, SysDesc *dst, hipStream_t s);
 
 }  // namespace examd

@@ -15,7 +15,8 @@ Module exaMD_hip_binding
        EXAMD_ERR_TILE_OVERFLOW = 5, EXAMD_ERR_NO_DEVICE = 6,              &
        EXAMD_ERR_NONFINITE = 7, EXAMD_ERR_NOT_READY = 8,                  &
        EXAMD_ERR_FIXED_RANGE = 9, EXAMD_ERR_ZERO_DISTANCE = 10,           &
-       EXAMD_ERR_TABLE_RANGE = 11, EXAMD_ERR_OUT_OF_BOX = 109
+       EXAMD_ERR_TABLE_RANGE = 11, EXAMD_ERR_STATIC_LIMIT = 12,          &
+       EXAMD_ERR_OUT_OF_BOX = 109
 
   Integer(c_int), Parameter :: EXAMD_PREC_FP64 = 0, EXAMD_PREC_MIXED = 1
 
@@ -125,6 +126,18 @@ Module exaMD_hip_binding
        Real(c_double), Intent(out) :: forces(*)
        Type(examd_stats), Intent(out) :: stats
      End Function examd_hip_forces
+
+     !! pos and forces are device addresses (3, natoms); stats may be
+     !! c_null_ptr.  Allocates nothing (a static device array per GPU).
+     Integer(c_int) Function examd_hip_forces_static(params, precision, pos, &
+          forces, stats) Bind(C, name="examd_hip_forces_static")
+       Import :: c_int, c_ptr, examd_params
+       Type(examd_params), Intent(in) :: params
+       Integer(c_int), Value :: precision
+       Type(c_ptr), Value :: pos
+       Type(c_ptr), Value :: forces
+       Type(c_ptr), Value :: stats
+     End Function examd_hip_forces_static
 
      Integer(c_int) Function examd_hip_run(handle, precision, nsteps, dt, &
           energy_every, out, out_cap, n_out) Bind(C, name="examd_hip_run")

@@ -44,6 +44,9 @@ enum {
                                    reference prints 'ZERO at' */
     EXAMD_ERR_TABLE_RANGE = 11, /* a pair closer than the range of the
                                    tabulated pair function */
+    EXAMD_ERR_STATIC_LIMIT = 12,/* examd_hip_forces_static: atoms, blocks
+                                   or cell blocks beyond its static
+                                   scratch limits */
     EXAMD_ERR_OUT_OF_BOX = 109  /* block number out of range (ref. 109) */
 };
 
@@ -186,6 +189,37 @@ int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
  * (SRC/exaMD_comms.f90:44, :330) done as an on-device ghost gather and
  * ghost-force return.  forces (3,n) receives the force (overwritten, not
  * accumulated; may be NULL); stats may be NULL. */
+/* Limits of examd_hip_forces_static and the size of its static device
+ * array (one per GPU, held until the process ends). */
+#define EXAMD_STATIC_MAX_ATOMS 65536
+#define EXAMD_STATIC_MAX_BLOCKS 131072
+#define EXAMD_STATIC_MAX_CELL_BLOCKS 4096
+#define EXAMD_STATIC_SCRATCH_BYTES 5194496
+
+/* No counterpart in the reference.  One force evaluation that allocates
+ * nothing, for callers that cannot allocate device memory (code that must
+ * not call hipMalloc: serving loops, harnesses that forbid it).  It is
+ * examd_hip_forces without a handle: the same kernels, with every buffer
+ * beyond the inputs and outputs in one static device array of
+ * EXAMD_STATIC_SCRATCH_BYTES per GPU, whose address is looked up once per
+ * device on the first call there.
+ *
+ * params: natoms, box, cutoff, epsilon, sigma, device, tiles (0 = the
+ * default map, split evenly), block_threads, run_groups; the others are
+ * ignored.  pos: DEVICE array (3, natoms) of positions, forces: DEVICE
+ * array (3, natoms) that receives the forces (overwritten); stats (host,
+ * may be NULL): pair counts and energy, ke = 0.  Runs on the null stream
+ * and returns when the forces are written.  Returns EXAMD_ERR_STATIC_LIMIT
+ * beyond EXAMD_STATIC_MAX_ATOMS atoms, EXAMD_STATIC_MAX_BLOCKS blocks or
+ * EXAMD_STATIC_MAX_CELL_BLOCKS cell blocks, EXAMD_ERR_TILE_OVERFLOW when a
+ * cell block's region does not fit the LDS (use more tiles), and the
+ * status flags of examd_hip_forces otherwise.  Every call on one GPU uses
+ * that GPU's single array: calls that overlap on one GPU (several host
+ * threads) race and must be serialised by the caller. */
+int examd_hip_forces_static(const examd_params *params, int precision,
+                            const double *pos, double *forces,
+                            examd_stats *stats);
+
 int examd_hip_forces(examd_handle *h, int precision, double *forces,
                      examd_stats *stats);
 
