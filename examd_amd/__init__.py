@@ -48,7 +48,7 @@ EXPORTS = (
     "examd_hip_batch_create", "examd_hip_batch_info",
     "examd_hip_generate_vacancy_fcc", "examd_hip_generate_slab",
     "examd_hip_set_species", "examd_hip_set_pair_table", "examd_hip_get_cuts",
-    "examd_hip_forces_static",
+    "examd_hip_forces_static", "examd_hip_regrow_count",
 )
 
 MAX_SPECIES = 8
@@ -123,6 +123,7 @@ def lib():
         L.examd_hip_set_state.argtypes = [C.c_void_p, dp, dp, C.c_int64]
         L.examd_hip_get_state.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64]
         L.examd_hip_forces.argtypes = [C.c_void_p, C.c_int, dp, P(Stats)]
+        L.examd_hip_regrow_count.argtypes = [C.c_void_p, ip64]
         L.examd_hip_forces_static.argtypes = [P(Params), C.c_int, C.c_void_p,
                                               C.c_void_p, P(Stats)]
         L.examd_hip_run.argtypes = [C.c_void_p, C.c_int, C.c_int64,
@@ -349,6 +350,14 @@ class _Handle:
                                    float(dt), int(energy_every), out, cap,
                                    C.byref(nout)), "examd_hip_run")
         return [out[i] for i in range(nout.value)]
+
+    def regrow_count(self):
+        """Chunks of run() that ran again from their start with a larger
+        LDS capacity after a cell-block region outgrew it."""
+        n = C.c_int64()
+        _check(lib().examd_hip_regrow_count(self._h, C.byref(n)),
+               "examd_hip_regrow_count")
+        return n.value
 
     def set_graph_chunk(self, chunk):
         """Steps per captured hipGraph chunk of run() (0 = launch eagerly)."""

@@ -105,6 +105,9 @@ struct examd_handle {
   int64_t since_repartition[2] = {0, 0};  // steps run since, per precision
   int *host_start = nullptr;      // pinned copy of start[] for re-partitions
   double *dev_stage = nullptr;    // 3 n doubles: x y z per atom, host layout
+  double *chunk_start = nullptr;  // 6 n doubles: x y z vx vy vz at the start
+                                  // of the chunk a run is replaying
+  int64_t regrows = 0;            // chunks run again with a larger capacity
   double *host_stage = nullptr;   // pinned 3 n doubles, the same layout
 };
 
@@ -174,6 +177,8 @@ void free_state(examd_handle *h) {
   if (h->host_start) (void)hipHostFree(h->host_start);
   h->host_start = nullptr;
   if (h->dev_stage) (void)hipFree(h->dev_stage);
+  if (h->chunk_start) (void)hipFree(h->chunk_start);
+  h->chunk_start = nullptr;
   h->dev_stage = nullptr;
   if (h->host_stage) (void)hipHostFree(h->host_stage);
   h->host_stage = nullptr;
@@ -563,6 +568,52 @@ int repartition(examd_handle *h, int prec, bool *changed) {
   if (st == EXAMD_OK) st = ensure_partials(h);
   *changed = st == EXAMD_OK;
   return st;
+}
+
+// After a chunk whose largest cell-block region outgrew the LDS capacity:
+// a new capacity planned from the block counts of the chunk's last binning
+// pass (with the configured slack) and at least 5/4 of the old one per
+// attempt, or a finer map when the map was chosen automatically.  Leaves the
+// map unchanged and returns EXAMD_ERR_TILE_OVERFLOW when no capacity fits.
+int regrow(examd_handle *h, int prec, int attempt) {
+  if (!h->host_start &&
+      hipHostMalloc(reinterpret_cast<void **>(&h->host_start),
+                    (h->nblocks + 1) * sizeof(int)) != hipSuccess) {
+    h->host_start = nullptr;
+    return EXAMD_ERR_NOMEM;
+  }
+  HIPCHK(hipMemcpyAsync(h->host_start, h->d.start, (h->nblocks + 1) * sizeof(int),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
+  for (int64_t b = 0; b < h->nblocks; ++b)
+    cnt[b] = h->host_start[b + 1] - h->host_start[b];
+  const TileConfig saved = h->tc[prec];
+  TileConfig &c = h->tc[prec];
+  if (choose_caps_counts(h, prec, cnt.data(), nullptr, false) != EXAMD_OK) {
+    c = saved;
+    return EXAMD_ERR_TILE_OVERFLOW;
+  }
+  // a grown capacity when the map is the same one
+  if (c.T == saved.T && c.cuts == saved.cuts) {
+    int64_t want = (int64_t)saved.cap;
+    for (int a = 0; a < attempt; ++a) want = want * 5 / 4;
+    want = (want + 15) / 16 * 16;
+    int64_t cap = std::max<int64_t>(c.cap, want);
+    const bool fp64 = prec == EXAMD_PREC_FP64;
+    while (cap > c.cap &&
+           (cap > 65535 ||
+            force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads, c.lane_queue,
+                            h->d.ntypes > 0) > h->lds_limit))
+      cap -= 16;
+    if (cap <= saved.cap) {
+      c = saved;
+      return EXAMD_ERR_TILE_OVERFLOW;
+    }
+    c.cap = (int)cap;
+  }
+  h->regrows += 1;
+  return upload_tables(h, prec);
 }
 
 ForceConfig force_config(const examd_handle *h, int prec) {
@@ -1620,36 +1671,106 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
     }
     return st;
   };
+  // A chunk whose largest cell-block region outgrew the LDS capacity is
+  // run again from its start with a larger capacity (or a finer automatic
+  // map): positions and velocities are copied aside before each chunk, and
+  // after an overflow they are put back, the forces are evaluated again
+  // under the new map and the chunk's tallies are dropped.  The result is
+  // that of a run started with the larger capacity.
+  const int64_t n = h->n;
+  if (!h->chunk_start && dalloc(&h->chunk_start, 6 * std::max<int64_t>(n, 1)))
+    return EXAMD_ERR_NOMEM;
+  double *const saved_arrays[6] = {h->d.x, h->d.y, h->d.z, h->d.vx, h->d.vy, h->d.vz};
+  auto save_chunk_start = [&]() -> int {
+    for (int a = 0; a < 6; ++a)
+      HIPCHK(hipMemcpyAsync(h->chunk_start + a * n, saved_arrays[a], n * sizeof(double),
+                            hipMemcpyDeviceToDevice, h->stream));
+    return EXAMD_OK;
+  };
+  constexpr int kMaxRegrow = 4;
+  // after a harvest: 1 = the chunk is to be run again, 0 = go on
+  // after a harvest: 1 = the chunk is to be run again, 0 = go on, -1 = a
+  // HIP error
+  auto regrown = [&](int &attempt, int64_t k0, unsigned long long fl0) -> int {
+    if (!(fl & kFlagOverflow) || (fl0 & kFlagOverflow) || attempt >= kMaxRegrow)
+      return 0;
+    for (int a = 0; a < 6; ++a)
+      if (hipMemcpyAsync(saved_arrays[a], h->chunk_start + a * n, n * sizeof(double),
+                         hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+        return -1;
+    k = k0;
+    fl = fl0;
+    if (regrow(h, precision, ++attempt) != EXAMD_OK) {
+      fl |= kFlagOverflow;   // the state is the chunk's start
+      return 0;
+    }
+    launch_fill_log(h->log, ns, h->stream);
+    initial_forces();
+    std::vector<StepLog> L0((size_t)ns);
+    if (hipMemcpyAsync(L0.data(), h->log, ns * sizeof(StepLog),
+                       hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      return -1;
+    for (const StepLog &q : L0) fl |= q.flags;
+    return 1;
+  };
   for (int64_t c = 0; c < nchunks; ++c) {
-    // re-captured only when a re-partition changed the map
-    GraphKey key{precision, C, energy_every > 0 ? energy_every : 0, dt,
-                 h->timing, h->timing_every, h->tc[precision]};
-    int st = ensure_graph(h, key);
-    if (st) return st;
-    HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
-    h->ev_used = h->graph_events;   // the events the replay recorded
-    st = harvest(c * C, C);
-    if (st) return st;
+    int attempt = 0;
+    for (;;) {
+      // re-captured only when a re-partition or a regrow changed the map
+      GraphKey key{precision, C, energy_every > 0 ? energy_every : 0, dt,
+                   h->timing, h->timing_every, h->tc[precision]};
+      int st = ensure_graph(h, key);
+      if (st) return st;
+      if ((st = save_chunk_start()) != EXAMD_OK) return st;
+      const int64_t k0 = k;
+      const unsigned long long fl0 = fl;
+      HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
+      h->ev_used = h->graph_events;   // the events the replay recorded
+      st = harvest(c * C, C);
+      if (st) return st;
+      const int again = regrown(attempt, k0, fl0);
+      if (again < 0) return EXAMD_ERR_HIP;
+      if (!again) break;
+    }
     if (fl & fatal) break;
-    if ((st = after_steps(C)) != EXAMD_OK) return st;
+    const int st = after_steps(C);
+    if (st != EXAMD_OK) return st;
   }
   // the rest eagerly, in pieces that end where a re-partition is due
   for (int64_t base = done_by_graph; base < nsteps && !(fl & fatal);) {
     int64_t count = nsteps - base;
     if (R > 0) count = std::min(count, std::max<int64_t>(1, R - since));
-    enqueue_steps(h, precision, count, dt,
-                  [&](int64_t c) { return is_energy(base + c); });
-    HIPCHK(hipGetLastError());
-    int st = harvest(base, count);
-    if (st) return st;
+    int attempt = 0;
+    for (;;) {
+      int st = save_chunk_start();
+      if (st) return st;
+      const int64_t k0 = k;
+      const unsigned long long fl0 = fl;
+      enqueue_steps(h, precision, count, dt,
+                    [&](int64_t c) { return is_energy(base + c); });
+      HIPCHK(hipGetLastError());
+      st = harvest(base, count);
+      if (st) return st;
+      const int again = regrown(attempt, k0, fl0);
+      if (again < 0) return EXAMD_ERR_HIP;
+      if (!again) break;
+    }
     base += count;
-    if ((st = after_steps(count)) != EXAMD_OK) return st;
+    const int st = after_steps(count);
+    if (st != EXAMD_OK) return st;
   }
   h->forces_valid = precision;
   if (n_out) *n_out = k;
   int st = flags_to_status(fl);
   if (st != EXAMD_OK) h->forces_valid = -1;
   return st;
+}
+
+int examd_hip_regrow_count(examd_handle *h, int64_t *regrows) {
+  if (!h || !regrows) return EXAMD_ERR_ARG;
+  *regrows = h->regrows;
+  return EXAMD_OK;
 }
 
 int examd_hip_set_graph_chunk(examd_handle *h, int64_t chunk) {

@@ -127,6 +127,13 @@ Module exaMD_hip_binding
        Type(examd_stats), Intent(out) :: stats
      End Function examd_hip_forces
 
+     Integer(c_int) Function examd_hip_regrow_count(handle, regrows) &
+          Bind(C, name="examd_hip_regrow_count")
+       Import :: c_int, c_ptr, c_int64_t
+       Type(c_ptr), Value :: handle
+       Integer(c_int64_t), Intent(out) :: regrows
+     End Function examd_hip_regrow_count
+
      !! pos and forces are device addresses (3, natoms); stats may be
      !! c_null_ptr.  Allocates nothing (a static device array per GPU).
      Integer(c_int) Function examd_hip_forces_static(params, precision, pos, &

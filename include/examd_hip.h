@@ -220,6 +220,11 @@ int examd_hip_forces_static(const examd_params *params, int precision,
                             const double *pos, double *forces,
                             examd_stats *stats);
 
+/* Chunks of examd_hip_run calls on this handle that ran again from their
+ * start because a cell-block region outgrew the LDS capacity (no reference
+ * counterpart: the reference's blocks live in host memory). */
+int examd_hip_regrow_count(examd_handle *h, int64_t *regrows);
+
 int examd_hip_forces(examd_handle *h, int precision, double *forces,
                      examd_stats *stats);
 
