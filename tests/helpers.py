@@ -257,3 +257,51 @@ def slab_system(E, natoms, seed, min_sep=0.8, temperature=0.85):
     pos, vel = E.generate_slab(box, ly, 2 * ly, 0.80, 0.05, min_sep=min_sep,
                                temperature=temperature, seed=seed)
     return pos, vel, box
+
+
+def pair_force_sums(pos, box, rc, eps=1.0, sig=1.0):
+    """Per atom, the sum of the magnitudes of its LJ pair forces within rc
+    (the scale an FP32 evaluation's rounding is relative to), and the
+    smallest pair distance.  Periodic minimum image (rc < L/2)."""
+    from scipy.spatial import cKDTree
+    box = np.asarray(box, dtype=np.float64)
+    p = np.mod(np.asarray(pos, dtype=np.float64), box)
+    p = np.where(p >= box, 0.0, p)
+    tree = cKDTree(p, boxsize=box)
+    pairs = tree.query_pairs(rc, output_type="ndarray")
+    d = p[pairs[:, 1]] - p[pairs[:, 0]]
+    d -= np.round(d / box) * box
+    r2 = (d * d).sum(1)
+    s6 = (sig * sig / r2) ** 3
+    fmag = np.abs(24.0 * eps * (2.0 * s6 * s6 - s6)) / np.sqrt(r2)
+    out = np.zeros(len(p))
+    np.add.at(out, pairs[:, 0], fmag)
+    np.add.at(out, pairs[:, 1], fmag)
+    return out, float(np.sqrt(r2.min()))
+
+
+def region_bound(cuts, bl):
+    """A bound on |p| over every cell-block region of a map (coordinates
+    relative to the cell-block centre): half the widest cell block plus two
+    blocks per axis."""
+    return float(np.sqrt(sum(((0.5 * np.diff(c).max() + 2.0) * bl) ** 2
+                             for c in cuts)))
+
+
+def mixed_error_bound(f, ref, pos, box, rc, R, ncross=0, eps=1.0, sig=1.0):
+    """Largest per-atom error of FP32 pair forces relative to the bound the
+    arithmetic implies: FP32 coordinates relative to a cell-block centre,
+    with |p| <= R, put up to 2^-24 R into each difference, which moves a
+    pair force that goes as r^-13 by about 13 * 2^-24 R / r of itself, so
+    an atom's error is at most that times the sum of its pair forces'
+    magnitudes; and each of the ncross pairs that FP32 r^2 puts on the
+    other side of the cutoff than FP64 adds or drops the force at the
+    cutoff, |F(rc)| (0.039 at rc = 2.5), on its two atoms.  Returns
+    max_i |dF_i| / bound_i."""
+    s, rmin = pair_force_sums(pos, box, rc, eps, sig)
+    err = np.sqrt(((np.asarray(f) - ref) ** 2).sum(1))
+    s6 = (sig / rc) ** 6
+    fc = abs(24.0 * eps * (2.0 * s6 * s6 - s6) / rc)
+    bound = (13.0 * 2.0 ** -24 * R / rmin * np.maximum(s, 1e-30) +
+             ncross * 1.01 * fc)
+    return float((err / bound).max())

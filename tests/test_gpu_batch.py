@@ -10,6 +10,8 @@ evaluations and over MD runs replayed from captured graphs.
 import numpy as np
 import pytest
 
+from tests import helpers as H
+
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-10
@@ -257,3 +259,42 @@ def test_balanced_batch_equals_standalone(E, oracle):
         coarser += T != T_edge
         assert np.array_equal(fb[off[k]:off[k + 1]], f1), k
     assert coarser > 0
+
+
+def test_batch_log_uniform_32_systems_match_oracle(E, oracle):
+    """The batching workload's shape at 32 systems: atom counts log-uniform
+    in [500, 20000], densities uniform in [0.60, 0.90], temperatures uniform
+    in [0.70, 1.50] (randomly depleted FCC lattices), under maps from a tile
+    edge coarsened within the batch's largest region, boundaries at equal
+    atom shares: every system's FP64 forces within 1e-10 of the oracle on
+    that system alone, pair counts exact; the mixed forces within the FP32
+    bars and summing to zero per system in fixed point."""
+    rng = np.random.default_rng(32)
+    n = np.rint(np.exp(rng.uniform(np.log(500), np.log(20000), 32))).astype(int)
+    rho = rng.uniform(0.60, 0.90, 32)
+    temp = rng.uniform(0.70, 1.50, 32)
+    sy = [E.generate_vacancy_fcc(int(n[k]), rho[k], temp[k], perturb=0.05,
+                                 seed=3200 + k) for k in range(32)]
+    print(f"[batch 32] atoms {sum(len(p) for p, _, _ in sy)}", flush=True)
+    with _batch(E, sy) as b:
+        for prec in ("fp64", "mixed"):
+            b.set_config(prec, block_threads=512, tile_edge=5, balance=1)
+        f64, s64, _ = b.forces("fp64")
+        fmx, smx, _ = b.forces("mixed")
+        off = b.offsets
+        R = [H.region_bound(b.get_cuts("mixed", k), 1.25) for k in range(32)]
+    for k, (pos, _, box) in enumerate(sy):
+        ref, t, _ = oracle.forces(pos, box, 2.5)
+        a, z = off[k], off[k + 1]
+        assert _rel(f64[a:z], ref) < FP64_TOL, k
+        assert (s64[k].pairs_cand, s64[k].pairs_cut) == (t.cand, t.cut), k
+        # FP32 rounding at the scale of the system's (coarsened) regions,
+        # relative to each atom's pair forces (a perturbed lattice's net
+        # forces are small beside them), with a factor 2 of margin
+        ncross = abs(smx[k].pairs_cut - t.cut)
+        assert ncross <= 2, k
+        assert H.mixed_error_bound(fmx[a:z], ref, pos, box, 2.5, R[k],
+                                   ncross) < 2.0, k
+        assert smx[k].pairs_cand == t.cand, k
+        q = np.round(fmx[a:z] * 2.0 ** 32).astype(np.int64)
+        assert np.all(q.sum(0) == 0), k
