@@ -54,7 +54,9 @@ def algorithmic_flops(cand, cut, energy):
 
 def drift_per_1k(stats):
     """Least-squares slope of the shifted total energy per 1,000 steps,
-    relative to |E|."""
+    relative to |E| (None when the accuracy leg was not run)."""
+    if stats is None:
+        return None
     if len(stats) < 3:
         raise RuntimeError(f"drift needs 3 or more tallies, got {len(stats)}")
     t = np.array([s.step for s in stats], dtype=float)
@@ -194,7 +196,8 @@ def timed_run(E, pos, vel, box, prec, steps, warmup, energy_every,
         force_ms, launches = m.timing(enable=False, reset=True)
         # accuracy leg, in 100-step chunks
         m.set_graph_chunk(graph_chunk if graph_chunk > 0 else 0)
-        drift = m.run(DRIFT_STEPS[prec], DT, prec, energy_every=DRIFT_EVERY)
+        drift = (m.run(DRIFT_STEPS[prec], DT, prec, energy_every=DRIFT_EVERY)
+                 if DRIFT_STEPS[prec] > 0 else None)
         if report_error:
             err = force_errors(m, box, oracle_threads)
             ms, ms_res = boundary_ms(m, prec)
@@ -538,6 +541,9 @@ def main():
     ap.add_argument("--slab-steps", type=int, default=2000)
     ap.add_argument("--species-steps", type=int, default=2000)
     ap.add_argument("--table-steps", type=int, default=1000)
+    ap.add_argument("--drift-steps", default="10000,1000",
+                    help="mixed,fp64 steps of the accuracy leg (0,0 skips it, "
+                         "for profiler runs)")
     ap.add_argument("--graph-chunk", type=int, default=100,
                     help="MD steps per captured hipGraph chunk (0 = eager)")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -565,6 +571,8 @@ def main():
     fsteps = args.fp64_steps if args.fp64_steps is not None else steps
     energy_every = 10
 
+    dm, df = (int(x) for x in args.drift_steps.split(","))
+    DRIFT_STEPS["mixed"], DRIFT_STEPS["fp64"] = dm, df
     host_threads = max(1, min(16, os.cpu_count() or 1))
     (wall_mx, st_mx, force_ms_mx, launches, tune_mx, ferr, drift_mx,
      chunk_mx) = timed_run(E, pos, vel, box, "mixed", steps, args.warmup,
