@@ -293,3 +293,45 @@ def test_driver_readfile_forces_per_atom(tmp_path, oracle, call):
     err = np.sqrt(((fm - ref) ** 2).sum(1))
     assert err.max() / scale < 2e-5
     assert np.all(np.round(fm * 2.0 ** 32).astype(np.int64).sum(0) == 0)
+
+
+@pytest.mark.gpu
+def test_driver_status_codes(tmp_path, oracle):
+    """The Fortran layer's handling of the device statuses, as the
+    reference behaves: a coincident pair (0.02 sigma apart) is reported and
+    the forces still come back (the reference prints "ZERO at" and goes on,
+    SRC/exaMD_forces.f90:525-527); on the mixed path that pair is beyond
+    the fixed-point range and the program stops with code 102; an atom
+    outside the box stops it with 109 (SRC/exaMD_module.f90:118-123)."""
+    pos, box = _particle_file(tmp_path, oracle)
+    pos = pos.copy()
+    pos[5] = pos[9] + np.array([0.02, 0.0, 0.0])
+    pos[5] = np.where(pos[5] >= box, pos[9] - np.array([0.02, 0.0, 0.0]), pos[5])
+    oracle.write_particles(tmp_path / "exaMD_particles.dat", pos, box)
+    pos, box = oracle.read_particles(tmp_path / "exaMD_particles.dat")
+    ref, t, _ = oracle.forces(pos, box, 2.5)
+    assert t.zero == 1
+    print(f"[fortran status] zero-distance pair, n={len(pos)}", flush=True)
+    r = subprocess.run([_driver(), "-readfile", "-dumpforces", "f64.txt"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    out = r.stdout
+    assert r.returncode == 0, out[-2000:] + r.stderr
+    assert "(status 10)" in out
+    m = re.search(r"Pairs: candidate (\d+), in cutoff (\d+)", out)
+    assert (int(m.group(1)), int(m.group(2))) == (t.cand, t.cut)
+    f = _read_dump(tmp_path / "f64.txt", pos)
+    scale = np.sqrt((ref ** 2).sum(1).mean())
+    assert np.abs(f - ref).max() < 1e-10 * scale
+    print("[fortran status] the same pair, mixed", flush=True)
+    r = subprocess.run([_driver(), "-readfile", "-precision", "mixed"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "(status 9)" in r.stdout and "aborting with code 102" in r.stdout
+    print("[fortran status] an atom outside the box", flush=True)
+    bad = pos.copy()
+    bad[3, 2] = -2.0 * 0.5 * 2.5   # block -1 along z: block number < 1
+    oracle.write_particles(tmp_path / "exaMD_particles.dat", bad, box)
+    r = subprocess.run([_driver(), "-readfile"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "aborting with code 109" in r.stdout
