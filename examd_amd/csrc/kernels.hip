@@ -565,55 +565,31 @@ struct RunAxisMasks {
           m[r][a][o + 1] = (unsigned char)axis_mask(r, a, o);
   }
 };
-// The same masks packed per (run, axis): byte o + 1 holds the mask for
-// o1 = o, so one 32-bit word per axis serves every o1.
-struct RunAxisPacked {
-  unsigned p[kNumRuns][3];
-  constexpr RunAxisPacked() : p() {
-    const RunAxisMasks M;
-    for (int r = 0; r < kNumRuns; ++r)
-      for (int a = 0; a < 3; ++a)
-        p[r][a] = (unsigned)M.m[r][a][0] | ((unsigned)M.m[r][a][1] << 8) |
-                  ((unsigned)M.m[r][a][2] << 16);
-  }
-};
-__constant__ RunAxisPacked kRunPacked;
-
-// Byte selector of an atom's own_bits: 0xff in byte k when bit k is set.
-__device__ __forceinline__ unsigned own_spread(unsigned ok) {
-  return ((ok & 1u) ? 0xffu : 0u) | ((ok & 2u) ? 0xff00u : 0u) |
-         ((ok & 4u) ? 0xff0000u : 0u);
-}
-__device__ __forceinline__ unsigned axis_or(unsigned v) {
-  return (v | (v >> 8) | (v >> 16)) & 0x1fu;
-}
 // Ownership mask of run r (bit k <-> dx = k - 2) for an I block whose
-// per-axis own_bits have the byte selectors sx, sy, sz.  r is uniform over
-// the wave, so the table words are scalar loads.
-__device__ __forceinline__ unsigned run_mask_rt(int r, unsigned sx,
-                                                unsigned sy, unsigned sz) {
-  return axis_or(kRunPacked.p[r][0] & sx) & axis_or(kRunPacked.p[r][1] & sy) &
-         axis_or(kRunPacked.p[r][2] & sz);
+// per-axis own_bits are the fields of key = ox + 8 oy + 64 oz: per axis the
+// union over the allowed o1, then the intersection of the axes.
+constexpr unsigned run_mask_of(const RunAxisMasks &M, int key, int r) {
+  const unsigned ox = key & 7, oy = (key >> 3) & 7, oz = key >> 6;
+  unsigned mx = 0, my = 0, mz = 0;
+  for (int o = -1; o <= 1; ++o) {
+    if (ox & (1u << (o + 1))) mx |= M.m[r][0][o + 1];
+    if (oy & (1u << (o + 1))) my |= M.m[r][1][o + 1];
+    if (oz & (1u << (o + 1))) mz |= M.m[r][2][o + 1];
+  }
+  return mx & my & mz & 31u;
 }
 
-// Owned J blocks of an atom as a function of its per-axis own_bits
-// (ox + 8 oy + 64 oz): the sum over the 13 runs of the popcount of the
-// ownership mask.  Used to sort a cell block's atoms by work.
+// Owned J blocks of an atom as a function of its own_bits key: the sum over
+// the 13 runs of the popcount of the ownership mask.  Used to sort a cell
+// block's atoms by work.
 struct OwnedWork {
   unsigned char v[512];
   constexpr OwnedWork() : v() {
     const RunAxisMasks M;
     for (int i = 0; i < 512; ++i) {
-      unsigned ox = i & 7, oy = (i >> 3) & 7, oz = i >> 6;
       int n = 0;
       for (int r = 0; r < kNumRuns; ++r) {
-        unsigned mx = 0, my = 0, mz = 0;
-        for (int o = -1; o <= 1; ++o) {
-          if (ox & (1u << (o + 1))) mx |= M.m[r][0][o + 1];
-          if (oy & (1u << (o + 1))) my |= M.m[r][1][o + 1];
-          if (oz & (1u << (o + 1))) mz |= M.m[r][2][o + 1];
-        }
-        unsigned m = mx & my & mz;
+        const unsigned m = run_mask_of(M, i, r);
         for (int k = 0; k < 5; ++k) n += (m >> k) & 1u;
       }
       v[i] = (unsigned char)n;
@@ -621,6 +597,20 @@ struct OwnedWork {
   }
 };
 __constant__ OwnedWork kOwnedWork;
+
+// The ownership masks of the 13 runs for each own_bits key: byte r of the
+// four words holds run r's mask, so a work item loads its masks once and
+// each run extracts one byte.
+struct RunMaskTable {
+  unsigned w[512][4];
+  constexpr RunMaskTable() : w() {
+    const RunAxisMasks M;
+    for (int i = 0; i < 512; ++i)
+      for (int r = 0; r < kNumRuns; ++r)
+        w[i][r >> 2] |= run_mask_of(M, i, r) << ((r & 3) * 8);
+  }
+};
+__constant__ __align__(16) RunMaskTable kRunMask;
 
 // Run groups: the 13 runs split into 1..4 groups of similar candidate
 // counts (run 0 covers 2.5 blocks, the others 5).
@@ -877,9 +867,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       acc += c;
     }
     s_hist[kWorkBins] = acc;
+    s_wave[31] = NW;   // next wave-item of the pair phase
   }
   __syncthreads();
-  const int RW = s_hist[kWorkBins];   // atoms that own at least one block
+  // atoms that own at least one block (uniform: kept in a scalar register)
+  const int RW = __builtin_amdgcn_readfirstlane(s_hist[kWorkBins]);
   for (int a = tid; a < R; a += NT) {
     const int w = work_of(a);
     if (w > 0) s_perm[atomicAdd(&s_hist[kWorkBins - 1 - w], 1)] = (unsigned short)a;
@@ -889,8 +881,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   // 3. pairs.  Work item w = (run group g, atom perm[i]): the lane lists the
   //    J ranges atom a owns in the group's runs (home block I - o1 inside
   //    the tile) and tests them a chunk at a time.  Each run group
-  //    takes whole waves, so the group and run indices are wave-uniform and
-  //    the run tables are scalar loads.  In-cutoff pairs are either
+  //    takes whole waves, so the group and run indices are wave-uniform:
+  //    the run tables are scalar loads and each run's row offset is scalar
+  //    arithmetic; the lane loads its 13 ownership masks once per work
+  //    item (kRunMask).  In-cutoff pairs are either
   //    evaluated at once or, with lane queues, deferred into a per-lane LDS
   //    queue (one entry per chunk with a hit: j << chunk | hit bits)
   //    that the wave empties together, one pair per lane per step.  The
@@ -915,7 +909,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   const bool LQ = QD > 0;
   unsigned *my_q = reinterpret_cast<unsigned *>(smem + LL.off_q) +
                    (size_t)(tid >> 6) * 64 * QD + (tid & 63);
-  for (int wb = tid & ~63; wb < W; wb += NT) {
+  // Wave-items are taken dynamically: wave w starts with item w, then
+  // takes the next free one from a counter in LDS.  The work order puts
+  // the heaviest items first, so the waves reach the barrier together.
+  for (int wb = __builtin_amdgcn_readfirstlane(tid & ~63); wb < W;) {
     const int grp = __builtin_amdgcn_readfirstlane(wb / RWp);
     const int ia = wb - grp * RWp + (tid & 63);
     const bool active = ia < RW;
@@ -941,8 +938,12 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     // an idle lane (group padding) owns nothing
     const unsigned okx = active ? own_bits(ix, tx) : 0u, oky = own_bits(iy, ty),
                    okz = own_bits(iz, tz);
-    const unsigned spx = own_spread(okx), spy = own_spread(oky),
-                   spz = own_spread(okz);
+    const uint4 mw = *reinterpret_cast<const uint4 *>(
+        kRunMask.w[okx | (oky << 3) | (okz << 6)]);
+    // region row of J block (ix - 2, iy, iz); a run adds its (dy, dz) rows
+    const int RXs = __builtin_amdgcn_readfirstlane(RX);
+    const int RYs = __builtin_amdgcn_readfirstlane(RY);
+    const int rowbase = RX * (iy + RY * iz) + ix - 2;
     const int sax = bc_s(bc, 0), say = bc_s(bc, 1), saz = bc_s(bc, 2);
     double fax = 0.0, fay = 0.0, faz = 0.0;
     long long qax = 0, qay = 0, qaz = 0;
@@ -1163,11 +1164,13 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
 
     const int r0 = kGroupStart[NG][grp], r1 = kGroupStart[NG][grp + 1];
     for (int r = r0; r < r1; ++r) {
-      unsigned mask = run_mask_rt(r, spx, spy, spz);
-      const int jy = iy + kRunDY[r], jz = iz + kRunDZ[r];
-      const int row0 = RX * (jy + RY * jz) + ix - 2;
+      const unsigned mwr = r < 4 ? mw.x : (r < 8 ? mw.y : (r < 12 ? mw.z : mw.w));
+      unsigned mask = (mwr >> ((r & 3) * 8)) & 31u;
+      // the run's row offset is uniform: computed on the scalar unit
+      const int row0 = rowbase + RXs * (kRunDY[r] + RYs * kRunDZ[r]);
       double bty = 0.0, btz = 0.0;
       if constexpr (FP64) {
+        const int jy = iy + kRunDY[r], jz = iz + kRunDZ[r];
         if (wraps) {
           const int sjy = (wly && jy == 0) ? -1 : ((why && jy == RY - 1) ? 1 : 0);
           const int sjz = (wlz && jz == 0) ? -1 : ((whz && jz == RZ - 1) ? 1 : 0);
@@ -1257,8 +1260,9 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
           }
         }
       }
-      // runs end together: empty the queues when one is half full
-      if (LQ) {
+      // after every second run (and the group's last): drain part of the
+      // queues once the wave holds two pending pairs per lane on average
+      if (LQ && (((r - r0) & 1) || r + 1 == r1)) {
         const int P = wave_total(pend);
         if (P >= 64 * kDrainMin) drain_some(P >> 6);
       }
@@ -1275,6 +1279,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         lds_add(&s_q[3 * a + 2], qaz);
       }
     }
+    // the wave's next item: the first free one (heaviest first)
+    int nx = 0;
+    if ((tid & 63) == 0) nx = atomicAdd(&s_wave[31], 1);
+    wb = __builtin_amdgcn_readlane(nx, 0) * 64;
   }
   __syncthreads();
   if (prof && tid == 0) prof[3] = clock64();
