@@ -494,7 +494,7 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
 // arrays hold cap + kChunkPad entries: the pair loop reads up to sixteen
 // candidates at a time and may look up to fifteen entries past the last
 // atom (masked off).
-constexpr int kWorkBins = 64;   // owned J blocks of an atom: 0..63
+constexpr int kWorkBins = 256;  // work-order classes (own_bits keys by work)
 constexpr int kMixedChunk = 16; // mixed candidates per step
 constexpr int kDrainMin = 2;    // partial drain once the wave's lanes hold
                                 // this many pending pairs each on average
@@ -531,7 +531,7 @@ __host__ __device__ inline LdsLayout lds_layout(bool fp64, int cap, int maxrb,
   o = align_up(o, 4);
   L.off_perm = o;
   o += (size_t)cap * 2;
-  o = align_up(o, 4);
+  o = align_up(o, 16);   // the class counts are scanned as int4
   L.off_hist = o;
   o += (kWorkBins + 1) * 4;
   o = align_up(o, 16);
@@ -580,8 +580,8 @@ constexpr unsigned run_mask_of(const RunAxisMasks &M, int key, int r) {
 }
 
 // Owned J blocks of an atom as a function of its own_bits key: the sum over
-// the 13 runs of the popcount of the ownership mask.  Used to sort a cell
-// block's atoms by work.
+// the 13 runs of the popcount of the ownership mask (the work order's
+// primary key, below).
 struct OwnedWork {
   unsigned char v[512];
   constexpr OwnedWork() : v() {
@@ -596,7 +596,6 @@ struct OwnedWork {
     }
   }
 };
-__constant__ OwnedWork kOwnedWork;
 
 // The ownership masks of the 13 runs for each own_bits key: byte r of the
 // four words holds run r's mask, so a work item loads its masks once and
@@ -611,6 +610,31 @@ struct RunMaskTable {
   }
 };
 __constant__ __align__(16) RunMaskTable kRunMask;
+
+// Work-order class of each own_bits key: the keys that own at least one J
+// block ranked by owned blocks (most first), then by key, so that atoms
+// with the same ownership masks are adjacent; 255 for keys that own
+// nothing.  Per axis only the own_bits 1, 2, 3, 4, 6 and 7 occur, so at
+// most 216 classes are used.
+struct KeyRank {
+  unsigned char r[512];
+  constexpr KeyRank() : r() {
+    const OwnedWork W;
+    int next = 0;
+    for (int w = 63; w >= 1; --w)
+      for (int k = 0; k < 512; ++k) {
+        const unsigned ox = k & 7, oy = (k >> 3) & 7, oz = k >> 6;
+        const bool ok = ox != 0 && ox != 5 && oy != 0 && oy != 5 && oz != 0 && oz != 5;
+        if (ok && W.v[k] == w) r[k] = (unsigned char)next++;
+      }
+    for (int k = 0; k < 512; ++k) {
+      const unsigned ox = k & 7, oy = (k >> 3) & 7, oz = k >> 6;
+      const bool ok = ox != 0 && ox != 5 && oy != 0 && oy != 5 && oz != 0 && oz != 5;
+      if (!ok || W.v[k] == 0) r[k] = 255;
+    }
+  }
+};
+__constant__ KeyRank kKeyRank;
 
 // Run groups: the 13 runs split into 1..4 groups of similar candidate
 // counts (run 0 covers 2.5 blocks, the others 5).
@@ -842,39 +866,46 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   if (prof && tid == 0) prof[2] = clock64();
 
   // 3a. order the region atoms by the number of J blocks they own, most
-  //     first, so that the lanes of a wave carry similar work; atoms that
-  //     own nothing (outer ghost layers) drop out.
+  //     first, and atoms with the same ownership masks together, so that
+  //     the lanes of a wave carry similar work and walk the same runs;
+  //     atoms that own nothing (outer ghost layers) drop out.
   unsigned short *s_perm = reinterpret_cast<unsigned short *>(smem + LL.off_perm);
   int *s_hist = reinterpret_cast<int *>(smem + LL.off_hist);
   for (int b = tid; b <= kWorkBins; b += NT) s_hist[b] = 0;
   __syncthreads();
-  auto work_of = [&](int a) {
+  auto class_of = [&](int a) {
     const int bc = s_bc[a];
     const unsigned key = own_bits(bc_x(bc), tx) | (own_bits(bc_y(bc), ty) << 3) |
                          (own_bits(bc_z(bc), tz) << 6);
-    return (int)kOwnedWork.v[key];
+    return (int)kKeyRank.r[key];
   };
   for (int a = tid; a < R; a += NT) {
-    const int w = work_of(a);
-    if (w > 0) atomicAdd(&s_hist[kWorkBins - 1 - w], 1);
+    const int c = class_of(a);
+    if (c != 255) atomicAdd(&s_hist[c], 1);
   }
   __syncthreads();
-  if (tid == 0) {
-    int acc = 0;
-    for (int b = 0; b < kWorkBins; ++b) {
-      const int c = s_hist[b];
-      s_hist[b] = acc;
-      acc += c;
+  // exclusive scan of the 256 class counts by the first wave, four per lane
+  if (tid < 64) {
+    const int4 c = reinterpret_cast<const int4 *>(s_hist)[tid];
+    int v = c.x + c.y + c.z + c.w;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(v, o, 64);
+      if (tid >= o) v += t;
     }
-    s_hist[kWorkBins] = acc;
-    s_wave[31] = NW;   // next wave-item of the pair phase
+    const int ex = v - (c.x + c.y + c.z + c.w);
+    reinterpret_cast<int4 *>(s_hist)[tid] =
+        make_int4(ex, ex + c.x, ex + c.x + c.y, ex + c.x + c.y + c.z);
+    if (tid == 63) {
+      s_hist[kWorkBins] = v;
+      s_wave[31] = NW;   // next wave-item of the pair phase
+    }
   }
   __syncthreads();
   // atoms that own at least one block (uniform: kept in a scalar register)
   const int RW = __builtin_amdgcn_readfirstlane(s_hist[kWorkBins]);
   for (int a = tid; a < R; a += NT) {
-    const int w = work_of(a);
-    if (w > 0) s_perm[atomicAdd(&s_hist[kWorkBins - 1 - w], 1)] = (unsigned short)a;
+    const int c = class_of(a);
+    if (c != 255) s_perm[atomicAdd(&s_hist[c], 1)] = (unsigned short)a;
   }
   __syncthreads();
 
