@@ -495,7 +495,7 @@ __host__ __device__ inline size_t align_up(size_t v, size_t a) {
 // candidates at a time and may look up to fifteen entries past the last
 // atom (masked off).
 constexpr int kWorkBins = 256;  // work-order classes (own_bits keys by work)
-constexpr int kMixedChunk = 16; // mixed candidates per step
+constexpr int kMixedChunk = 16; // mixed candidates per step, at most
 constexpr int kDrainMin = 2;    // partial drain once the wave's lanes hold
                                 // this many pending pairs each on average
 constexpr int kChunkPad = 16;   // atom entries past the last one a chunk may read
@@ -929,8 +929,9 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   double pe64 = 0.0;
   long long pefix = 0;
   long long shiftfix = 0;   // several species, tally launches: sum of V(rc)
-  // candidates tested per step (a chunk): sixteen (mixed) or four (FP64,
-  // whose differences take twice the registers)
+  // candidates tested per step (a chunk): up to sixteen (mixed: 4, 8, 12
+  // or 16 by the wave's longest range) or four (FP64, whose differences
+  // take twice the registers)
   constexpr int kChunk = FP64 ? 4 : kMixedChunk;
   constexpr unsigned kChunkMask = (1u << kChunk) - 1u;
   const int NG = A.groups;
@@ -1218,34 +1219,44 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         const int j1 = s_off[row0 + hi + 1];
         cand += (unsigned)(j1 > j0 ? j1 - j0 : 0);
         if constexpr (!FP64) {
-          // mixed: sixteen candidates per step; hits are evaluated (at once
-          // or from the queue) from the LDS positions
-          for (int j = j0; j < j1; j += kChunk) {
-            // prefilter: rc'^2 - d^2 = (rc'^2 - |a|^2) - |p|^2 + 2 a.p by
-            // one subtraction and three FMAs; its sign bit (set beyond the
-            // cutoff) is shifted into the miss mask, last candidate first,
-            // so that bit k ends up holding candidate k
-            unsigned miss = 0u;
+          // mixed: up to sixteen candidates per step; hits are evaluated
+          // (at once or from the queue) from the LDS positions
+          auto scan = [&](auto width) {
+            constexpr int CW = decltype(width)::value;
+            for (int j = j0; j < j1; j += CW) {
+              // prefilter: rc'^2 - d^2 = (rc'^2 - |a|^2) - |p|^2 + 2 a.p by
+              // one subtraction and three FMAs; its sign bit (set beyond
+              // the cutoff) is shifted into the miss mask, last candidate
+              // first, so that bit k ends up holding candidate k
+              unsigned miss = 0u;
 #pragma unroll
-            for (int k = kChunk - 1; k >= 0; --k) {
-              const float4 p = s_p4[j + k];
-              const float t = __builtin_fmaf(
-                  tzf, p.z, __builtin_fmaf(tyf, p.y, __builtin_fmaf(txf, p.x, kaf - p.w)));
-              miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(t), 31);
-            }
-            // candidates past the range end are masked off
-            const int lim = j1 - j;
-            unsigned hits = ~miss & ((1u << min(lim, kChunk)) - 1u);
-            if (LQ) {
-              if (hits) push(j, hits);
-            } else {
-              while (hits) {
-                const int k = __builtin_ctz(hits);
-                hits &= hits - 1u;
-                eval_queued(j + k);
+              for (int k = CW - 1; k >= 0; --k) {
+                const float4 p = s_p4[j + k];
+                const float t = __builtin_fmaf(
+                    tzf, p.z, __builtin_fmaf(tyf, p.y, __builtin_fmaf(txf, p.x, kaf - p.w)));
+                miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(t), 31);
+              }
+              // candidates past the range end are masked off
+              const int lim = j1 - j;
+              unsigned hits = ~miss & ((1u << min(lim, CW)) - 1u);
+              if (LQ) {
+                if (hits) push(j, hits);
+              } else {
+                while (hits) {
+                  const int k = __builtin_ctz(hits);
+                  hits &= hits - 1u;
+                  eval_queued(j + k);
+                }
               }
             }
-          }
+          };
+          // the step width follows the longest range among the wave's
+          // active lanes, so short segments do not pay for sixteen slots
+          const int nj = j1 - j0;
+          if (!__ballot(nj > 4)) scan(std::integral_constant<int, 4>{});
+          else if (!__ballot(nj > 8)) scan(std::integral_constant<int, 8>{});
+          else if (!__ballot(nj > 12)) scan(std::integral_constant<int, 12>{});
+          else scan(std::integral_constant<int, kChunk>{});
         } else {
           for (int j = j0; j < j1; j += kChunk) {
             unsigned hits;
