@@ -35,6 +35,7 @@ ERR_FIXED_RANGE = 9
 ERR_ZERO_DISTANCE = 10
 ERR_TABLE_RANGE = 11
 ERR_STATIC_LIMIT = 12
+ERR_BIN_OVERFLOW = 13
 ERR_OUT_OF_BOX = 109
 
 # every symbol include/examd_hip.h declares

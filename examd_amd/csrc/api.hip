@@ -103,7 +103,7 @@ struct examd_handle {
   double force_ms = 0.0;
   int64_t force_launches = 0;
   int64_t since_repartition[2] = {0, 0};  // steps run since, per precision
-  int *host_start = nullptr;      // pinned copy of start[] for re-partitions
+  int *pinned_counts = nullptr;   // pinned copy of the block counts (re-partitions)
   double *dev_stage = nullptr;    // 3 n doubles: x y z per atom, host layout
   double *chunk_start = nullptr;  // 6 n doubles: x y z vx vy vz at the start
                                   // of the chunk a run is replaying
@@ -112,6 +112,11 @@ struct examd_handle {
 };
 
 namespace {
+
+// Slots per block of the atom lists, at least (more when set_state finds a
+// fuller block, or after a block ran out of slots).
+constexpr int kMinBinShift = 3;
+constexpr int64_t kMinBinSlots = (int64_t)1 << kMinBinShift;
 
 #define HIPCHK(expr)                          \
   do {                                        \
@@ -123,6 +128,7 @@ int flags_to_status(unsigned long long f) {
   if (f & kFlagOutOfBox) return EXAMD_ERR_OUT_OF_BOX;
   if (f & kFlagNonFinite) return EXAMD_ERR_NONFINITE;
   if (f & kFlagOverflow) return EXAMD_ERR_TILE_OVERFLOW;
+  if (f & kFlagBinOverflow) return EXAMD_ERR_BIN_OVERFLOW;
   // a pair beyond the fixed-point range makes the mixed forces invalid; a
   // 'ZERO at' pair alone is reported and the forces kept, as the reference
   // prints and goes on
@@ -161,8 +167,8 @@ void free_tables(DevTables &t) {
 void free_state(examd_handle *h) {
   DevState &d = h->d;
   void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
-                  d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.rank,
-                  d.count, d.start, d.sorted, d.chunk, d.home, d.part, h->log,
+                  d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.slot,
+                  d.count, d.sorted, d.bflag, d.home, d.part, h->log,
                   h->atom_seg, d.type, d.pair32, d.pair64, d.pair_shift,
                   d.tab32, d.tab64};
   for (void *p : ptrs)
@@ -174,8 +180,8 @@ void free_state(examd_handle *h) {
   h->part_cap = 0;
   free_tables(h->tab[0]);
   free_tables(h->tab[1]);
-  if (h->host_start) (void)hipHostFree(h->host_start);
-  h->host_start = nullptr;
+  if (h->pinned_counts) (void)hipHostFree(h->pinned_counts);
+  h->pinned_counts = nullptr;
   if (h->dev_stage) (void)hipFree(h->dev_stage);
   if (h->chunk_start) (void)hipFree(h->chunk_start);
   h->chunk_start = nullptr;
@@ -531,8 +537,45 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
   return choose_caps_counts(h, prec, cnt.data(), nullptr, true);
 }
 
+// The block counts of the last binning pass on the host (4 B per block,
+// through a pinned buffer).
+int device_block_counts(examd_handle *h, std::vector<int> &cnt) {
+  if (!h->pinned_counts &&
+      hipHostMalloc(reinterpret_cast<void **>(&h->pinned_counts),
+                    std::max<int64_t>(h->nblocks, 1) * sizeof(int)) != hipSuccess) {
+    h->pinned_counts = nullptr;
+    return EXAMD_ERR_NOMEM;
+  }
+  HIPCHK(hipMemcpyAsync(h->pinned_counts, h->d.count, h->nblocks * sizeof(int),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  cnt.assign(h->pinned_counts, h->pinned_counts + h->nblocks);
+  if (cnt.empty()) cnt.push_back(0);
+  return EXAMD_OK;
+}
+
+// Slots per block for blocks of up to `need` atoms: a power of two of at
+// least 8 and need + 2 (a block that fills up later is given twice the
+// slots during the run).  The list array only grows; a new one starts
+// empty, so the next evaluation bins afresh.
+int ensure_bin_slots(examd_handle *h, int64_t need) {
+  int shift = kMinBinShift;
+  while (((int64_t)1 << shift) < need + 2) ++shift;
+  if (shift <= h->d.bin_shift) return EXAMD_OK;
+  if ((h->nblocks << shift) >= ((int64_t)1 << 31)) return EXAMD_ERR_BIN_OVERFLOW;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  int *p = nullptr;
+  if (dalloc(&p, h->nblocks << shift)) return EXAMD_ERR_NOMEM;
+  (void)hipFree(h->d.sorted);
+  h->d.sorted = p;
+  h->d.bin_shift = shift;
+  drop_graph(h);
+  h->forces_valid = -1;
+  return EXAMD_OK;
+}
+
 // Re-partition the map of one precision from the block counts of the last
-// binning pass (start[] holds their exclusive scan): new balanced
+// binning pass: new balanced
 // boundaries (balanced maps) and a new LDS capacity.  The old map stays when
 // the new one would not fit, and when the new one has the same boundaries
 // and the old capacity still holds the fullest region with 5% to spare, so
@@ -540,18 +583,9 @@ int choose_caps(examd_handle *h, int prec, const double *pos) {
 // map was replaced.
 int repartition(examd_handle *h, int prec, bool *changed) {
   *changed = false;
-  if (!h->host_start &&
-      hipHostMalloc(reinterpret_cast<void **>(&h->host_start),
-                    (h->nblocks + 1) * sizeof(int)) != hipSuccess) {
-    h->host_start = nullptr;
-    return EXAMD_ERR_NOMEM;
-  }
-  const int *start = h->host_start;
-  HIPCHK(hipMemcpyAsync(h->host_start, h->d.start, (h->nblocks + 1) * sizeof(int),
-                        hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
-  for (int64_t b = 0; b < h->nblocks; ++b) cnt[b] = start[b + 1] - start[b];
+  std::vector<int> cnt;
+  const int stc = device_block_counts(h, cnt);
+  if (stc != EXAMD_OK) return stc;
   const TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
   int64_t mra = 0;
@@ -576,18 +610,9 @@ int repartition(examd_handle *h, int prec, bool *changed) {
 // attempt, or a finer map when the map was chosen automatically.  Leaves the
 // map unchanged and returns EXAMD_ERR_TILE_OVERFLOW when no capacity fits.
 int regrow(examd_handle *h, int prec, int attempt) {
-  if (!h->host_start &&
-      hipHostMalloc(reinterpret_cast<void **>(&h->host_start),
-                    (h->nblocks + 1) * sizeof(int)) != hipSuccess) {
-    h->host_start = nullptr;
-    return EXAMD_ERR_NOMEM;
-  }
-  HIPCHK(hipMemcpyAsync(h->host_start, h->d.start, (h->nblocks + 1) * sizeof(int),
-                        hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
-  for (int64_t b = 0; b < h->nblocks; ++b)
-    cnt[b] = h->host_start[b + 1] - h->host_start[b];
+  std::vector<int> cnt;
+  const int stc = device_block_counts(h, cnt);
+  if (stc != EXAMD_OK) return stc;
   const TileConfig saved = h->tc[prec];
   TileConfig &c = h->tc[prec];
   if (choose_caps_counts(h, prec, cnt.data(), nullptr, false) != EXAMD_OK) {
@@ -695,7 +720,7 @@ void evaluate(examd_handle *h, int prec, bool energy, bool homecount,
               StepLog *log) {
   launch_step_bin(h->d, table(h, prec), prec == EXAMD_PREC_MIXED, kBinOnly,
                   0.0, 0.0, nullptr, log, h->stream);
-  launch_sort(h->d, h->nblocks, prec == EXAMD_PREC_FP64, h->stream);
+  launch_bin_update(h->d, true, prec == EXAMD_PREC_FP64, log, h->stream);
   forces_timed(h, prec, energy, homecount, log);
 }
 
@@ -737,7 +762,7 @@ void enqueue_steps(examd_handle *h, int prec, int64_t count, double dt,
       launch_step_bin(h->d, t, mixed, kKickDriftBin, hk, dt, nullptr, Lc,
                       h->stream);
     }
-    launch_sort(h->d, h->nblocks, !mixed, h->stream);
+    launch_bin_update(h->d, false, !mixed, Lc, h->stream);
     // with timing on, every timing_every-th step's force launch is
     // bracketed by events (mid-stride, away from the tally steps)
     const int k = h->timing_every;
@@ -929,7 +954,8 @@ int create_handle(const examd_params *prm, const examd_system *systems,
     S.g.bl = 0.5 * prm->cutoff;
     atoms += S.n;
     blocks += nb;
-    if (atoms > (int64_t)2000000000 || blocks > (int64_t)1 << 30)
+    // every block's slots are addressed with 32-bit ints
+    if (atoms > (int64_t)2000000000 || blocks > ((int64_t)1 << 31) / kMinBinSlots - 1)
       return EXAMD_ERR_ARG;
   }
   const bool fixed = prm->tiles[0] > 0 || prm->tiles[1] > 0 || prm->tiles[2] > 0;
@@ -986,9 +1012,11 @@ int create_handle(const examd_params *prm, const examd_system *systems,
   rc |= dalloc(&d.vx, n) | dalloc(&d.vy, n) | dalloc(&d.vz, n);
   rc |= dalloc(&d.fx, n) | dalloc(&d.fy, n) | dalloc(&d.fz, n);
   rc |= dalloc(&d.qx, n) | dalloc(&d.qy, n) | dalloc(&d.qz, n);
-  rc |= dalloc(&d.cell, n) | dalloc(&d.rank, n) | dalloc(&d.sorted, n);
-  rc |= dalloc(&d.count, blocks) | dalloc(&d.start, blocks + 1);
-  rc |= dalloc(&d.chunk, scan_chunk_ints(blocks));
+  d.nblocks = blocks;
+  d.bin_shift = kMinBinShift;
+  rc |= dalloc(&d.cell, n) | dalloc(&d.slot, n);
+  rc |= dalloc(&d.sorted, blocks << kMinBinShift) | dalloc(&d.count, blocks);
+  rc |= dalloc(&d.bflag, bin_flag_bytes(blocks));
   rc |= dalloc(&h->atom_seg, nsys + 1);
   if (rc == EXAMD_OK) rc = ensure_log(h, 1);
   if (rc != EXAMD_OK) {
@@ -1003,6 +1031,8 @@ int create_handle(const examd_params *prm, const examd_system *systems,
   hipError_t e = hipMemcpy(h->atom_seg, seg.data(), seg.size() * sizeof(long long),
                            hipMemcpyHostToDevice);
   launch_fill_i32(d.count, 0, blocks, h->stream);
+  launch_fill_i32(reinterpret_cast<int *>(d.bflag), 0, bin_flag_bytes(blocks) / 4,
+                  h->stream);
   if (e != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
     free_state(h);
     (void)hipStreamDestroy(h->stream);
@@ -1016,14 +1046,20 @@ int create_handle(const examd_params *prm, const examd_system *systems,
 // ------------------------------------------------ allocation-free form ----
 // examd_hip_forces_static keeps everything beyond its inputs and outputs in
 // one static device array per GPU: the atoms' SoA positions, force sums,
-// block ids, ranks and sorted ids, the block counts and their scan, the
-// per-cell-block partial tallies, the step log and the system descriptor.
+// block ids and list slots, the block lists (16 slots per block at the
+// block limit, more for fewer blocks), counts and flags, the per-cell-block
+// partial tallies, the step log and the system descriptor.
 constexpr int64_t kStaticAtoms = EXAMD_STATIC_MAX_ATOMS;
 constexpr int64_t kStaticBlocks = EXAMD_STATIC_MAX_BLOCKS;
 constexpr int64_t kStaticTiles = EXAMD_STATIC_MAX_CELL_BLOCKS;
+// the block lists hold 16 slots per block at kStaticBlocks blocks; a call
+// with fewer blocks gives each block the largest power of two that fits
+// (at most 256)
+constexpr int kStaticListShift = 4;
+constexpr int64_t kStaticListInts = kStaticBlocks << kStaticListShift;
 constexpr size_t scratch_align(size_t v) { return (v + 255) / 256 * 256; }
 struct StaticLayout {
-  size_t xyz, f, ints, count, start, chunk, part, log, sys, total;
+  size_t xyz, f, ints, sorted, count, bflag, part, log, sys, total;
 };
 constexpr StaticLayout static_layout() {
   StaticLayout L{};
@@ -1032,14 +1068,14 @@ constexpr StaticLayout static_layout() {
   o = scratch_align(o + 3 * (size_t)kStaticAtoms * sizeof(double));
   L.f = o;   // FP64 sums or fixed-point sums, one precision per call
   o = scratch_align(o + 3 * (size_t)kStaticAtoms * sizeof(double));
-  L.ints = o;   // cell, rank, sorted
-  o = scratch_align(o + 3 * (size_t)kStaticAtoms * sizeof(int));
+  L.ints = o;   // cell, slot
+  o = scratch_align(o + 2 * (size_t)kStaticAtoms * sizeof(int));
+  L.sorted = o;
+  o = scratch_align(o + (size_t)kStaticListInts * sizeof(int));
   L.count = o;
   o = scratch_align(o + (size_t)kStaticBlocks * sizeof(int));
-  L.start = o;
-  o = scratch_align(o + ((size_t)kStaticBlocks + 1) * sizeof(int));
-  L.chunk = o;
-  o = scratch_align(o + 4096 * sizeof(int));
+  L.bflag = o;
+  o = scratch_align(o + (size_t)bin_flag_bytes(kStaticBlocks));
   L.part = o;
   o = scratch_align(o + (size_t)std::max<int64_t>(kStaticTiles, kStaticAtoms / 256 + 1) *
                             sizeof(Partial));
@@ -1110,6 +1146,8 @@ const char *examd_hip_strerror(int s) {
     case EXAMD_ERR_TABLE_RANGE:
       return "pair closer than the tabulated pair function's range";
     case EXAMD_ERR_OUT_OF_BOX: return "block number out of range";
+    case EXAMD_ERR_BIN_OVERFLOW:
+      return "a block holds more atoms than its slots";
     case EXAMD_ERR_STATIC_LIMIT:
       return "system beyond the static scratch limits of "
              "examd_hip_forces_static";
@@ -1194,6 +1232,9 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
     // leaves both maps as they were (the device tables and the state on the
     // device still agree with them).
     const std::vector<int> cnt = host_counts(h, pos);
+    const int maxocc = *std::max_element(cnt.begin(), cnt.end());
+    const int stb = ensure_bin_slots(h, maxocc);
+    if (stb != EXAMD_OK) return stb;
     const TileConfig saved[2] = {h->tc[0], h->tc[1]};
     int st1 = EXAMD_OK;
     std::thread other([&] {
@@ -1433,21 +1474,28 @@ int examd_hip_forces(examd_handle *h, int precision, double *forces,
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   HIPCHK(hipSetDevice(h->dev));
   const int ns = h->nsys;
-  launch_fill_log(h->log, ns, h->stream);
-  evaluate(h, precision, true, false, h->log);
-  kinetic(h, precision, h->log);
-  HIPCHK(hipGetLastError());
   std::vector<StepLog> L((size_t)ns);
-  HIPCHK(hipMemcpyAsync(L.data(), h->log, ns * sizeof(StepLog),
-                        hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  if (h->timing) collect_timing(h);
-  h->forces_valid = precision;
   unsigned long long fl = 0;
-  for (int k = 0; k < ns; ++k) {
-    fl |= L[k].flags;
-    if (stats) fill_stats(h, L[k], precision, 0, &stats[k]);
+  // a block that runs out of slots is given twice the slots and the
+  // evaluation repeated
+  for (int attempt = 0;; ++attempt) {
+    launch_fill_log(h->log, ns, h->stream);
+    evaluate(h, precision, true, false, h->log);
+    kinetic(h, precision, h->log);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(L.data(), h->log, ns * sizeof(StepLog),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->timing) collect_timing(h);
+    fl = 0;
+    for (int k = 0; k < ns; ++k) fl |= L[k].flags;
+    if (!(fl & kFlagBinOverflow) || attempt >= 4) break;
+    const int sb = ensure_bin_slots(h, ((int64_t)1 << h->d.bin_shift) + 1);
+    if (sb != EXAMD_OK) return sb;
   }
+  h->forces_valid = precision;
+  if (stats)
+    for (int k = 0; k < ns; ++k) fill_stats(h, L[k], precision, 0, &stats[k]);
   int st = flags_to_status(fl);
   if (st != EXAMD_OK && st != EXAMD_ERR_ZERO_DISTANCE) {
     h->forces_valid = -1;
@@ -1538,13 +1586,15 @@ int examd_hip_forces_static(const examd_params *prm, int precision,
   d.qx = reinterpret_cast<unsigned long long *>(f);
   d.qy = d.qx + n;
   d.qz = d.qx + 2 * n;
+  d.nblocks = nblocks;
+  d.bin_shift = kStaticListShift;
+  while (d.bin_shift < 8 && (nblocks << (d.bin_shift + 1)) <= kStaticListInts) ++d.bin_shift;
   int *ints = reinterpret_cast<int *>(base + kStatic.ints);
   d.cell = ints;
-  d.rank = ints + n;
-  d.sorted = ints + 2 * n;
+  d.slot = ints + n;
+  d.sorted = reinterpret_cast<int *>(base + kStatic.sorted);
   d.count = reinterpret_cast<int *>(base + kStatic.count);
-  d.start = reinterpret_cast<int *>(base + kStatic.start);
-  d.chunk = reinterpret_cast<int *>(base + kStatic.chunk);
+  d.bflag = base + kStatic.bflag;
   d.part = reinterpret_cast<Partial *>(base + kStatic.part);
   StepLog *log = reinterpret_cast<StepLog *>(base + kStatic.log);
   SysDesc *sysd = reinterpret_cast<SysDesc *>(base + kStatic.sys);
@@ -1564,10 +1614,10 @@ int examd_hip_forces_static(const examd_params *prm, int precision,
   // array keeps the last call's contents)
   launch_store_sys(sd, sysd, s);
   launch_aos_to_soa(pos, d.x, d.y, d.z, n, s);
-  launch_fill_i32(d.count, 0, nblocks, s);
   launch_fill_log(log, 1, s);
+  // a kBinOnly pass clears the counts and flags itself
   launch_step_bin(d, t, mixed, kBinOnly, 0.0, 0.0, nullptr, log, s);
-  launch_sort(d, nblocks, !mixed, s);
+  launch_bin_update(d, true, !mixed, log, s);
   launch_forces(d, t, p, cfg, mixed, true, false, log, s);
   launch_forces_to_fp64(d, forces, mixed, s);
   HIPCHK(hipGetLastError());
@@ -1648,7 +1698,8 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
     }
     return EXAMD_OK;
   };
-  const unsigned long long fatal = kFlagOverflow | kFlagOutOfBox | kFlagNonFinite;
+  const unsigned long long fatal =
+      kFlagOverflow | kFlagBinOverflow | kFlagOutOfBox | kFlagNonFinite;
   // periodic re-partition of the map, between chunks; a new map
   // re-evaluates the forces, so the run continues exactly as one that
   // stopped here, applied the map and started again
@@ -1692,18 +1743,27 @@ int examd_hip_run(examd_handle *h, int precision, int64_t nsteps, double dt,
   // after a harvest: 1 = the chunk is to be run again, 0 = go on, -1 = a
   // HIP error
   auto regrown = [&](int &attempt, int64_t k0, unsigned long long fl0) -> int {
-    if (!(fl & kFlagOverflow) || (fl0 & kFlagOverflow) || attempt >= kMaxRegrow)
-      return 0;
+    constexpr unsigned long long kGrow = kFlagOverflow | kFlagBinOverflow;
+    if (!(fl & kGrow) || (fl0 & kGrow) || attempt >= kMaxRegrow) return 0;
+    const bool lds = (fl & kFlagOverflow) != 0, bins = (fl & kFlagBinOverflow) != 0;
     for (int a = 0; a < 6; ++a)
       if (hipMemcpyAsync(saved_arrays[a], h->chunk_start + a * n, n * sizeof(double),
                          hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
         return -1;
     k = k0;
     fl = fl0;
-    if (regrow(h, precision, ++attempt) != EXAMD_OK) {
-      fl |= kFlagOverflow;   // the state is the chunk's start
+    ++attempt;
+    // a block that ran out of slots: twice the slots (the state is the
+    // chunk's start either way)
+    if (bins && ensure_bin_slots(h, ((int64_t)1 << h->d.bin_shift) + 1) != EXAMD_OK) {
+      fl |= kFlagBinOverflow;
       return 0;
     }
+    if (lds && regrow(h, precision, attempt) != EXAMD_OK) {
+      fl |= kFlagOverflow;
+      return 0;
+    }
+    if (!lds) h->regrows++;
     launch_fill_log(h->log, ns, h->stream);
     initial_forces();
     std::vector<StepLog> L0((size_t)ns);
@@ -1785,10 +1845,10 @@ int examd_hip_block_ids(examd_handle *h, int64_t *blocknum, int32_t *blockxyz,
   if (!h->state_set) return EXAMD_ERR_NOT_READY;
   HIPCHK(hipSetDevice(h->dev));
   launch_fill_log(h->log, 1, h->stream);
+  // a pass that starts the block lists afresh and leaves them empty: the
+  // next evaluation bins again (forces_valid below)
   launch_step_bin(h->d, table(h, 0), false, kBinOnly, 0.0, 0.0, nullptr,
                   h->log, h->stream);
-  // clear the counts the binning pass left behind
-  launch_fill_i32(h->d.count, 0, h->nblocks, h->stream);
   HIPCHK(hipGetLastError());
   std::vector<int> cell((size_t)std::max<int64_t>(n, 1));
   StepLog L;
@@ -1958,7 +2018,7 @@ int examd_hip_autotune(examd_handle *h, int precision, int trials,
     if (e != hipSuccess) return EXAMD_ERR_HIP;
     unsigned long long fl = 0;
     for (int k = 0; k < ns; ++k) fl |= L[k].flags;
-    *ms = (fl & (kFlagOverflow | kFlagOutOfBox | kFlagNonFinite))
+    *ms = (fl & (kFlagOverflow | kFlagBinOverflow | kFlagOutOfBox | kFlagNonFinite))
               ? 1e300
               : h->force_ms / std::max<int64_t>(h->force_launches, 1);
     return EXAMD_OK;

@@ -86,6 +86,7 @@ enum Flag : unsigned {
   kFlagFixedRange = 8u,   // pair force beyond the fixed-point range
   kFlagNonFinite = 16u,   // NaN/Inf position or force
   kFlagTableRange = 32u,  // pair closer than a tabulated function's range
+  kFlagBinOverflow = 64u, // a block holds more atoms than its slots
 };
 
 // Per-step tallies written by device kernels.

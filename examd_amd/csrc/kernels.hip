@@ -2,9 +2,11 @@
 //
 //   k_step_bin   velocity-Verlet kick/drift/wrap fused with cell assignment
 //                (exaMD_identify_block, SRC/exaMD_module.f90:101-125)
-//   k_scan_*     exclusive scan of per-block counts
-//   k_scatter    counting-sort scatter, then (FP64 path) k_block_order
-//                restores input order inside each block
+//   k_bin_compact, k_bin_insert
+//                the per-block atom lists kept from step to step: holes
+//                left by atoms that changed block closed up, those atoms
+//                appended to their new block; (FP64 path) k_block_order
+//                restores input order inside each changed block
 //                (SRC/exaMD_input_formats.f90:80-88)
 //   k_forces     one workgroup per cell block: gathers the home blocks plus
 //                the rc/2 ghost shell into LDS (the halo exchange of
@@ -210,8 +212,11 @@ __device__ __forceinline__ int step_bin_atom(const DevState &d,
   return cell;
 }
 
-// kBinPer atoms per thread (coalesced, kBinThreads apart): their counting
-// atomics are issued together, so their round trips overlap.
+// kBinPer atoms per thread (coalesced, kBinThreads apart).  kBinOnly
+// starts the block lists afresh: every atom is inserted by k_bin_insert.
+// The other modes keep them: an atom that changes block leaves a hole in
+// its old block's list (closed up by k_bin_compact) and is inserted into
+// the new one; the others keep their slots.
 template <bool MIXED, int MODE>
 __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     DevState d, SysTable t, double h, double dt, int want_ke,
@@ -220,23 +225,92 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
       (int64_t)blockIdx.x * (kBinThreads * kBinPer) + threadIdx.x;
   double ke = 0.0;
   unsigned flags = 0;
-  int cell[kBinPer];
 #pragma unroll
   for (int k = 0; k < kBinPer; ++k) {
     const int64_t i = base + (int64_t)k * kBinThreads;
-    cell[k] = i < d.n ? step_bin_atom<MIXED, MODE>(d, t, h, dt, i, ke, flags)
-                      : -1;
+    if (i < d.n) {
+      const int old = MODE == kBinOnly ? -1 : d.cell[i];
+      const int c = step_bin_atom<MIXED, MODE>(d, t, h, dt, i, ke, flags);
+      if (MODE == kBinOnly) {
+        d.slot[i] = -1;
+      } else if (c != old) {
+        const int sl = d.slot[i];
+        if (sl >= 0) d.sorted[sl] = -1;
+        d.bflag[old] = 1;
+        d.slot[i] = -1;
+      }
+    }
   }
-  int rank[kBinPer];
-#pragma unroll
-  for (int k = 0; k < kBinPer; ++k)
-    if (cell[k] >= 0) rank[k] = atomicAdd(&d.count[cell[k]], 1);
-#pragma unroll
-  for (int k = 0; k < kBinPer; ++k)
-    if (cell[k] >= 0) d.rank[base + (int64_t)k * kBinThreads] = rank[k];
   if constexpr (MODE == kFinishKickDriftBin) {
     if (want_ke) write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull, 0ull},
                                             d.part, blockIdx.x);
+  }
+  flags = wave_or(flags);
+  if ((threadIdx.x & 63) == 0 && flags)
+    atomicOr(&flag_log->flags, (unsigned long long)flags);
+}
+
+// Lists of the blocks that lost atoms in this step's binning pass, closed
+// up in order (the holes removed, the moved atoms' slots updated).  The
+// flags are cleared; k_bin_insert sets them again for the blocks it
+// appends to.  Four blocks per thread: their flags are read as one word.
+__global__ void k_bin_compact(DevState d) {
+  const int64_t w4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w4 * 4 >= d.nblocks) return;
+  unsigned *fw = reinterpret_cast<unsigned *>(d.bflag) + w4;
+  const unsigned f = *fw;
+  if (!f) return;
+  *fw = 0u;
+  for (int q = 0; q < 4; ++q) {
+    if (!((f >> (8 * q)) & 0xffu)) continue;
+    const int64_t b = w4 * 4 + q;
+    const int base = (int)(b << d.bin_shift);
+    const int c = min(d.count[b], 1 << d.bin_shift);
+    int *p = d.sorted + base;
+    int w = 0;
+    for (int r = 0; r < c; ++r) {
+      const int v = p[r];
+      if (v < 0) continue;
+      if (w != r) {
+        p[w] = v;
+        d.slot[v] = base + w;
+      }
+      ++w;
+    }
+    d.count[b] = w;
+  }
+}
+
+// Atoms without a slot (those that changed block, or all of them after a
+// kBinOnly pass) appended to their block's list, kBinPer per thread with
+// their counting atomics issued together.  A block with more atoms than
+// slots keeps the first ones and raises kFlagBinOverflow.
+__global__ __launch_bounds__(kBinThreads) void k_bin_insert(DevState d,
+                                                            StepLog *flag_log) {
+  const int64_t base =
+      (int64_t)blockIdx.x * (kBinThreads * kBinPer) + threadIdx.x;
+  int c[kBinPer], r[kBinPer];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    const int64_t i = base + (int64_t)k * kBinThreads;
+    c[k] = (i < d.n && d.slot[i] < 0) ? d.cell[i] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k)
+    if (c[k] >= 0) r[k] = atomicAdd(&d.count[c[k]], 1);
+  unsigned flags = 0;
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    if (c[k] < 0) continue;
+    if (r[k] < (1 << d.bin_shift)) {
+      const int64_t i = base + (int64_t)k * kBinThreads;
+      const int sl = (c[k] << d.bin_shift) + r[k];
+      d.sorted[sl] = (int)i;
+      d.slot[i] = sl;
+      d.bflag[c[k]] = 1;
+    } else {
+      flags |= kFlagBinOverflow;
+    }
   }
   flags = wave_or(flags);
   if ((threadIdx.x & 63) == 0 && flags)
@@ -291,11 +365,6 @@ __global__ __launch_bounds__(kBinThreads) void k_sys_ke(DevState d,
 }
 
 // ------------------------------------------------------------- scan ----
-constexpr int kScanThreads = 256;
-constexpr int kScanPer = 8;
-constexpr int kScanChunk = kScanThreads * kScanPer;  // 2048 counts
-constexpr int kScanDirect = 2048;  // chunks whose totals k_scan_down sums itself
-
 // Inclusive scan of one int per thread across the workgroup; returns the
 // inclusive prefix and writes the workgroup total to *total.
 template <int NT>
@@ -324,154 +393,33 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int *s_wave,
   return r;
 }
 
-// kScanPer consecutive counts of one thread: two 16-byte loads when the
-// segment lies inside the array (count is 256-byte aligned, segments start
-// at multiples of 8), else one at a time.
-__device__ __forceinline__ void load_counts(const int *count, int64_t base,
-                                            int64_t n, int (&c)[kScanPer]) {
-  if (base + kScanPer <= n) {
-    const int4 lo = *reinterpret_cast<const int4 *>(count + base);
-    const int4 hi = *reinterpret_cast<const int4 *>(count + base + 4);
-    c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w;
-    c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
-  } else {
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k) c[k] = base + k < n ? count[base + k] : 0;
-  }
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(
-    const int *count, int64_t n, int *chunk) {
-  __shared__ int s_wave[kScanThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
-  int c[kScanPer];
-  load_counts(count, base, n, c);
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) s += c[k];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int w = 0; w < kScanThreads / 64; ++w) t += s_wave[w];
-    chunk[blockIdx.x] = t;
-  }
-}
-
-// Exclusive scan of the block counts: k_scan_reduce writes each chunk's
-// total; k_scan_down takes the offset of its chunk from chunk_off when
-// given (the scanned chunk totals of a large array), else adds the totals
-// of the chunks before its own (at most kScanDirect cached reads per
-// workgroup, summed in the same pass as the scan inside the chunk).
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(
-    int *count, int64_t n, const int *chunk, const int *chunk_off, int *start,
-    int clear) {
-  __shared__ int s_wave[kScanThreads / 64];
-  __shared__ int s_pre[kScanThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
-  const bool full = base + kScanPer <= n;
-  int c[kScanPer];
-  load_counts(count, base, n, c);
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) s += c[k];
-  if (clear) {
-    if (full) {
-      *reinterpret_cast<int4 *>(count + base) = make_int4(0, 0, 0, 0);
-      *reinterpret_cast<int4 *>(count + base + 4) = make_int4(0, 0, 0, 0);
-    } else {
-#pragma unroll
-      for (int k = 0; k < kScanPer; ++k)
-        if (base + k < n) count[base + k] = 0;
-    }
-  }
-  // this thread's share of the totals of the chunks before this one
-  int pre = 0;
-  if (chunk_off) {
-    if (threadIdx.x == 0) pre = chunk_off[blockIdx.x];
-  } else {
-    for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) pre += chunk[b];
-  }
-  // wave: inclusive scan of s, sum of pre
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int inc = s;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int t = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += t;
-  }
-  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
-  if (lane == 63) s_wave[wave] = inc;
-  if (lane == 0) s_pre[wave] = pre;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int w = 0; w < kScanThreads / 64; ++w) acc += s_pre[w];
-    for (int w = 0; w < kScanThreads / 64; ++w) {
-      const int t = s_wave[w];
-      s_wave[w] = acc;
-      acc += t;
-    }
-  }
-  __syncthreads();
-  int run = s_wave[wave] + inc - s;
-  int out[kScanPer];
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    out[k] = run;
-    run += c[k];
-  }
-  if (full) {
-    *reinterpret_cast<int4 *>(start + base) = make_int4(out[0], out[1], out[2], out[3]);
-    *reinterpret_cast<int4 *>(start + base + 4) = make_int4(out[4], out[5], out[6], out[7]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k)
-      if (base + k < n) start[base + k] = out[k];
-  }
-  if (base + kScanPer >= n && base < n) start[n] = run;
-}
-
-// kBinPer atoms per thread, kBinThreads apart: the gathers of start[]
-// are issued together.
-__global__ __launch_bounds__(kBinThreads) void k_scatter(DevState d) {
-  const int64_t base =
-      (int64_t)blockIdx.x * (kBinThreads * kBinPer) + threadIdx.x;
-  int c[kBinPer], r[kBinPer];
-#pragma unroll
-  for (int k = 0; k < kBinPer; ++k) {
-    const int64_t i = base + (int64_t)k * kBinThreads;
-    c[k] = i < d.n ? d.cell[i] : -1;
-    r[k] = i < d.n ? d.rank[i] : 0;
-  }
-#pragma unroll
-  for (int k = 0; k < kBinPer; ++k)
-    if (c[k] >= 0) r[k] += d.start[c[k]];
-#pragma unroll
-  for (int k = 0; k < kBinPer; ++k)
-    if (c[k] >= 0) d.sorted[r[k]] = (int)(base + (int64_t)k * kBinThreads);
-}
-
 // Input order inside each block (the order the reference's linked list
 // gives, SRC/exaMD_input_formats.f90:80-88): insertion sort of the few
-// atom ids of one block.  Only the FP64 path needs it (its sums depend on
-// the order of the pairs); the mixed path's fixed-point sums do not.  Also
-// clears the block's count for the next binning pass.
-__global__ void k_block_order(DevState d, int64_t nblocks) {
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nblocks) return;
-  const int c = d.count[b];
-  d.count[b] = 0;
-  if (c < 2) return;
-  int *p = d.sorted + d.start[b];
-  for (int k = 1; k < c; ++k) {
-    int v = p[k];
-    int m = k - 1;
-    while (m >= 0 && p[m] > v) {
-      p[m + 1] = p[m];
-      --m;
+// atom ids of each block that received atoms, their slots updated.  Only
+// the FP64 path needs it (its sums depend on the order of the pairs); the
+// mixed path's fixed-point sums do not.
+__global__ void k_block_order(DevState d) {
+  const int64_t w4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w4 * 4 >= d.nblocks) return;
+  const unsigned f = reinterpret_cast<const unsigned *>(d.bflag)[w4];
+  if (!f) return;
+  for (int q = 0; q < 4; ++q) {
+    if (!((f >> (8 * q)) & 0xffu)) continue;
+    const int64_t b = w4 * 4 + q;
+    const int c = min(d.count[b], 1 << d.bin_shift);
+    if (c < 2) continue;
+    const int base = (int)(b << d.bin_shift);
+    int *p = d.sorted + base;
+    for (int k = 1; k < c; ++k) {
+      int v = p[k];
+      int m = k - 1;
+      while (m >= 0 && p[m] > v) {
+        p[m + 1] = p[m];
+        --m;
+      }
+      p[m + 1] = v;
     }
-    p[m + 1] = v;
+    for (int k = 0; k < c; ++k) d.slot[p[k]] = base + k;
   }
 }
 
@@ -764,10 +712,9 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     gy -= gy >= nby ? nby : 0;
     gz += gz < 0 ? nbz : 0;
     gz -= gz >= nbz ? nbz : 0;
-    int gb = block0 + gx + nbx * (gy + nby * gz);
-    int st = A.d.start[gb];
-    s_off[rb] = A.d.start[gb + 1] - st;
-    s_gst[rb] = st;
+    const int gb = block0 + gx + nbx * (gy + nby * gz);
+    s_off[rb] = min(A.d.count[gb], 1 << A.d.bin_shift);
+    s_gst[rb] = gb << A.d.bin_shift;
   }
   __syncthreads();
   {
@@ -801,8 +748,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   // 2. ghost gather: atoms of the region into LDS.  The mixed image is
   //    relative to the tile centre and pre-shifted by the periodic image;
   //    the FP64 image keeps raw coordinates (the minimum image is applied
-  //    per pair, as the reference does).  First every slot learns its
-  //    region block, then one thread per slot gathers.
+  //    per pair, as the reference does).  First one thread per region
+  //    block copies the block's atom ids from its list (consecutive slots)
+  //    with the block's packed region coordinates and image codes, then
+  //    one thread per slot gathers the positions.
   const bool wlx = h0x == 0, whx = h1x == nbx;
   const bool wly = h0y == 0, why = h1y == nby;
   const bool wlz = h0z == 0, whz = h1z == nbz;
@@ -817,21 +766,38 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
                hzr = (0.5 * tz + 1.0) * g.bl;
   const float rc2pre =
       A.rc2f + (float)((hxr * hxr + hyr * hyr + hzr * hzr) * 0x1p-17);
-  for (int rb = tid; rb < NRB; rb += NT)
-    for (int s = s_off[rb]; s < s_off[rb + 1]; ++s) s_id[s] = rb;
-  __syncthreads();
-  for (int s = tid; s < R; s += NT) {
-    const int rb = s_id[s];
+  for (int rb = tid; rb < NRB; rb += NT) {
     const int cx = rb % RX, cy = (rb / RX) % RY, cz = rb / RXY;
-    const int id = A.d.sorted[s_gst[rb] + (s - s_off[rb])];
     const int sx = (wlx && cx == 0) ? -1 : ((whx && cx == RX - 1) ? 1 : 0);
     const int sy = (wly && cy == 0) ? -1 : ((why && cy == RY - 1) ? 1 : 0);
     const int sz = (wlz && cz == 0) ? -1 : ((whz && cz == RZ - 1) ? 1 : 0);
     const int bc = cx | (cy << 8) | (cz << 16) | ((sx + 1) << 24) |
                    ((sy + 1) << 26) | ((sz + 1) << 28);
+    const int o = s_off[rb], c = s_off[rb + 1] - o;
+    // a block's slots start 32-byte aligned (at least 8 per block): the
+    // first eight ids come in two 16-byte loads issued together
+    const int *src = A.d.sorted + s_gst[rb];
+    int4 q0 = make_int4(0, 0, 0, 0), q1 = make_int4(0, 0, 0, 0);
+    if (c > 0) q0 = *reinterpret_cast<const int4 *>(src);
+    if (c > 4) q1 = *reinterpret_cast<const int4 *>(src + 4);
+    const int ids[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < c) {
+        s_id[o + k] = ids[k];
+        s_bc[o + k] = bc;
+      }
+    for (int k = 8; k < c; ++k) {
+      s_id[o + k] = src[k];
+      s_bc[o + k] = bc;
+    }
+  }
+  __syncthreads();
+  for (int s = tid; s < R; s += NT) {
+    const int id = s_id[s];
+    const int bc = s_bc[s];
+    const int sx = bc_s(bc, 0), sy = bc_s(bc, 1), sz = bc_s(bc, 2);
     const double X = A.d.x[id], Y = A.d.y[id], Z = A.d.z[id];
-    s_id[s] = id;
-    s_bc[s] = bc;
     if constexpr (TYPED) s_ty[s] = A.d.type[id];
     if constexpr (FP64) {
       s_x[s] = X;
@@ -1463,6 +1429,11 @@ void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
                      StepLog *flag_log, hipStream_t s) {
   unsigned nb = nblk(d.n, kBinThreads * kBinPer);
   if (nb == 0) return;
+  if (mode == kBinOnly) {
+    launch_fill_i32(d.count, 0, d.nblocks, s);
+    launch_fill_i32(reinterpret_cast<int *>(d.bflag), 0,
+                    bin_flag_bytes(d.nblocks) / 4, s);
+  }
   const int want = (mode == kFinishKickDriftBin && ke_log) ? 1 : 0;
 #define EXAMD_BIN(M, MD)                                               \
   k_step_bin<M, MD><<<nb, kBinThreads, 0, s>>>(d, t, h, dt, want, flag_log)
@@ -1498,36 +1469,13 @@ void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
   if (t.nsys > 0) k_sys_ke<<<t.nsys, kBinThreads, 0, s>>>(d, t, log);
 }
 
-int64_t scan_chunk_ints(int64_t nblocks) {
-  const int64_t nchunk = (nblocks + kScanChunk - 1) / kScanChunk;
-  const int64_t n2 = (nchunk + kScanChunk - 1) / kScanChunk;
-  return 2 * ((nchunk + 1 + 7) / 8 * 8) + n2 + 8;
-}
-
-void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
-                 hipStream_t s) {
-  const int nchunk = (int)((nblocks + kScanChunk - 1) / kScanChunk);
-  k_scan_reduce<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk);
-  // beyond kScanDirect chunks the chunk totals are scanned first (chunk
-  // totals, then their offsets, then the second-level totals, each part
-  // 32-byte aligned for the vector loads)
-  const int *chunk_off = nullptr;
-  if (nchunk > kScanDirect) {
-    const int stride = (nchunk + 1 + 7) / 8 * 8;
-    int *off = d.chunk + stride, *tot2 = d.chunk + 2 * stride;
-    const int n2 = (nchunk + kScanChunk - 1) / kScanChunk;
-    k_scan_reduce<<<n2, kScanThreads, 0, s>>>(d.chunk, nchunk, tot2);
-    k_scan_down<<<n2, kScanThreads, 0, s>>>(d.chunk, nchunk, tot2, nullptr,
-                                            off, 0);
-    chunk_off = off;
-  }
-  k_scan_down<<<nchunk, kScanThreads, 0, s>>>(d.count, nblocks, d.chunk,
-                                              chunk_off, d.start,
-                                              block_order ? 0 : 1);
-  if (d.n > 0)
-    k_scatter<<<nblk(d.n, kBinThreads * kBinPer), kBinThreads, 0, s>>>(d);
-  if (block_order)
-    k_block_order<<<nblk(nblocks, 256), 256, 0, s>>>(d, nblocks);
+void launch_bin_update(const DevState &d, bool after_bin_only, bool block_order,
+                       StepLog *flag_log, hipStream_t s) {
+  if (d.n <= 0) return;
+  const unsigned gb = nblk(bin_flag_bytes(d.nblocks) / 4, 256);
+  if (!after_bin_only) k_bin_compact<<<gb, 256, 0, s>>>(d);
+  k_bin_insert<<<nblk(d.n, kBinThreads * kBinPer), kBinThreads, 0, s>>>(d, flag_log);
+  if (block_order) k_block_order<<<gb, 256, 0, s>>>(d);
 }
 
 void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,

@@ -6,19 +6,23 @@
 namespace examd {
 
 // Device state of one system.  Per-atom arrays are SoA in input order; the
-// block-sorted order exists only as the index array `sorted`.
+// block-sorted order exists only as the index array `sorted`, which gives
+// every block 2^bin_shift slots: block b's atoms are the first count[b]
+// entries from b << bin_shift.  The lists persist from step to step and
+// only the atoms that change block are moved.
 struct DevState {
   int64_t n = 0;
+  int64_t nblocks = 0;
   double *x = nullptr, *y = nullptr, *z = nullptr;
   double *vx = nullptr, *vy = nullptr, *vz = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr;          // FP64 force
   unsigned long long *qx = nullptr, *qy = nullptr, *qz = nullptr;  // 2^-32
   int *cell = nullptr;     // 0-based block id per atom
-  int *rank = nullptr;     // provisional slot inside the block
+  int *slot = nullptr;     // the atom's entry in `sorted` (-1: to insert)
   int *count = nullptr;    // atoms per block (nblocks)
-  int *start = nullptr;    // exclusive scan of count (nblocks + 1)
-  int *sorted = nullptr;   // atom ids grouped by block, input order inside
-  int *chunk = nullptr;    // scan partials
+  int *sorted = nullptr;   // atom ids by block (nblocks << bin_shift)
+  unsigned char *bflag = nullptr;  // blocks whose list changed this step
+  int bin_shift = 3;       // log2 of the slots per block
   StepLog *log = nullptr;  // per-step tallies
   unsigned long long *home = nullptr;  // home-block pair counts (diagnostic)
   Partial *part = nullptr; // per-workgroup partial tallies
@@ -86,6 +90,8 @@ void launch_fill_log(StepLog *p, int n, hipStream_t s);
 void launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
 
 // ke_log (single system only): kinetic energy of the finished step.
+// kBinOnly starts the block lists afresh (counts and flags cleared, every
+// atom to be inserted); the other modes mark the atoms that change block.
 void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
                      int mode, double h, double dt, StepLog *ke_log,
                      StepLog *flag_log, hipStream_t s);
@@ -95,12 +101,16 @@ void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
 // Kinetic energy of every system into log[sys].ke (sum of v^2).
 void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
                    hipStream_t s);
-// Ints of DevState::chunk the block-count scan needs for nblocks blocks.
-int64_t scan_chunk_ints(int64_t nblocks);
-// Counting sort of the atoms by block (scan of the counts, which it clears,
-// and scatter); block_order puts each block's atoms in input order.
-void launch_sort(const DevState &d, int64_t nblocks, bool block_order,
-                 hipStream_t s);
+// Bytes of DevState::bflag for nblocks blocks (whole ints, so that it can
+// be cleared by launch_fill_i32).
+constexpr int64_t bin_flag_bytes(int64_t nblocks) { return (nblocks + 3) / 4 * 4; }
+// The block lists after a binning pass: blocks that lost atoms closed up
+// in order (not after a kBinOnly pass, which cleared them), the atoms
+// without a slot appended to their block (kFlagBinOverflow into
+// flag_log when a block runs out of slots), and with block_order each
+// changed block put in input order.
+void launch_bin_update(const DevState &d, bool after_bin_only, bool block_order,
+                       StepLog *flag_log, hipStream_t s);
 // Tallies of system k go to log[k].
 void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                    const ForceConfig &c, bool mixed, bool energy,

@@ -16,7 +16,7 @@ Module exaMD_hip_binding
        EXAMD_ERR_NONFINITE = 7, EXAMD_ERR_NOT_READY = 8,                  &
        EXAMD_ERR_FIXED_RANGE = 9, EXAMD_ERR_ZERO_DISTANCE = 10,           &
        EXAMD_ERR_TABLE_RANGE = 11, EXAMD_ERR_STATIC_LIMIT = 12,          &
-       EXAMD_ERR_OUT_OF_BOX = 109
+       EXAMD_ERR_BIN_OVERFLOW = 13, EXAMD_ERR_OUT_OF_BOX = 109
 
   Integer(c_int), Parameter :: EXAMD_PREC_FP64 = 0, EXAMD_PREC_MIXED = 1
 

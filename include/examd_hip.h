@@ -47,6 +47,10 @@ enum {
     EXAMD_ERR_STATIC_LIMIT = 12,/* examd_hip_forces_static: atoms, blocks
                                    or cell blocks beyond its static
                                    scratch limits */
+    EXAMD_ERR_BIN_OVERFLOW = 13,/* a block held more atoms than its slots
+                                   (the handle's entry points enlarge the
+                                   slots and evaluate again; the
+                                   allocation-free form returns this) */
     EXAMD_ERR_OUT_OF_BOX = 109  /* block number out of range (ref. 109) */
 };
 
@@ -194,7 +198,7 @@ int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
 #define EXAMD_STATIC_MAX_ATOMS 65536
 #define EXAMD_STATIC_MAX_BLOCKS 131072
 #define EXAMD_STATIC_MAX_CELL_BLOCKS 4096
-#define EXAMD_STATIC_SCRATCH_BYTES 5194496
+#define EXAMD_STATIC_SCRATCH_BYTES 12911104
 
 /* No counterpart in the reference.  One force evaluation that allocates
  * nothing, for callers that cannot allocate device memory (code that must
@@ -212,8 +216,10 @@ int examd_hip_get_state(examd_handle *h, double *pos, double *vel,
  * and returns when the forces are written.  Returns EXAMD_ERR_STATIC_LIMIT
  * beyond EXAMD_STATIC_MAX_ATOMS atoms, EXAMD_STATIC_MAX_BLOCKS blocks or
  * EXAMD_STATIC_MAX_CELL_BLOCKS cell blocks, EXAMD_ERR_TILE_OVERFLOW when a
- * cell block's region does not fit the LDS (use more tiles), and the
- * status flags of examd_hip_forces otherwise.  Every call on one GPU uses
+ * cell block's region does not fit the LDS (use more tiles),
+ * EXAMD_ERR_BIN_OVERFLOW when a block holds more atoms than its list
+ * slots (16 per block at EXAMD_STATIC_MAX_BLOCKS blocks, up to 256 with
+ * fewer blocks), and the status flags of examd_hip_forces otherwise.  Every call on one GPU uses
  * that GPU's single array: calls that overlap on one GPU (several host
  * threads) race and must be serialised by the caller. */
 int examd_hip_forces_static(const examd_params *params, int precision,
