@@ -1,6 +1,8 @@
 """A run whose cell-block regions outgrow their LDS capacity mid-run: the
 chunk that overflowed is run again from its start with a larger capacity,
 and the run's result is the one a run started with that capacity gives.
+Likewise a block whose atom list outgrows its slots: the chunk is run again
+with twice the slots.
 
 The system is made to compress (velocities pointing to the box centre), so
 the central cell blocks' regions grow past a 1% capacity slack.  These are
@@ -58,3 +60,56 @@ def test_overflow_mid_run_is_recovered_fp64(E):
     d -= np.round(d / box) * box
     assert np.abs(d).max() < 1e-8
     assert [s[3] for s in sa] == [s[3] for s in sb]
+
+
+def _converging(steps=100, dt=0.05):
+    """27 atoms that fly (epsilon = 0: no forces) radially inwards over 5
+    sigma onto a 3 x 3 x 3 grid, 0.7 sigma apart, inside the enlarged last
+    block of a 13.7-sigma box (rc 2.5: 10 blocks per side, the last one
+    2.45 sigma wide), so that the block's list outgrows its slots."""
+    L = 13.7
+    box = np.array([L, L, L])
+    g = 11.5 + 0.7 * np.arange(3)
+    tgt = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    u = tgt - tgt.mean(0)
+    nrm = np.linalg.norm(u, axis=1)
+    u[nrm == 0] = (1.0, 0.0, 0.0)
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    vel = -u * 5.0 / (steps * dt)
+    pos = np.mod(tgt - vel * steps * dt, L)
+    # no two atoms ever come closer than 0.1 sigma (minimum image)
+    for s in range(steps + 1):
+        p = np.mod(pos + vel * s * dt, L)
+        r = p[:, None, :] - p[None, :, :]
+        r -= np.round(r / L) * L
+        r2 = (r ** 2).sum(-1) + np.eye(len(p)) * 1e9
+        assert r2.min() > 0.01
+    return pos, vel, box
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_block_slots_grow_mid_run(E, oracle, prec):
+    """A block that receives more atoms than its list slots during a run:
+    the chunk is run again from its start with twice the slots (as often as
+    needed), and every tally of the run counts the pairs the oracle counts
+    at the same positions, so no atom was left out of the binning."""
+    O = oracle
+    steps, dt, every = 100, 0.05, 25
+    pos, vel, box = _converging()
+    print(f"[bin slots {prec}] n={len(pos)}", flush=True)
+    with E.MidpointLJ(len(pos), box, 2.5, epsilon=0.0, graph_chunk=every) as m:
+        m.set_state(pos, vel)
+        st = m.run(steps, dt, prec, energy_every=every)
+        grown = m.regrow_count()
+        p, _ = m.get_state()
+    assert grown >= 1
+    expect = np.mod(pos + vel * steps * dt, box)
+    d = p - expect
+    d -= np.round(d / box) * box
+    assert np.abs(d).max() < 1e-12
+    for s in st:
+        ps = np.mod(pos + vel * s.step * dt, box)
+        _, t, _ = O.forces(ps, box, 2.5)
+        assert s.pairs_cand == t.cand, s.step
+        # FP32 r^2 at the cutoff may move a pair across it on the mixed path
+        assert abs(s.pairs_cut - t.cut) <= (2 if prec == "mixed" else 0), s.step

@@ -1,9 +1,8 @@
-"""The counting sort's scan of the block counts on a box of more than
-2048 x 2048 blocks, where the chunk totals are scanned in a second level
-before the down-sweep: a thin liquid slab (ρ 0.8, 4σ thick) in a 206σ box,
-164^3 = 4,410,944 blocks (2154 chunks of 2048).  FP64 forces within 1e-10 of the oracle, pair
-counts and per-home-block ownership exact; the mixed path within its
-FP32 bars."""
+"""A grid of 164^3 = 4,410,944 blocks, most of them empty: a thin liquid
+slab (ρ 0.8, 4σ thick) in a 206σ box, so that the block lists (8 slots per
+block) span 141 MB and block ids reach 2^22.  FP64 forces within 1e-10 of
+the oracle, pair counts and per-home-block ownership exact; the mixed path
+within its FP32 bars."""
 import numpy as np
 import pytest
 
