@@ -47,6 +47,13 @@ DRIFT_STEPS = {"mixed": 10000, "fp64": 1000}
 DRIFT_EVERY = 10
 
 
+def progress(what, t_start):
+    """One line on stderr as each leg of the bench ends (the JSON line on
+    stdout stays the only output a caller parses)."""
+    print(f"bench: {what} done at {time.perf_counter() - t_start:.1f} s",
+          file=sys.stderr, flush=True)
+
+
 def algorithmic_flops(cand, cut, energy):
     """W = 8 P_cand + 18 P_cut (+3 P_cut when energies are tallied)."""
     return 8.0 * cand + 18.0 * cut + (3.0 * cut if energy else 0.0)
@@ -574,14 +581,17 @@ def main():
     dm, df = (int(x) for x in args.drift_steps.split(","))
     DRIFT_STEPS["mixed"], DRIFT_STEPS["fp64"] = dm, df
     host_threads = max(1, min(16, os.cpu_count() or 1))
+    t_start = time.perf_counter()
     (wall_mx, st_mx, force_ms_mx, launches, tune_mx, ferr, drift_mx,
      chunk_mx) = timed_run(E, pos, vel, box, "mixed", steps, args.warmup,
                            energy_every, parse_shape(args.mixed_shape),
                            args.graph_chunk, report_error=True,
                            oracle_threads=host_threads)
+    progress("mixed timed run, accuracy leg and force errors", t_start)
     wall_64, st_64, force_ms_64, _, tune_64, _, drift_64, chunk_64 = timed_run(
         E, pos, vel, box, "fp64", fsteps, args.warmup, energy_every,
         parse_shape(args.fp64_shape), args.graph_chunk)
+    progress("fp64 timed run and accuracy leg", t_start)
 
     rate_mx = n * steps / wall_mx
     rate_64 = n * fsteps / wall_64
@@ -599,19 +609,24 @@ def main():
     batch = None
     if args.batch_steps > 0:
         batch = batch_bench(E, args.batch_steps, args.warmup, args.loop_steps)
+        progress("batch", t_start)
     slab = None
     if args.slab_steps > 0:
         slab = slab_bench(E, args.slab_steps, args.warmup)
+        progress("slab", t_start)
     species = None
     if args.species_steps > 0:
         species = species_bench(E, args.species_steps, args.warmup)
+        progress("species", t_start)
     table = None
     if args.table_steps > 0:
         table = table_bench(E, args.table_steps, args.warmup)
+        progress("table", t_start)
 
     c1 = None
     if not args.no_cpu and args.config1_steps > 0:
         c1 = config1(E, args.config1_steps)
+        progress("32,000-atom case against the oracle", t_start)
 
     cpu = None
     if not args.no_cpu and args.cpu_mt_steps > 0:
@@ -623,6 +638,7 @@ def main():
                          f"oracle C restatement of the reference path on "
                          f"{host_threads} threads (z slabs in two phases)",
                "host": host_cpu()}
+        progress("cpu baseline on all cores", t_start)
         if args.cpu_steps > 0:
             v1 = cpu_baseline(pos, vel, box, args.cpu_steps, 1)
             cpu["one_core"] = {

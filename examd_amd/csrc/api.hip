@@ -474,9 +474,39 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
     }
     if (ok) {
       c.cap = (int)cap;
-      // a batch's maps from a tile edge: every system's map coarsened as
-      // far as the launch's largest region (atoms and blocks) allows
+      // a batch's maps from a tile edge: every system's map coarsened while
+      // its largest region fits the LDS budget of one workgroup (the whole
+      // LDS for 1024 threads, one workgroup per CU; a proportional share
+      // for smaller workgroups, several per CU; never less than the largest
+      // region now needs).  The limits in atoms and in region blocks come
+      // from that budget, not from the largest region itself: a small
+      // system whose one cell block spans it, periodic images included,
+      // has a region of twice its atoms and far more blocks than its
+      // neighbours' tiles, and would otherwise hold every map back.
       if (c.edge > 0 && nsys > 1) {
+        const size_t now = force_lds_bytes(fp64, (int)cap, c.maxrb, c.threads,
+                                           c.lane_queue, typed);
+        const size_t budget =
+            std::max(now, h->lds_limit * (size_t)c.threads / 1024);
+        auto cap_of = [&](int64_t a) {
+          return ((int64_t)std::ceil((double)a * (1.0 + slack)) + 32 + 15) / 16 * 16;
+        };
+        auto fits = [&](int64_t a, int64_t b) {
+          const int64_t ca = cap_of(a);
+          return ca <= 65535 &&
+                 force_lds_bytes(fp64, (int)ca, (int)b, c.threads, c.lane_queue,
+                                 typed) <= budget;
+        };
+        int64_t atom_lim = mra, blk_lim = c.maxrb;
+        if (fits(atom_lim, blk_lim)) {
+          // at least one atom per region block for the denser systems, then
+          // whatever the LDS leaves for the blocks of the dilute ones
+          while (fits(atom_lim + 16, std::max(blk_lim, atom_lim + 16))) {
+            atom_lim += 16;
+            blk_lim = std::max(blk_lim, atom_lim);
+          }
+          while (fits(atom_lim, blk_lim + 16)) blk_lim += 16;
+        }
         host_parallel_for(nsys, [&](int64_t k0, int64_t k1) {
           for (int64_t k = k0; k < k1; ++k) {
             const SysHost &S = h->sys[k];
@@ -490,11 +520,26 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
                   for (int a = 0; a < 3; ++a) b[a] = even_cuts(t.nb[a], t.T[a]);
                   return b;
                 },
-                sc, mra, c.maxrb);
+                sc, atom_lim, (int)blk_lim);
             for (int a = 0; a < 3; ++a) c.T[k][a] = g.T[a];
             if (c.balance) c.cuts[k] = balanced_cuts(g, sc, S.n);
           }
         }, 1);
+        // the launch's largest region after coarsening sets the capacity
+        c.maxrb = 0;
+        mra = 0;
+        for (int k = 0; k < nsys; ++k) {
+          const Grid g = grid_of(h, prec, k);
+          const Cuts cb = cuts_of(h, prec, k, g);
+          c.maxrb = std::max(c.maxrb, max_region_blocks(cb));
+          mra = std::max(mra, max_region_atoms(g, cb, cnt + h->sys[k].block0));
+        }
+        int64_t cap2 = std::max<int64_t>(cap, cap_of(mra));
+        while (cap2 > std::max<int64_t>(cap, needed_cap(mra)) &&
+               force_lds_bytes(fp64, (int)cap2, c.maxrb, c.threads, c.lane_queue,
+                               typed) > h->lds_limit)
+          cap2 -= 16;
+        c.cap = (int)cap2;
       }
       // about three work items per lane: run groups from the mean region
       double reg_atoms = 0.0, reg_tiles = 0.0;
