@@ -1185,44 +1185,50 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         const int j1 = s_off[row0 + hi + 1];
         cand += (unsigned)(j1 > j0 ? j1 - j0 : 0);
         if constexpr (!FP64) {
-          // mixed: up to sixteen candidates per step; hits are evaluated
-          // (at once or from the queue) from the LDS positions
-          auto scan = [&](auto width) {
-            constexpr int CW = decltype(width)::value;
-            for (int j = j0; j < j1; j += CW) {
-              // prefilter: rc'^2 - d^2 = (rc'^2 - |a|^2) - |p|^2 + 2 a.p by
-              // one subtraction and three FMAs; its sign bit (set beyond
-              // the cutoff) is shifted into the miss mask, last candidate
-              // first, so that bit k ends up holding candidate k
-              unsigned miss = 0u;
+          // mixed: up to sixteen candidates per step, in groups of four;
+          // hits are evaluated (at once or from the queue) from the LDS
+          // positions.  The step width (4, 8, 12 or 16) follows the longest
+          // range among the wave's active lanes, so short segments do not
+          // pay for sixteen slots.  The width is wave-uniform and the groups
+          // beyond it are skipped by uniform branches inside one loop, so
+          // the state carried through the loop (force sums, queue) keeps
+          // one set of registers whatever the width.
+          const int nj = j1 - j0;
+          const int ng = __builtin_amdgcn_readfirstlane(
+              !__ballot(nj > 4) ? 1 : (!__ballot(nj > 8) ? 2 : (!__ballot(nj > 12) ? 3 : 4)));
+          const int CW = 4 * ng;
+          for (int j = j0; j < j1; j += CW) {
+            // prefilter: rc'^2 - d^2 = (rc'^2 - |a|^2) - |p|^2 + 2 a.p by
+            // one subtraction and three FMAs; its sign bit (set beyond the
+            // cutoff) is shifted into the miss mask, last candidate first,
+            // so that bit k ends up holding candidate k
+            unsigned miss = 0u;
+            auto group = [&](int k0) {
 #pragma unroll
-              for (int k = CW - 1; k >= 0; --k) {
+              for (int k = k0 + 3; k >= k0; --k) {
                 const float4 p = s_p4[j + k];
                 const float t = __builtin_fmaf(
                     tzf, p.z, __builtin_fmaf(tyf, p.y, __builtin_fmaf(txf, p.x, kaf - p.w)));
                 miss = __builtin_amdgcn_alignbit(miss, __float_as_uint(t), 31);
               }
-              // candidates past the range end are masked off
-              const int lim = j1 - j;
-              unsigned hits = ~miss & ((1u << min(lim, CW)) - 1u);
-              if (LQ) {
-                if (hits) push(j, hits);
-              } else {
-                while (hits) {
-                  const int k = __builtin_ctz(hits);
-                  hits &= hits - 1u;
-                  eval_queued(j + k);
-                }
+            };
+            if (ng > 3) group(12);
+            if (ng > 2) group(8);
+            if (ng > 1) group(4);
+            group(0);
+            // candidates past the range end are masked off
+            const int lim = j1 - j;
+            unsigned hits = ~miss & ((1u << min(lim, CW)) - 1u);
+            if (LQ) {
+              if (hits) push(j, hits);
+            } else {
+              while (hits) {
+                const int k = __builtin_ctz(hits);
+                hits &= hits - 1u;
+                eval_queued(j + k);
               }
             }
-          };
-          // the step width follows the longest range among the wave's
-          // active lanes, so short segments do not pay for sixteen slots
-          const int nj = j1 - j0;
-          if (!__ballot(nj > 4)) scan(std::integral_constant<int, 4>{});
-          else if (!__ballot(nj > 8)) scan(std::integral_constant<int, 8>{});
-          else if (!__ballot(nj > 12)) scan(std::integral_constant<int, 12>{});
-          else scan(std::integral_constant<int, kChunk>{});
+          }
         } else {
           for (int j = j0; j < j1; j += kChunk) {
             unsigned hits;
