@@ -25,7 +25,8 @@ namespace examd {
 namespace {
 
 constexpr int kBinThreads = 256;
-constexpr int kBinPer = 4;   // atoms per thread of the binning kernel
+constexpr int kBinPer = 2;   // atoms per thread of the binning kernel
+constexpr int kInsertPer = 4;  // atoms per thread of the list insertion
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -140,28 +141,53 @@ __device__ __forceinline__ int system_of(const SysTable &t, int64_t i) {
   return lo;
 }
 
-// Integrate atom i (mode permitting), zero its force accumulators and
-// return its global block id (flags for a position the block map rejects).
+// One atom of the binning pass: what it reads before any store (the
+// force as loaded: FP64, or the fixed-point sums converted at the store,
+// so that no load waits for another's conversion).
+struct BinAtom {
+  double x, y, z, vx, vy, vz;
+  union { double f; long long q; } fx, fy, fz;
+  int old, slot;
+};
+
 template <bool MIXED, int MODE>
-__device__ __forceinline__ int step_bin_atom(const DevState &d,
-                                             const SysTable &t, double h,
-                                             double dt, int64_t i, double &ke,
-                                             unsigned &flags) {
-  const SysDesc &sd = t.sys[t.nsys > 1 ? system_of(t, i) : 0];
-  const Grid g = sd.g;
-  double X = d.x[i], Y = d.y[i], Z = d.z[i];
+__device__ __forceinline__ void bin_load(const DevState &d, int64_t i,
+                                         BinAtom &a) {
+  a.x = d.x[i];
+  a.y = d.y[i];
+  a.z = d.z[i];
   if constexpr (MODE != kBinOnly) {
-    double Fx, Fy, Fz;
     if constexpr (MIXED) {
-      Fx = (double)(long long)d.qx[i] * kFixInv;
-      Fy = (double)(long long)d.qy[i] * kFixInv;
-      Fz = (double)(long long)d.qz[i] * kFixInv;
+      a.fx.q = (long long)d.qx[i];
+      a.fy.q = (long long)d.qy[i];
+      a.fz.q = (long long)d.qz[i];
     } else {
-      Fx = d.fx[i];
-      Fy = d.fy[i];
-      Fz = d.fz[i];
+      a.fx.f = d.fx[i];
+      a.fy.f = d.fy[i];
+      a.fz.f = d.fz[i];
     }
-    double Vx = d.vx[i], Vy = d.vy[i], Vz = d.vz[i];
+    a.vx = d.vx[i];
+    a.vy = d.vy[i];
+    a.vz = d.vz[i];
+    a.old = d.cell[i];
+    a.slot = d.slot[i];
+  }
+}
+
+// Integrate one loaded atom (mode permitting), store it, zero its force
+// accumulators and return its global block id, assigned as
+// exaMD_identify_block does (flags for a position the block map rejects).
+template <bool MIXED, int MODE>
+__device__ __forceinline__ int bin_store(const DevState &d, const Grid &g,
+                                         int block0, double h, double dt,
+                                         int64_t i, BinAtom &a, double &ke,
+                                         unsigned &flags) {
+  double X = a.x, Y = a.y, Z = a.z;
+  if constexpr (MODE != kBinOnly) {
+    const double Fx = MIXED ? (double)a.fx.q * kFixInv : a.fx.f;
+    const double Fy = MIXED ? (double)a.fy.q * kFixInv : a.fy.f;
+    const double Fz = MIXED ? (double)a.fz.q * kFixInv : a.fz.f;
+    double Vx = a.vx, Vy = a.vy, Vz = a.vz;
     if constexpr (MODE == kFinishKickDriftBin) {
       Vx = Vx + h * Fx;
       Vy = Vy + h * Fy;
@@ -206,44 +232,83 @@ __device__ __forceinline__ int step_bin_atom(const DevState &d,
                     (long long)(bz - 1) * ((long long)g.nb[0] * g.nb[1]);
     long long nblk = (long long)g.nb[0] * g.nb[1] * g.nb[2];
     if (num < 1 || num > nblk) flags |= kFlagOutOfBox;
-    else cell = sd.block0 + (int)(num - 1);
+    else cell = block0 + (int)(num - 1);
   }
   d.cell[i] = cell;
   return cell;
 }
 
-// kBinPer atoms per thread (coalesced, kBinThreads apart).  kBinOnly
-// starts the block lists afresh: every atom is inserted by k_bin_insert.
-// The other modes keep them: an atom that changes block leaves a hole in
-// its old block's list (closed up by k_bin_compact) and is inserted into
-// the new one; the others keep their slots.
-template <bool MIXED, int MODE>
+// kBinPer atoms per thread (coalesced, kBinThreads apart).  Every atom's
+// loads are issued before the first store, so they are in flight together
+// (the stores could otherwise alias the next atom's loads as far as the
+// compiler knows, and each atom would wait for the last).  One system
+// (SINGLE) reads its grid once per thread.  kBinOnly starts the block
+// lists afresh: every atom is inserted by k_bin_insert.  The other modes
+// keep them: an atom that changes block leaves a hole in its old block's
+// list (closed up by k_bin_compact) and is inserted into the new one; the
+// others keep their slots.
+template <bool MIXED, int MODE, bool SINGLE>
 __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     DevState d, SysTable t, double h, double dt, int want_ke,
     StepLog *flag_log) {
   const int64_t base =
       (int64_t)blockIdx.x * (kBinThreads * kBinPer) + threadIdx.x;
-  double ke = 0.0;
+  // kinetic energy per slab of kBinThreads atoms, each slab's partial
+  // written on its own: the partials and their sums are those of k_kick
+  // (one atom per thread), so fused and separate closing kicks agree bit
+  // for bit
+  double ke[kBinPer];
   unsigned flags = 0;
+  BinAtom at[kBinPer];
+#pragma unroll
+  for (int k = 0; k < kBinPer; ++k) {
+    // past the last atom the thread reads the last one again (no branch
+    // among the loads) and stores nothing
+    const int64_t i = base + (int64_t)k * kBinThreads;
+    bin_load<MIXED, MODE>(d, i < d.n ? i : d.n - 1, at[k]);
+  }
+  // nothing of the store phase is scheduled among the loads
+  __builtin_amdgcn_sched_barrier(0);
+  Grid g1;
+  int b1 = 0;
+  if constexpr (SINGLE) {
+    g1 = t.sys[0].g;
+    b1 = t.sys[0].block0;
+  }
 #pragma unroll
   for (int k = 0; k < kBinPer; ++k) {
     const int64_t i = base + (int64_t)k * kBinThreads;
+    ke[k] = 0.0;
     if (i < d.n) {
-      const int old = MODE == kBinOnly ? -1 : d.cell[i];
-      const int c = step_bin_atom<MIXED, MODE>(d, t, h, dt, i, ke, flags);
+      int c;
+      if constexpr (SINGLE) {
+        c = bin_store<MIXED, MODE>(d, g1, b1, h, dt, i, at[k], ke[k], flags);
+      } else {
+        const SysDesc &sd = t.sys[system_of(t, i)];
+        c = bin_store<MIXED, MODE>(d, sd.g, sd.block0, h, dt, i, at[k], ke[k], flags);
+      }
       if (MODE == kBinOnly) {
         d.slot[i] = -1;
-      } else if (c != old) {
-        const int sl = d.slot[i];
-        if (sl >= 0) d.sorted[sl] = -1;
-        d.bflag[old] = 1;
+      } else if (c != at[k].old) {
+        if (at[k].slot >= 0) d.sorted[at[k].slot] = -1;
+        d.bflag[at[k].old] = 1;
         d.slot[i] = -1;
       }
     }
   }
   if constexpr (MODE == kFinishKickDriftBin) {
-    if (want_ke) write_partial<kBinThreads>(Partial{ke, 0.0, 0ull, 0ull, 0ull, 0ull},
-                                            d.part, blockIdx.x);
+    if (want_ke) {
+#pragma unroll
+      for (int k = 0; k < kBinPer; ++k) {
+        // slabs past the last atom have no partial slot
+        const int64_t slab = (int64_t)blockIdx.x * kBinPer + k;
+        if (slab * kBinThreads >= d.n) break;
+        // the previous slab's sum has been read out of write_partial's LDS
+        if (k > 0) __syncthreads();
+        write_partial<kBinThreads>(Partial{ke[k], 0.0, 0ull, 0ull, 0ull, 0ull},
+                                   d.part, (int)slab);
+      }
+    }
   }
   flags = wave_or(flags);
   if ((threadIdx.x & 63) == 0 && flags)
@@ -282,25 +347,25 @@ __global__ void k_bin_compact(DevState d) {
 }
 
 // Atoms without a slot (those that changed block, or all of them after a
-// kBinOnly pass) appended to their block's list, kBinPer per thread with
+// kBinOnly pass) appended to their block's list, kInsertPer per thread with
 // their counting atomics issued together.  A block with more atoms than
 // slots keeps the first ones and raises kFlagBinOverflow.
 __global__ __launch_bounds__(kBinThreads) void k_bin_insert(DevState d,
                                                             StepLog *flag_log) {
   const int64_t base =
-      (int64_t)blockIdx.x * (kBinThreads * kBinPer) + threadIdx.x;
-  int c[kBinPer], r[kBinPer];
+      (int64_t)blockIdx.x * (kBinThreads * kInsertPer) + threadIdx.x;
+  int c[kInsertPer], r[kInsertPer];
 #pragma unroll
-  for (int k = 0; k < kBinPer; ++k) {
+  for (int k = 0; k < kInsertPer; ++k) {
     const int64_t i = base + (int64_t)k * kBinThreads;
     c[k] = (i < d.n && d.slot[i] < 0) ? d.cell[i] : -1;
   }
 #pragma unroll
-  for (int k = 0; k < kBinPer; ++k)
+  for (int k = 0; k < kInsertPer; ++k)
     if (c[k] >= 0) r[k] = atomicAdd(&d.count[c[k]], 1);
   unsigned flags = 0;
 #pragma unroll
-  for (int k = 0; k < kBinPer; ++k) {
+  for (int k = 0; k < kInsertPer; ++k) {
     if (c[k] < 0) continue;
     if (r[k] < (1 << d.bin_shift)) {
       const int64_t i = base + (int64_t)k * kBinThreads;
@@ -1441,8 +1506,12 @@ void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
                     bin_flag_bytes(d.nblocks) / 4, s);
   }
   const int want = (mode == kFinishKickDriftBin && ke_log) ? 1 : 0;
-#define EXAMD_BIN(M, MD)                                               \
-  k_step_bin<M, MD><<<nb, kBinThreads, 0, s>>>(d, t, h, dt, want, flag_log)
+#define EXAMD_BIN(M, MD)                                                      \
+  (t.nsys == 1                                                                \
+       ? k_step_bin<M, MD, true><<<nb, kBinThreads, 0, s>>>(d, t, h, dt, want, \
+                                                            flag_log)          \
+       : k_step_bin<M, MD, false><<<nb, kBinThreads, 0, s>>>(d, t, h, dt, want, \
+                                                             flag_log))
   if (mixed) {
     if (mode == kBinOnly) EXAMD_BIN(true, kBinOnly);
     else if (mode == kKickDriftBin) EXAMD_BIN(true, kKickDriftBin);
@@ -1454,8 +1523,8 @@ void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
   }
 #undef EXAMD_BIN
   if (want)
-    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(d.part, nullptr, (int)nb,
-                                                    ke_log, kSumKE);
+    k_reduce_partials<<<1, kReduceThreads, 0, s>>>(
+        d.part, nullptr, (int)nblk(d.n, kBinThreads), ke_log, kSumKE);
 }
 
 void launch_kick(const DevState &d, bool mixed, double h, StepLog *ke_log,
@@ -1480,7 +1549,7 @@ void launch_bin_update(const DevState &d, bool after_bin_only, bool block_order,
   if (d.n <= 0) return;
   const unsigned gb = nblk(bin_flag_bytes(d.nblocks) / 4, 256);
   if (!after_bin_only) k_bin_compact<<<gb, 256, 0, s>>>(d);
-  k_bin_insert<<<nblk(d.n, kBinThreads * kBinPer), kBinThreads, 0, s>>>(d, flag_log);
+  k_bin_insert<<<nblk(d.n, kBinThreads * kInsertPer), kBinThreads, 0, s>>>(d, flag_log);
   if (block_order) k_block_order<<<gb, 256, 0, s>>>(d);
 }
 
