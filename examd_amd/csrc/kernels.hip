@@ -284,8 +284,20 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
       if constexpr (SINGLE) {
         c = bin_store<MIXED, MODE>(d, g1, b1, h, dt, i, at[k], ke[k], flags);
       } else {
-        const SysDesc &sd = t.sys[system_of(t, i)];
-        c = bin_store<MIXED, MODE>(d, sd.g, sd.block0, h, dt, i, at[k], ke[k], flags);
+        // the system of the wave's first atom of this slab by binary
+        // search, then each lane steps forward (systems are contiguous and
+        // hold hundreds of atoms or more: rarely more than one step)
+        int sys = system_of(t, (int64_t)__builtin_amdgcn_readfirstlane((int)i));
+        while (sys + 1 < t.nsys && t.atom_seg[sys + 1] <= i) ++sys;
+        // a wave inside one system reads its grid as one (scalar) copy
+        const int s0 = __builtin_amdgcn_readfirstlane(sys);
+        if (!__ballot(sys != s0)) {
+          const SysDesc &sd = t.sys[s0];
+          c = bin_store<MIXED, MODE>(d, sd.g, sd.block0, h, dt, i, at[k], ke[k], flags);
+        } else {
+          const SysDesc &sd = t.sys[sys];
+          c = bin_store<MIXED, MODE>(d, sd.g, sd.block0, h, dt, i, at[k], ke[k], flags);
+        }
       }
       if (MODE == kBinOnly) {
         d.slot[i] = -1;
@@ -1340,10 +1352,14 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
         }
       }
       // after every second run (and the group's last): drain part of the
-      // queues once the wave holds two pending pairs per lane on average
+      // queues once the wave holds two pending pairs per lane on average.
+      // The mixed path drains three quarters of the mean pending count
+      // (lanes holding fewer pairs than that idle in the last steps of a
+      // longer drain); the FP64 path, whose four-candidate entries fill
+      // the rings sooner, the whole mean.
       if (LQ && (((r - r0) & 1) || r + 1 == r1)) {
         const int P = wave_total(pend);
-        if (P >= 64 * kDrainMin) drain_some(P >> 6);
+        if (P >= 64 * kDrainMin) drain_some(FP64 ? (P >> 6) : ((3 * P) >> 8));
       }
     }
     if (LQ) drain();
