@@ -402,8 +402,8 @@ class _Handle:
         captured-chunk boundaries).  Takes the dict get_config() and
         autotune() return: force_ms is ignored, and a map that came from a
         tile edge is derived from the edge again (in a batch, each system's
-        map is then coarsened while its largest region stays within the
-        batch's largest)."""
+        map is then coarsened while its largest region fits the LDS budget
+        of one workgroup, in atoms and in region blocks)."""
         if int(tile_edge) > 0:
             tiles = (0, 0, 0)
         t = Tune()

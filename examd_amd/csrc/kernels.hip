@@ -2,12 +2,11 @@
 //
 //   k_step_bin   velocity-Verlet kick/drift/wrap fused with cell assignment
 //                (exaMD_identify_block, SRC/exaMD_module.f90:101-125)
-//   k_bin_compact, k_bin_insert
-//                the per-block atom lists kept from step to step: holes
-//                left by atoms that changed block closed up, those atoms
-//                appended to their new block; (FP64 path) k_block_order
-//                restores input order inside each changed block
-//                (SRC/exaMD_input_formats.f90:80-88)
+//   k_bin_insert the per-block atom lists kept from step to step: atoms
+//                that changed block take the first free slot of their new
+//                block's list (the slot they left is free); (FP64 path)
+//                k_block_order closes up each changed list and restores
+//                input order inside it (SRC/exaMD_input_formats.f90:80-88)
 //   k_forces     one workgroup per cell block: gathers the home blocks plus
 //                the rc/2 ghost shell into LDS (the halo exchange of
 //                SRC/exaMD_comms.f90:44-376 done on device), evaluates every
@@ -244,9 +243,9 @@ __device__ __forceinline__ int bin_store(const DevState &d, const Grid &g,
 // compiler knows, and each atom would wait for the last).  One system
 // (SINGLE) reads its grid once per thread.  kBinOnly starts the block
 // lists afresh: every atom is inserted by k_bin_insert.  The other modes
-// keep them: an atom that changes block leaves a hole in its old block's
-// list (closed up by k_bin_compact) and is inserted into the new one; the
-// others keep their slots.
+// keep them: an atom that changes block frees its slot in its old block's
+// list (the block's count drops by one) and is inserted into the new one;
+// the others keep their slots.
 template <bool MIXED, int MODE, bool SINGLE>
 __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     DevState d, SysTable t, double h, double dt, int want_ke,
@@ -302,8 +301,13 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
       if (MODE == kBinOnly) {
         d.slot[i] = -1;
       } else if (c != at[k].old) {
-        if (at[k].slot >= 0) d.sorted[at[k].slot] = -1;
-        d.bflag[at[k].old] = 1;
+        // the atom leaves a hole in its old list (refilled by later
+        // insertions) and the old block's count drops by one; the other
+        // atoms keep their slots and their order
+        if (at[k].slot >= 0) {
+          d.sorted[at[k].slot] = -1;
+          atomicSub(&d.count[at[k].old], 1);
+        }
         d.slot[i] = -1;
       }
     }
@@ -327,64 +331,38 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     atomicOr(&flag_log->flags, (unsigned long long)flags);
 }
 
-// Lists of the blocks that lost atoms in this step's binning pass, closed
-// up in order (the holes removed, the moved atoms' slots updated).  The
-// flags are cleared; k_bin_insert sets them again for the blocks it
-// appends to.  Four blocks per thread: their flags are read as one word.
-__global__ void k_bin_compact(DevState d) {
-  const int64_t w4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w4 * 4 >= d.nblocks) return;
-  unsigned *fw = reinterpret_cast<unsigned *>(d.bflag) + w4;
-  const unsigned f = *fw;
-  if (!f) return;
-  *fw = 0u;
-  for (int q = 0; q < 4; ++q) {
-    if (!((f >> (8 * q)) & 0xffu)) continue;
-    const int64_t b = w4 * 4 + q;
-    const int base = (int)(b << d.bin_shift);
-    const int c = min(d.count[b], 1 << d.bin_shift);
-    int *p = d.sorted + base;
-    int w = 0;
-    for (int r = 0; r < c; ++r) {
-      const int v = p[r];
-      if (v < 0) continue;
-      if (w != r) {
-        p[w] = v;
-        d.slot[v] = base + w;
-      }
-      ++w;
-    }
-    d.count[b] = w;
-  }
-}
-
 // Atoms without a slot (those that changed block, or all of them after a
-// kBinOnly pass) appended to their block's list, kInsertPer per thread with
-// their counting atomics issued together.  A block with more atoms than
-// slots keeps the first ones and raises kFlagBinOverflow.
+// kBinOnly pass) put into the first free slot of their block's list by an
+// atomic compare-and-swap, so holes left by atoms that moved away are
+// refilled and a list does not need closing up; the block's count (its
+// atoms) rises by one.  kInsertPer atoms per thread.  A block without a
+// free slot keeps its atoms and raises kFlagBinOverflow.
 __global__ __launch_bounds__(kBinThreads) void k_bin_insert(DevState d,
                                                             StepLog *flag_log) {
   const int64_t base =
       (int64_t)blockIdx.x * (kBinThreads * kInsertPer) + threadIdx.x;
-  int c[kInsertPer], r[kInsertPer];
-#pragma unroll
-  for (int k = 0; k < kInsertPer; ++k) {
-    const int64_t i = base + (int64_t)k * kBinThreads;
-    c[k] = (i < d.n && d.slot[i] < 0) ? d.cell[i] : -1;
-  }
-#pragma unroll
-  for (int k = 0; k < kInsertPer; ++k)
-    if (c[k] >= 0) r[k] = atomicAdd(&d.count[c[k]], 1);
+  const int S = 1 << d.bin_shift;
   unsigned flags = 0;
 #pragma unroll
   for (int k = 0; k < kInsertPer; ++k) {
-    if (c[k] < 0) continue;
-    if (r[k] < (1 << d.bin_shift)) {
-      const int64_t i = base + (int64_t)k * kBinThreads;
-      const int sl = (c[k] << d.bin_shift) + r[k];
-      d.sorted[sl] = (int)i;
-      d.slot[i] = sl;
-      d.bflag[c[k]] = 1;
+    const int64_t i = base + (int64_t)k * kBinThreads;
+    if (i >= d.n || d.slot[i] >= 0) continue;
+    const int c = d.cell[i];
+    int *p = d.sorted + ((int64_t)c << d.bin_shift);
+    int got = -1;
+    // eight slots read at a time; a slot read as free may have been taken
+    // since, and then the exchange fails and the search goes on
+    for (int s0 = 0; s0 < S && got < 0; s0 += 8) {
+      const int4 a = *reinterpret_cast<const int4 *>(p + s0);
+      const int4 b = *reinterpret_cast<const int4 *>(p + s0 + 4);
+      const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      for (int q = 0; q < 8 && got < 0; ++q)
+        if (v[q] < 0 && atomicCAS(&p[s0 + q], -1, (int)i) == -1) got = s0 + q;
+    }
+    if (got >= 0) {
+      d.slot[i] = (int)(((int64_t)c << d.bin_shift) + got);
+      atomicAdd(&d.count[c], 1);
+      d.bflag[c] = 1;
     } else {
       flags |= kFlagBinOverflow;
     }
@@ -471,22 +449,34 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int *s_wave,
 }
 
 // Input order inside each block (the order the reference's linked list
-// gives, SRC/exaMD_input_formats.f90:80-88): insertion sort of the few
-// atom ids of each block that received atoms, their slots updated.  Only
-// the FP64 path needs it (its sums depend on the order of the pairs); the
-// mixed path's fixed-point sums do not.
+// gives, SRC/exaMD_input_formats.f90:80-88): each list that received atoms
+// in this step (flagged by k_bin_insert) closed up (holes removed, the tail
+// left free) and put in input order by an insertion sort of its few ids,
+// the slots updated and the flag cleared.  A list that only lost atoms
+// keeps the rest in order around its holes.  Only the FP64 path needs it
+// (its sums depend on the order of the pairs); the mixed path's fixed-point
+// sums do not, and its lists keep their holes until insertions refill them.
 __global__ void k_block_order(DevState d) {
   const int64_t w4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w4 * 4 >= d.nblocks) return;
-  const unsigned f = reinterpret_cast<const unsigned *>(d.bflag)[w4];
+  unsigned *fw = reinterpret_cast<unsigned *>(d.bflag) + w4;
+  const unsigned f = *fw;
   if (!f) return;
+  *fw = 0u;
+  const int S = 1 << d.bin_shift;
   for (int q = 0; q < 4; ++q) {
     if (!((f >> (8 * q)) & 0xffu)) continue;
     const int64_t b = w4 * 4 + q;
-    const int c = min(d.count[b], 1 << d.bin_shift);
-    if (c < 2) continue;
     const int base = (int)(b << d.bin_shift);
     int *p = d.sorted + base;
+    int c = 0, last = -1;
+    for (int r = 0; r < S; ++r) {
+      const int v = p[r];
+      if (v < 0) continue;
+      p[c++] = v;
+      last = r;
+    }
+    for (int r = c; r <= last; ++r) p[r] = -1;
     for (int k = 1; k < c; ++k) {
       int v = p[k];
       int m = k - 1;
@@ -851,22 +841,34 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     const int bc = cx | (cy << 8) | (cz << 16) | ((sx + 1) << 24) |
                    ((sy + 1) << 26) | ((sz + 1) << 28);
     const int o = s_off[rb], c = s_off[rb + 1] - o;
-    // a block's slots start 32-byte aligned (at least 8 per block): the
-    // first eight ids come in two 16-byte loads issued together
+    // the block's c atoms are the valid entries of its slots (a list may
+    // hold holes, -1, where atoms moved away); the slots start 32-byte
+    // aligned (at least 8 per block) and come eight at a time in two
+    // 16-byte loads issued together
     const int *src = A.d.sorted + s_gst[rb];
-    int4 q0 = make_int4(0, 0, 0, 0), q1 = make_int4(0, 0, 0, 0);
-    if (c > 0) q0 = *reinterpret_cast<const int4 *>(src);
-    if (c > 4) q1 = *reinterpret_cast<const int4 *>(src + 4);
-    const int ids[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    const int S = 1 << A.d.bin_shift;
+    int w = 0;
+    for (int s0 = 0; w < c && s0 < S; s0 += 8) {
+      const int4 q0 = *reinterpret_cast<const int4 *>(src + s0);
+      const int4 q1 = *reinterpret_cast<const int4 *>(src + s0 + 4);
+      const int ids[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < c) {
-        s_id[o + k] = ids[k];
-        s_bc[o + k] = bc;
+      for (int k = 0; k < 8; ++k)
+        if (ids[k] >= 0 && w < c) {
+          s_id[o + w] = ids[k];
+          s_bc[o + w] = bc;
+          ++w;
+        }
+    }
+    // a count above the list's valid entries would leave LDS entries
+    // unset: they get the system's first atom (so that no read goes out of
+    // bounds) and the list flag, whose recovery bins afresh
+    if (w < c) {
+      atomicOr(&log->flags, (unsigned long long)kFlagBinOverflow);
+      for (; w < c; ++w) {
+        s_id[o + w] = (int)sd.atom0;
+        s_bc[o + w] = bc;
       }
-    for (int k = 8; k < c; ++k) {
-      s_id[o + k] = src[k];
-      s_bc[o + k] = bc;
     }
   }
   __syncthreads();
@@ -1517,6 +1519,8 @@ void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
   unsigned nb = nblk(d.n, kBinThreads * kBinPer);
   if (nb == 0) return;
   if (mode == kBinOnly) {
+    // empty lists: every slot free, every count zero
+    launch_fill_i32(d.sorted, -1, d.nblocks << d.bin_shift, s);
     launch_fill_i32(d.count, 0, d.nblocks, s);
     launch_fill_i32(reinterpret_cast<int *>(d.bflag), 0,
                     bin_flag_bytes(d.nblocks) / 4, s);
@@ -1560,11 +1564,10 @@ void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
   if (t.nsys > 0) k_sys_ke<<<t.nsys, kBinThreads, 0, s>>>(d, t, log);
 }
 
-void launch_bin_update(const DevState &d, bool after_bin_only, bool block_order,
-                       StepLog *flag_log, hipStream_t s) {
+void launch_bin_update(const DevState &d, bool block_order, StepLog *flag_log,
+                       hipStream_t s) {
   if (d.n <= 0) return;
   const unsigned gb = nblk(bin_flag_bytes(d.nblocks) / 4, 256);
-  if (!after_bin_only) k_bin_compact<<<gb, 256, 0, s>>>(d);
   k_bin_insert<<<nblk(d.n, kBinThreads * kInsertPer), kBinThreads, 0, s>>>(d, flag_log);
   if (block_order) k_block_order<<<gb, 256, 0, s>>>(d);
 }

@@ -7,9 +7,12 @@ namespace examd {
 
 // Device state of one system.  Per-atom arrays are SoA in input order; the
 // block-sorted order exists only as the index array `sorted`, which gives
-// every block 2^bin_shift slots: block b's atoms are the first count[b]
-// entries from b << bin_shift.  The lists persist from step to step and
-// only the atoms that change block are moved.
+// every block 2^bin_shift slots from b << bin_shift: block b's atoms are
+// its valid slots (atom ids; -1 marks a free slot) and count[b] their
+// number.  The lists persist from step to step and only the atoms that
+// change block are moved: a leaving atom frees its slot, an arriving one
+// takes the first free slot (FP64 lists are then closed up and put in
+// input order).
 struct DevState {
   int64_t n = 0;
   int64_t nblocks = 0;
@@ -21,7 +24,7 @@ struct DevState {
   int *slot = nullptr;     // the atom's entry in `sorted` (-1: to insert)
   int *count = nullptr;    // atoms per block (nblocks)
   int *sorted = nullptr;   // atom ids by block (nblocks << bin_shift)
-  unsigned char *bflag = nullptr;  // blocks whose list changed this step
+  unsigned char *bflag = nullptr;  // blocks whose list received atoms
   int bin_shift = 3;       // log2 of the slots per block
   StepLog *log = nullptr;  // per-step tallies
   unsigned long long *home = nullptr;  // home-block pair counts (diagnostic)
@@ -104,13 +107,12 @@ void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
 // Bytes of DevState::bflag for nblocks blocks (whole ints, so that it can
 // be cleared by launch_fill_i32).
 constexpr int64_t bin_flag_bytes(int64_t nblocks) { return (nblocks + 3) / 4 * 4; }
-// The block lists after a binning pass: blocks that lost atoms closed up
-// in order (not after a kBinOnly pass, which cleared them), the atoms
-// without a slot appended to their block (kFlagBinOverflow into
-// flag_log when a block runs out of slots), and with block_order each
-// changed block put in input order.
-void launch_bin_update(const DevState &d, bool after_bin_only, bool block_order,
-                       StepLog *flag_log, hipStream_t s);
+// The block lists after a binning pass: the atoms without a slot put into
+// the first free slot of their block's list (kFlagBinOverflow into
+// flag_log when a block has none), and with block_order each changed list
+// closed up and put in input order.
+void launch_bin_update(const DevState &d, bool block_order, StepLog *flag_log,
+                       hipStream_t s);
 // Tallies of system k go to log[k].
 void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                    const ForceConfig &c, bool mixed, bool energy,

@@ -277,8 +277,10 @@ typedef struct examd_tune {
     int32_t tile_edge;    /* set_config: when tiles are 0, every system's map
                              is T = max(1, round(nblocks_xyz / tile_edge));
                              in a batch each map is then coarsened while its
-                             largest region stays within the batch's largest
-                             (atoms and blocks);
+                             largest region fits the LDS budget of one
+                             workgroup (the whole LDS for 1024 threads, a
+                             proportional share for fewer), in atoms and in
+                             region blocks;
                              get_config: the edge the map came from (0 when
                              tiles were given) */
     int32_t workgroups;   /* persistent force grid: workgroups that loop over

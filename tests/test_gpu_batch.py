@@ -225,8 +225,8 @@ def test_balanced_map_matches_oracle(E, oracle, prec):
 
 def test_balanced_batch_equals_standalone(E, oracle):
     """Balanced maps from a tile edge in a batch: each system's map is
-    coarsened as far as the batch's largest region allows (some systems get
-    fewer cell blocks than from the edge alone), its mixed forces are
+    coarsened as far as the LDS budget of one workgroup allows (some systems
+    get fewer cell blocks than from the edge alone), its mixed forces are
     bit-identical to the system alone under the same map, and its FP64
     forces under the coarsened map match the oracle (1e-10, pair counts
     exact)."""
@@ -265,7 +265,7 @@ def test_batch_log_uniform_32_systems_match_oracle(E, oracle):
     """The batching workload's shape at 32 systems: atom counts log-uniform
     in [500, 20000], densities uniform in [0.60, 0.90], temperatures uniform
     in [0.70, 1.50] (randomly depleted FCC lattices), under maps from a tile
-    edge coarsened within the batch's largest region, boundaries at equal
+    edge coarsened within the LDS budget of a workgroup, boundaries at equal
     atom shares: every system's FP64 forces within 1e-10 of the oracle on
     that system alone, pair counts exact; the mixed forces within the FP32
     bars and summing to zero per system in fixed point."""
