@@ -48,6 +48,9 @@ struct TileConfig {
   bool fixed_tiles = false;
   int fixed_groups = 0;
   int rebalance = 0;      // steps between re-partitions in a run (0 = never)
+  // tiles of each XCD's share of the force launch (9 boundaries: contiguous
+  // ranges in launch order of about equal region atoms)
+  std::vector<int> xcd;
 };
 
 // What a captured chunk depends on.
@@ -65,6 +68,8 @@ struct DevTables {
   SysDesc *sys = nullptr;
   int *tile_sys = nullptr;
   int *tile_seg = nullptr;
+  int *xcd_start = nullptr;   // kXcds + 1 tile boundaries
+  int *xcd_next = nullptr;    // kXcds fetch counters and a finish counter
   int *cuts = nullptr;
   int tile_cap = 0;
   int cut_cap = 0;
@@ -160,6 +165,8 @@ void free_tables(DevTables &t) {
   if (t.sys) (void)hipFree(t.sys);
   if (t.tile_sys) (void)hipFree(t.tile_sys);
   if (t.tile_seg) (void)hipFree(t.tile_seg);
+  if (t.xcd_start) (void)hipFree(t.xcd_start);
+  if (t.xcd_next) (void)hipFree(t.xcd_next);
   if (t.cuts) (void)hipFree(t.cuts);
   t = DevTables{};
 }
@@ -222,6 +229,8 @@ SysTable table(const examd_handle *h, int prec) {
   t.sys = h->tab[prec].sys;
   t.tile_sys = h->tab[prec].tile_sys;
   t.tile_seg = h->tab[prec].tile_seg;
+  t.xcd_start = h->tab[prec].xcd_start;
+  t.xcd_next = h->tab[prec].xcd_next;
   t.atom_seg = h->atom_seg;
   t.cuts = h->tab[prec].cuts;
   t.nsys = h->nsys;
@@ -262,6 +271,21 @@ int upload_tables(examd_handle *h, int prec) {
   seg[nsys] = tiles;
   if (!t.sys && dalloc(&t.sys, nsys)) return EXAMD_ERR_NOMEM;
   if (!t.tile_seg && dalloc(&t.tile_seg, nsys + 1)) return EXAMD_ERR_NOMEM;
+  if (!t.xcd_start && dalloc(&t.xcd_start, kXcds + 1)) return EXAMD_ERR_NOMEM;
+  if (!t.xcd_next) {
+    // the counters start at zero; each force launch leaves them at zero
+    if (dalloc(&t.xcd_next, kXcds + 1)) return EXAMD_ERR_NOMEM;
+    HIPCHK(hipMemset(t.xcd_next, 0, (kXcds + 1) * sizeof(int)));
+  }
+  // each XCD's share of the tiles: the planned equal-cost ranges, or equal
+  // counts for a map planned without block counts
+  std::vector<int> xr = c.xcd;
+  if ((int)xr.size() != kXcds + 1 || xr[kXcds] != tiles) {
+    xr.assign(kXcds + 1, 0);
+    for (int x = 0; x <= kXcds; ++x) xr[x] = (int)((int64_t)tiles * x / kXcds);
+  }
+  HIPCHK(hipMemcpy(t.xcd_start, xr.data(), (kXcds + 1) * sizeof(int),
+                   hipMemcpyHostToDevice));
   if (t.tile_cap < tiles) {
     if (t.tile_sys) (void)hipFree(t.tile_sys);
     t.tile_sys = nullptr;
@@ -540,6 +564,21 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
                                typed) > h->lds_limit)
           cap2 -= 16;
         c.cap = (int)cap2;
+      }
+      // the tiles in launch order (systems in order, x fastest) split into
+      // eight contiguous ranges of about equal region atoms, one per XCD
+      {
+        std::vector<int64_t> cost;
+        for (int k = 0; k < nsys; ++k) {
+          const Grid g = grid_of(h, prec, k);
+          const size_t t0 = cost.size();
+          cost.resize(t0 + (size_t)g.T[0] * g.T[1] * g.T[2]);
+          for_region_atoms(g, cuts_of(h, prec, k, g), cnt + h->sys[k].block0,
+                           [&](int tx, int ty, int tz, int64_t v) {
+                             cost[t0 + tx + (size_t)g.T[0] * (ty + (size_t)g.T[1] * tz)] = v;
+                           });
+        }
+        c.xcd = equal_cost_ranges(cost, kXcds);
       }
       // about three work items per lane: run groups from the mean region
       double reg_atoms = 0.0, reg_tiles = 0.0;

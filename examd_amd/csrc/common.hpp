@@ -190,8 +190,9 @@ __host__ __device__ inline void tile_range(int t, int nb, int T, int &lo,
 // Spread consecutive tiles over the 8 XCDs in contiguous chunks: the
 // hardware hands workgroup w to XCD w % 8, so workgroup w takes tile
 // base(w % 8) + w / 8 and each XCD's L2 sees a compact slab of tiles.
+constexpr int kXcds = 8;   // XCDs of an MI355X; workgroup w runs on XCD w % 8
 __device__ inline int xcd_remap(int wg, int G) {
-  const int nx = 8;
+  const int nx = kXcds;
   int q = G / nx, r = G % nx;
   int x = wg % nx, k = wg / nx;
   int base = x * q + (x < r ? x : r);

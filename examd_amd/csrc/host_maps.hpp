@@ -93,12 +93,12 @@ void region_line(int n, const std::vector<int> &c, int T, V v,
   for (int t = 0; t < T; ++t) out[(size_t)t * ostride] = q[c[t + 1] + 2] - q[c[t]];
 }
 
-// Largest number of atoms any tile's region (the tile plus one block on each
-// side, periodic) holds.  cnt: the system's atoms per block, x fastest.  On
-// a rectilinear map the region sums separate: along x to one sum per (tile
-// column, y, z), then along y, then along z.
-template <class G>
-int64_t max_region_atoms(const G &g, const Cuts &cb, const int *cnt) {
+// The atoms of every tile's region (the tile plus one block on each side,
+// periodic), passed to f(tx, ty, tz, atoms).  cnt: the system's atoms per
+// block, x fastest.  On a rectilinear map the region sums separate: along x
+// to one sum per (tile column, y, z), then along y, then along z.
+template <class G, class F>
+void for_region_atoms(const G &g, const Cuts &cb, const int *cnt, F f) {
   const int nx = g.nb[0], ny = g.nb[1], nz = g.nb[2];
   const int Tx = g.T[0], Ty = g.T[1], Tz = g.T[2];
   std::vector<int64_t> pre;
@@ -119,16 +119,42 @@ int64_t max_region_atoms(const G &g, const Cuts &cb, const int *cnt) {
                   &B[(size_t)z * Ty * Tx + tx], (size_t)Tx);
     }
   // then over z
-  int64_t best = 0;
   std::vector<int64_t> C((size_t)Tz);
   for (int ty = 0; ty < Ty; ++ty)
     for (int tx = 0; tx < Tx; ++tx) {
       const int64_t *col = &B[(size_t)ty * Tx + tx];
       region_line(nz, cb[2], Tz, [&](int z) { return col[(size_t)z * Ty * Tx]; },
                   pre, C.data(), 1);
-      for (int64_t v : C) best = std::max(best, v);
+      for (int tz = 0; tz < Tz; ++tz) f(tx, ty, tz, C[(size_t)tz]);
     }
+}
+
+// Largest number of atoms any tile's region holds.
+template <class G>
+int64_t max_region_atoms(const G &g, const Cuts &cb, const int *cnt) {
+  int64_t best = 0;
+  for_region_atoms(g, cb, cnt,
+                   [&](int, int, int, int64_t v) { best = std::max(best, v); });
   return best;
+}
+
+// Boundaries of nparts contiguous ranges of the tiles (in launch order) of
+// about equal total cost: part x is [b[x], b[x + 1]).
+inline std::vector<int> equal_cost_ranges(const std::vector<int64_t> &cost,
+                                          int nparts) {
+  const int n = (int)cost.size();
+  std::vector<int> b((size_t)nparts + 1, n);
+  b[0] = 0;
+  int64_t total = 0;
+  for (int64_t v : cost) total += v;
+  int64_t acc = 0;
+  int x = 1;
+  for (int i = 0; i < n && x < nparts; ++i) {
+    acc += cost[(size_t)i];
+    // part x starts after the tile that brings the sum to x / nparts
+    while (x < nparts && acc * nparts >= total * x) b[(size_t)x++] = i + 1;
+  }
+  return b;
 }
 
 // Coarsen a map while its largest region stays within a launch's limits:

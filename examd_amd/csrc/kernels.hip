@@ -1420,18 +1420,58 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   if (prof && tid == 0) prof[4] = clock64();
 }
 
-// Force kernel: workgroup b evaluates tiles b, b + gridDim, ... (one tile
-// per workgroup when the grid covers every tile; a persistent grid loops).
-// Tiles are taken in the XCD-aware order, so the tiles of one XCD's
-// workgroups are contiguous in space (the grid is a multiple of 8 when it
-// loops).
+// Force kernel.  A persistent grid (fewer workgroups than tiles) with
+// per-XCD tile ranges (xcd_start): each workgroup takes the next tile of its
+// XCD's range from that XCD's counter (workgroup w runs on XCD w % 8), so
+// the tiles of one XCD stay contiguous in launch order (shared ghost atoms
+// in its L2) while the ranges, of about equal region atoms, and the dynamic
+// take balance the load; a workgroup whose range is used up takes tiles of
+// the next XCDs' ranges.  Every tile is taken exactly once whichever XCD a
+// workgroup runs on, and the last workgroup to finish clears the counters
+// for the next launch.  Otherwise (one workgroup per tile, whose launch the
+// hardware already spreads as CUs free up, or the allocation-free form
+// without ranges) workgroup b evaluates tiles b, b + gridDim, ... in the
+// XCD-aware order.
 template <bool FP64, bool ENERGY, bool HOMECNT, int PK, int NT>
 __global__ __launch_bounds__(NT) void k_forces(ForceArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int ntiles = A.t.ntiles;
-  for (int wg = blockIdx.x; wg < ntiles; wg += gridDim.x) {
-    if (wg != (int)blockIdx.x) __syncthreads();  // LDS of the last tile
-    force_tile<FP64, ENERGY, HOMECNT, PK, NT>(A, xcd_remap(wg, ntiles), smem);
+  if (!A.t.xcd_start || (int)gridDim.x >= ntiles) {
+    for (int wg = blockIdx.x; wg < ntiles; wg += gridDim.x) {
+      if (wg != (int)blockIdx.x) __syncthreads();  // LDS of the last tile
+      force_tile<FP64, ENERGY, HOMECNT, PK, NT>(A, xcd_remap(wg, ntiles), smem);
+    }
+    return;
+  }
+  __shared__ int s_tile;
+  int x = (int)(blockIdx.x % kXcds), tried = 0;
+  for (int first = 1;; first = 0) {
+    if (!first) __syncthreads();  // LDS of the last tile, and s_tile read
+    if (threadIdx.x == 0) {
+      int tile = -1;
+      while (tried < kXcds) {
+        const int k = A.t.xcd_start[x] + atomicAdd(&A.t.xcd_next[x], 1);
+        if (k < A.t.xcd_start[x + 1]) {
+          tile = k;
+          break;
+        }
+        x = (x + 1) % kXcds;
+        ++tried;
+      }
+      s_tile = tile;
+    }
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile < 0) break;
+    force_tile<FP64, ENERGY, HOMECNT, PK, NT>(A, tile, smem);
+  }
+  if (threadIdx.x == 0) {
+    // every fetch of this workgroup came before its finish
+    __threadfence();
+    if (atomicAdd(&A.t.xcd_next[kXcds], 1) == (int)gridDim.x - 1) {
+      for (int i = 0; i < kXcds; ++i) A.t.xcd_next[i] = 0;
+      A.t.xcd_next[kXcds] = 0;
+    }
   }
 }
 

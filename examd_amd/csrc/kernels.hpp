@@ -60,6 +60,12 @@ struct SysTable {
   const SysDesc *sys = nullptr;        // nsys entries, tile map of one precision
   const int *tile_sys = nullptr;       // system of each tile (ntiles)
   const int *tile_seg = nullptr;       // first tile of each system (nsys + 1)
+  // the force launch's tiles split into one contiguous range per XCD
+  // (kXcds + 1 boundaries) and the workgroups' fetch counters (kXcds, then
+  // a finish counter; all zero between launches); nullptr: the static
+  // XCD-aware order
+  const int *xcd_start = nullptr;
+  int *xcd_next = nullptr;
   const long long *atom_seg = nullptr; // first atom of each system (nsys + 1)
   const int *cuts = nullptr;           // cell-block boundaries of balanced maps
   int nsys = 1;

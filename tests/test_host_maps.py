@@ -10,7 +10,11 @@ compiled with the host compiler alone:
 - the block of a position (exaMD_identify_block: x/bl + 1 truncated, clamped
   to the grid) on exact faces, one ulp below them, and past the last face;
 - the coarsening of a batch system's map: within the limits on the largest
-  region (atoms, blocks), and no axis can give up one more cell block.
+  region (atoms, blocks), and no axis can give up one more cell block;
+- every cell block's region atoms (the per-XCD cost split of a force launch)
+  against the same brute force, cell block by cell block;
+- the split of a launch's tiles into contiguous ranges of about equal cost:
+  ordered, covering, each range's cost within one tile of an equal share.
 """
 import os
 import shutil
@@ -64,6 +68,27 @@ int main() {
           g, [&](const G &t) { return examd::balanced_cuts(t, cnt.data(), n); },
           cnt.data(), lim_atoms, lim_blocks);
       std::printf("%d %d %d\n", g.T[0], g.T[1], g.T[2]);
+    } else if (mode == 4) {     // every tile's region atoms, x fastest
+      examd::Cuts cb;
+      for (int a = 0; a < 3; ++a) {
+        cb[a].resize(g.T[a] + 1);
+        for (int &v : cb[a]) std::scanf("%d", &v);
+      }
+      std::vector<int> cnt(nblk);
+      for (int &v : cnt) std::scanf("%d", &v);
+      std::vector<long long> out((size_t)g.T[0] * g.T[1] * g.T[2]);
+      examd::for_region_atoms(g, cb, cnt.data(), [&](int x, int y, int z, int64_t v) {
+        out[x + (size_t)g.T[0] * (y + (size_t)g.T[1] * z)] = v;
+      });
+      for (long long v : out) std::printf("%lld ", v);
+      std::printf("\n");
+    } else if (mode == 5) {     // equal-cost ranges: nparts, n, costs
+      int parts, n;
+      std::scanf("%d %d", &parts, &n);
+      std::vector<int64_t> cost(n);
+      for (auto &v : cost) { long long t; std::scanf("%lld", &t); v = t; }
+      for (int b : examd::equal_cost_ranges(cost, parts)) std::printf("%d ", b);
+      std::printf("\n");
     } else {                    // block of positions: bl, count, values
       double bl;
       int m;
@@ -223,3 +248,71 @@ def test_coarsened_map_within_limits_and_maximal(driver):
                 assert (_region_max(cnt, cm) > lim_a or
                         _region_blocks(cm) > lim_b), (k, a, T)
     assert coarsened >= 6
+
+
+def _region_all(cnt, cuts):
+    """Brute force: every cell block's region atoms, x fastest."""
+    nz, ny, nx = cnt.shape
+    out = []
+    for tz in range(len(cuts[2]) - 1):
+        iz = np.arange(cuts[2][tz] - 1, cuts[2][tz + 1] + 1) % nz
+        for ty in range(len(cuts[1]) - 1):
+            iy = np.arange(cuts[1][ty] - 1, cuts[1][ty + 1] + 1) % ny
+            for tx in range(len(cuts[0]) - 1):
+                ix = np.arange(cuts[0][tx] - 1, cuts[0][tx + 1] + 1) % nx
+                out.append(int(cnt[np.ix_(iz, iy, ix)].sum()))
+    return out
+
+
+def test_region_atoms_of_every_cell_block(driver):
+    """for_region_atoms (the costs of the per-XCD split) equals a numpy sum
+    over each cell block's periodic region, on random maps and counts,
+    including one cell block per axis (the region wraps onto itself)."""
+    rng = np.random.default_rng(12)
+    lines, want = [], []
+    for case in range(30):
+        nb = rng.integers(5, 14, 3)
+        T = [int(rng.integers(1, max(2, v // 2) + 1)) for v in nb]
+        if case % 7 == 0:
+            T = [1, 1, 1]
+        cuts = [_random_cuts(rng, int(nb[a]), T[a]) for a in range(3)]
+        cnt = rng.integers(0, 6, (nb[2], nb[1], nb[0]))
+        lines.append(" ".join(str(v) for v in
+                              [4, *nb.tolist(), *T, *sum(cuts, []), *cnt.ravel().tolist()]))
+        want.append(_region_all(cnt, cuts))
+    out = driver(f"{len(lines)}\n" + "\n".join(lines) + "\n")
+    for k, w in enumerate(want):
+        assert [int(v) for v in out[k].split()] == w, k
+
+
+def test_equal_cost_ranges(driver):
+    """Ranges of tiles for the XCDs: boundaries ordered from 0 to n, the
+    smallest ones at which the prefix cost reaches each x / parts of the
+    total (so every range's cost is within the largest tile of an equal
+    share), and equal counts for equal costs."""
+    rng = np.random.default_rng(13)
+    lines, cases = [], []
+    for case in range(40):
+        n = int(rng.integers(1, 300))
+        parts = 8
+        cost = (rng.integers(1, 3000, n) if case % 5 else np.full(n, 7)).tolist()
+        lines.append(" ".join(str(v) for v in [5, 1, 1, 1, 1, 1, 1, parts, n, *cost]))
+        cases.append((parts, cost))
+    out = driver(f"{len(lines)}\n" + "\n".join(lines) + "\n")
+    for k, (parts, cost) in enumerate(cases):
+        b = [int(v) for v in out[k].split()]
+        n = len(cost)
+        assert len(b) == parts + 1 and b[0] == 0 and b[-1] == n, k
+        assert all(b[i] <= b[i + 1] for i in range(parts)), k
+        pre = np.concatenate([[0], np.cumsum(cost)])
+        total = pre[-1]
+        for x in range(1, parts):
+            # the first boundary whose prefix reaches x / parts of the total
+            i = int(np.argmax(pre * parts >= total * x))
+            assert b[x] == min(i, n), (k, x)
+        share = total / parts
+        for x in range(parts):
+            assert pre[b[x + 1]] - pre[b[x]] <= share + max(cost) + 1e-9, (k, x)
+        if len(set(cost)) == 1:
+            sizes = np.diff(b)
+            assert sizes.max() - sizes.min() <= 1, k
