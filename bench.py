@@ -236,10 +236,11 @@ def batch_systems(E, nsys=256, seed=5):
 
 def batch_bench(E, steps, warmup, loop_steps, nsys=256):
     """Aggregate atom-steps/s of the batch in one handle (every kernel of a
-    step covers all systems; 100-step hipGraph chunks; autotuned tile edge
-    and launch shape) against a per-system launch loop: each system in its
-    own handle with the same shape and its own 100-step graphs, run in turn
-    for loop_steps steps."""
+    step covers all systems; 100-step hipGraph chunks; tile edge and launch
+    shape autotuned after 300 untimed steps that take the perturbed lattices
+    towards their liquids, as for the single system) against a per-system
+    launch loop: each system in its own handle with the same shape and its
+    own 100-step graphs, run in turn for loop_steps steps."""
     sy = batch_systems(E, nsys)
     total = int(sum(len(p) for p, _, _ in sy))
     systems = [(len(p), box) for p, _, box in sy]
@@ -247,6 +248,7 @@ def batch_bench(E, steps, warmup, loop_steps, nsys=256):
     vel = np.vstack([v for _, v, _ in sy])
     with E.MidpointLJBatch(systems, RC, graph_chunk=100) as b:
         b.set_state(pos, vel)
+        b.run(300, DT, "mixed", energy_every=0)
         tune = b.autotune("mixed", trials=2)
         if warmup:
             b.run(warmup, DT, "mixed")

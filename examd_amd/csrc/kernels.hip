@@ -237,6 +237,34 @@ __device__ __forceinline__ int bin_store(const DevState &d, const Grid &g,
   return cell;
 }
 
+// Atom i into the first free slot of block c's list by an atomic
+// compare-and-swap, so holes left by atoms that moved away are refilled and
+// a list never needs closing up; the block's count rises by one and its
+// list is flagged as changed.  The slots are read eight at a time as a
+// hint: a slot read as free may have been taken since, and then the
+// exchange fails and the search goes on.  Without a free slot the atom
+// stays out of every list (slot -1) and false is returned.
+__device__ __forceinline__ bool list_insert(const DevState &d, int64_t i, int c) {
+  const int S = 1 << d.bin_shift;
+  int *p = d.sorted + ((int64_t)c << d.bin_shift);
+  int got = -1;
+  for (int s0 = 0; s0 < S && got < 0; s0 += 8) {
+    const int4 a = *reinterpret_cast<const int4 *>(p + s0);
+    const int4 b = *reinterpret_cast<const int4 *>(p + s0 + 4);
+    const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    for (int q = 0; q < 8 && got < 0; ++q)
+      if (v[q] < 0 && atomicCAS(&p[s0 + q], -1, (int)i) == -1) got = s0 + q;
+  }
+  if (got < 0) {
+    d.slot[i] = -1;
+    return false;
+  }
+  d.slot[i] = (int)(((int64_t)c << d.bin_shift) + got);
+  atomicAdd(&d.count[c], 1);
+  d.bflag[c] = 1;
+  return true;
+}
+
 // kBinPer atoms per thread (coalesced, kBinThreads apart).  Every atom's
 // loads are issued before the first store, so they are in flight together
 // (the stores could otherwise alias the next atom's loads as far as the
@@ -244,8 +272,14 @@ __device__ __forceinline__ int bin_store(const DevState &d, const Grid &g,
 // (SINGLE) reads its grid once per thread.  kBinOnly starts the block
 // lists afresh: every atom is inserted by k_bin_insert.  The other modes
 // keep them: an atom that changes block frees its slot in its old block's
-// list (the block's count drops by one) and is inserted into the new one;
-// the others keep their slots.
+// list (the block's count drops by one) and takes a free slot in the new
+// one's (list_insert); the others keep their slots.  Every change of a
+// slot in this pass is an atomic operation (the free an exchange, the take
+// a compare-and-swap), since other threads take slots of the same lists
+// meanwhile.  A list whose last free slot is held by an atom that leaves
+// in the same pass may look full to an arriving atom: that takes eight
+// atoms in one block of rc/2, and it raises kFlagBinOverflow as a full
+// list does.
 template <bool MIXED, int MODE, bool SINGLE>
 __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     DevState d, SysTable t, double h, double dt, int want_ke,
@@ -305,10 +339,10 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
         // insertions) and the old block's count drops by one; the other
         // atoms keep their slots and their order
         if (at[k].slot >= 0) {
-          d.sorted[at[k].slot] = -1;
+          atomicExch(&d.sorted[at[k].slot], -1);
           atomicSub(&d.count[at[k].old], 1);
         }
-        d.slot[i] = -1;
+        if (!list_insert(d, i, c)) flags |= kFlagBinOverflow;
       }
     }
   }
@@ -331,41 +365,21 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     atomicOr(&flag_log->flags, (unsigned long long)flags);
 }
 
-// Atoms without a slot (those that changed block, or all of them after a
-// kBinOnly pass) put into the first free slot of their block's list by an
-// atomic compare-and-swap, so holes left by atoms that moved away are
-// refilled and a list does not need closing up; the block's count (its
-// atoms) rises by one.  kInsertPer atoms per thread.  A block without a
-// free slot keeps its atoms and raises kFlagBinOverflow.
+// Atoms without a slot (all of them after a kBinOnly pass) put into the
+// first free slot of their block's list, kInsertPer atoms per thread, as
+// list_insert does for the atoms that change block in the other binning
+// modes.  A block without a free slot keeps its atoms and raises
+// kFlagBinOverflow.
 __global__ __launch_bounds__(kBinThreads) void k_bin_insert(DevState d,
                                                             StepLog *flag_log) {
   const int64_t base =
       (int64_t)blockIdx.x * (kBinThreads * kInsertPer) + threadIdx.x;
-  const int S = 1 << d.bin_shift;
   unsigned flags = 0;
 #pragma unroll
   for (int k = 0; k < kInsertPer; ++k) {
     const int64_t i = base + (int64_t)k * kBinThreads;
     if (i >= d.n || d.slot[i] >= 0) continue;
-    const int c = d.cell[i];
-    int *p = d.sorted + ((int64_t)c << d.bin_shift);
-    int got = -1;
-    // eight slots read at a time; a slot read as free may have been taken
-    // since, and then the exchange fails and the search goes on
-    for (int s0 = 0; s0 < S && got < 0; s0 += 8) {
-      const int4 a = *reinterpret_cast<const int4 *>(p + s0);
-      const int4 b = *reinterpret_cast<const int4 *>(p + s0 + 4);
-      const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-      for (int q = 0; q < 8 && got < 0; ++q)
-        if (v[q] < 0 && atomicCAS(&p[s0 + q], -1, (int)i) == -1) got = s0 + q;
-    }
-    if (got >= 0) {
-      d.slot[i] = (int)(((int64_t)c << d.bin_shift) + got);
-      atomicAdd(&d.count[c], 1);
-      d.bflag[c] = 1;
-    } else {
-      flags |= kFlagBinOverflow;
-    }
+    if (!list_insert(d, i, d.cell[i])) flags |= kFlagBinOverflow;
   }
   flags = wave_or(flags);
   if ((threadIdx.x & 63) == 0 && flags)
@@ -1604,11 +1618,12 @@ void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
   if (t.nsys > 0) k_sys_ke<<<t.nsys, kBinThreads, 0, s>>>(d, t, log);
 }
 
-void launch_bin_update(const DevState &d, bool block_order, StepLog *flag_log,
-                       hipStream_t s) {
+void launch_bin_update(const DevState &d, bool fresh, bool block_order,
+                       StepLog *flag_log, hipStream_t s) {
   if (d.n <= 0) return;
   const unsigned gb = nblk(bin_flag_bytes(d.nblocks) / 4, 256);
-  k_bin_insert<<<nblk(d.n, kBinThreads * kInsertPer), kBinThreads, 0, s>>>(d, flag_log);
+  if (fresh)
+    k_bin_insert<<<nblk(d.n, kBinThreads * kInsertPer), kBinThreads, 0, s>>>(d, flag_log);
   if (block_order) k_block_order<<<gb, 256, 0, s>>>(d);
 }
 

@@ -100,7 +100,8 @@ void launch_fill_u64(unsigned long long *p, int64_t n, hipStream_t s);
 
 // ke_log (single system only): kinetic energy of the finished step.
 // kBinOnly starts the block lists afresh (counts and flags cleared, every
-// atom to be inserted); the other modes mark the atoms that change block.
+// atom to be inserted by launch_bin_update); the other modes move the atoms
+// that change block from their old block's list to their new one's.
 void launch_step_bin(const DevState &d, const SysTable &t, bool mixed,
                      int mode, double h, double dt, StepLog *ke_log,
                      StepLog *flag_log, hipStream_t s);
@@ -113,12 +114,12 @@ void launch_sys_ke(const DevState &d, const SysTable &t, StepLog *log,
 // Bytes of DevState::bflag for nblocks blocks (whole ints, so that it can
 // be cleared by launch_fill_i32).
 constexpr int64_t bin_flag_bytes(int64_t nblocks) { return (nblocks + 3) / 4 * 4; }
-// The block lists after a binning pass: the atoms without a slot put into
-// the first free slot of their block's list (kFlagBinOverflow into
-// flag_log when a block has none), and with block_order each changed list
-// closed up and put in input order.
-void launch_bin_update(const DevState &d, bool block_order, StepLog *flag_log,
-                       hipStream_t s);
+// The block lists after a binning pass: after a kBinOnly pass (fresh) every
+// atom put into the first free slot of its block's list (kFlagBinOverflow
+// into flag_log when a block has none), and with block_order each changed
+// list closed up and put in input order.
+void launch_bin_update(const DevState &d, bool fresh, bool block_order,
+                       StepLog *flag_log, hipStream_t s);
 // Tallies of system k go to log[k].
 void launch_forces(const DevState &d, const SysTable &t, const LJParams &p,
                    const ForceConfig &c, bool mixed, bool energy,
