@@ -175,7 +175,8 @@ __device__ __forceinline__ void bin_load(const DevState &d, int64_t i,
 
 // Integrate one loaded atom (mode permitting), store it, zero its force
 // accumulators and return its global block id, assigned as
-// exaMD_identify_block does (flags for a position the block map rejects).
+// exaMD_identify_block does (flags for a position the block map rejects);
+// the caller stores the id where it changed.
 template <bool MIXED, int MODE>
 __device__ __forceinline__ int bin_store(const DevState &d, const Grid &g,
                                          int block0, double h, double dt,
@@ -233,7 +234,6 @@ __device__ __forceinline__ int bin_store(const DevState &d, const Grid &g,
     if (num < 1 || num > nblk) flags |= kFlagOutOfBox;
     else cell = block0 + (int)(num - 1);
   }
-  d.cell[i] = cell;
   return cell;
 }
 
@@ -333,8 +333,10 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
         }
       }
       if (MODE == kBinOnly) {
+        d.cell[i] = c;
         d.slot[i] = -1;
       } else if (c != at[k].old) {
+        d.cell[i] = c;
         // the atom leaves a hole in its old list (refilled by later
         // insertions) and the old block's count drops by one; the other
         // atoms keep their slots and their order
