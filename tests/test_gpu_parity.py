@@ -347,3 +347,52 @@ def test_get_state_returns_forces_of_the_last_evaluation(E, prec):
         assert np.array_equal(f_run, f_now)
     else:
         assert _rel(f_run, f_now) < 1e-13
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_persistent_lists_match_fresh_binning(E, prec):
+    """A hot liquid (256,000 atoms started at T* 5) in which over 1% of the
+    atoms change block every step: 100 steps on block lists kept from step to
+    step, whose binning pass moves the atoms that change block (atomic slot
+    frees and compare-and-swap takes by threads all over the chip), against
+    the same 100 steps with the lists built from scratch before every step
+    (a new state each step).  The map is fixed, so both runs see the same
+    cell-block origins.  Mixed sums do not depend on the order of the atoms
+    in the lists, so the trajectories are bit-identical; the FP64 lists are
+    put back in input order and its sums differ only in the order of their
+    atomic additions (1e-10 bar on positions and velocities)."""
+    pos, vel, box = E.generate_fcc(40, RHO, 5.0, seed=77)
+    nb = np.floor(box / 1.25).astype(int)
+    shape = dict(tiles=(8, 8, 8), block_threads=1024, run_groups=1,
+                 lane_queue=4, workgroups=64, balance=0)
+    steps = 100
+    print(f"[lists {prec}] n={len(pos)} steps={steps}", flush=True)
+
+    def blocks(p):
+        b = np.minimum(np.floor(p / (box / nb)).astype(int), nb - 1)
+        return b[:, 0] + nb[0] * (b[:, 1] + nb[1] * b[:, 2])
+
+    with E.MidpointLJ(len(pos), box, 2.5) as m:
+        m.set_state(pos, vel)
+        m.set_config(prec, **shape)
+        cuts = m.get_cuts(prec)
+        m.run(steps, 0.005, prec)
+        p_run, v_run = m.get_state()[:2]
+        p, v = pos, vel
+        moved = []
+        for _ in range(steps):
+            m.set_state(p, v)
+            b0 = blocks(p)
+            m.run(1, 0.005, prec)
+            p, v = m.get_state()[:2]
+            moved.append(np.mean(blocks(p) != b0))
+        assert all(np.array_equal(a, b) for a, b in zip(cuts, m.get_cuts(prec)))
+    # the stress the test relies on: thousands of list moves per step
+    assert np.mean(moved) > 0.01, np.mean(moved)
+    if prec == "mixed":
+        assert np.array_equal(p_run, p) and np.array_equal(v_run, v)
+    else:
+        dp = np.abs(p_run - p)
+        dp = np.minimum(dp, box - dp).max()
+        assert dp < 1e-10 * box.max(), dp
+        assert np.abs(v_run - v).max() < 1e-10 * np.abs(v).max()
