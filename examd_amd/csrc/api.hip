@@ -406,23 +406,43 @@ int download_aos(examd_handle *h, double *dst) {
 }
 
 // Atoms per block of every system for host positions, in the block order
-// of the binning kernel (x fastest, systems one after another).  Each atom's
-// block is found on several threads, then counted in one pass.
-std::vector<int> host_counts(const examd_handle *h, const double *pos) {
-  std::vector<int> blk((size_t)std::max<int64_t>(h->n, 1));
+// of the binning kernel (x fastest, systems one after another), in one pass
+// on several threads: each atom's block found and counted (relaxed atomic
+// increments: atoms of one block are rare in one thread's range).  With a
+// staging buffer the positions are copied into it in the same pass, and a
+// non-finite coordinate makes the call return false (its atom is not
+// counted).
+bool scan_positions(const examd_handle *h, const double *pos, double *stage,
+                    std::vector<int> &cnt) {
+  cnt.assign((size_t)std::max<int64_t>(h->nblocks, 1), 0);
+  std::atomic<bool> finite(true);
   host_parallel_for(h->n, [&](int64_t i0, int64_t i1) {
+    if (stage)
+      std::memcpy(stage + 3 * i0, pos + 3 * i0, (size_t)(i1 - i0) * 3 * sizeof(double));
+    bool ok = true;
     for (const SysHost &S : h->sys) {
       const int64_t a0 = std::max(i0, S.atom0), a1 = std::min(i1, S.atom0 + S.n);
       const Grid &g = S.g;
       for (int64_t i = a0; i < a1; ++i) {
-        int b[3];
-        for (int a = 0; a < 3; ++a) b[a] = host_block(pos[3 * i + a], g.bl, g.nb[a]);
-        blk[i] = (int)(S.block0 + b[0] + (int64_t)g.nb[0] * (b[1] + (int64_t)g.nb[1] * b[2]));
+        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        if (!(std::isfinite(x) && std::isfinite(y) && std::isfinite(z))) {
+          ok = false;
+          continue;
+        }
+        const int64_t b = S.block0 + host_block(x, g.bl, g.nb[0]) +
+                          (int64_t)g.nb[0] * (host_block(y, g.bl, g.nb[1]) +
+                                              (int64_t)g.nb[1] * host_block(z, g.bl, g.nb[2]));
+        __atomic_fetch_add(&cnt[(size_t)b], 1, __ATOMIC_RELAXED);
       }
     }
+    if (!ok) finite.store(false, std::memory_order_relaxed);
   });
-  std::vector<int> cnt((size_t)std::max<int64_t>(h->nblocks, 1), 0);
-  for (int64_t i = 0; i < h->n; ++i) cnt[blk[i]]++;
+  return finite.load();
+}
+
+std::vector<int> host_counts(const examd_handle *h, const double *pos) {
+  std::vector<int> cnt;
+  scan_positions(h, pos, nullptr, cnt);
   return cnt;
 }
 
@@ -1299,23 +1319,22 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
                         int64_t n) {
   if (!h || !pos || n != h->n) return EXAMD_ERR_ARG;
   HIPCHK(hipSetDevice(h->dev));
-  {
-    std::atomic<bool> nonfinite(false);
-    host_parallel_for(3 * n, [&](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i)
-        if (!std::isfinite(pos[i])) {
-          nonfinite.store(true, std::memory_order_relaxed);
-          return;
-        }
-    });
-    if (nonfinite.load()) return EXAMD_ERR_NONFINITE;
-  }
+  if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
+  // one pass over the caller's positions: finiteness, the block counts and
+  // the copy into the pinned staging buffer (which an earlier copy may
+  // still be reading)
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<int> cnt;
+  if (!scan_positions(h, pos, h->host_stage, cnt)) return EXAMD_ERR_NONFINITE;
+  // the positions travel to the device while the host plans the maps; the
+  // device state changes only once the plans have succeeded
+  HIPCHK(hipMemcpyAsync(h->dev_stage, h->host_stage, 3 * n * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
   {
     // both precisions' maps planned on two threads (each writes only its
     // own TileConfig), then their tables uploaded here.  A plan that fails
     // leaves both maps as they were (the device tables and the state on the
     // device still agree with them).
-    const std::vector<int> cnt = host_counts(h, pos);
     const int maxocc = *std::max_element(cnt.begin(), cnt.end());
     const int stb = ensure_bin_slots(h, maxocc);
     if (stb != EXAMD_OK) return stb;
@@ -1340,13 +1359,10 @@ int examd_hip_set_state(examd_handle *h, const double *pos, const double *vel,
     }
   }
   if (ensure_partials(h)) return EXAMD_ERR_NOMEM;
-  if (ensure_stage(h)) return EXAMD_ERR_NOMEM;
   drop_graph(h);
-  HIPCHK(hipStreamSynchronize(h->stream));
   // one copy per array in the host layout, transposed on the device
-  int st = upload_aos(h, pos);
-  if (st) return st;
   launch_aos_to_soa(h->dev_stage, h->d.x, h->d.y, h->d.z, n, h->stream);
+  int st = EXAMD_OK;
   if (vel) {
     if ((st = upload_aos(h, vel)) != EXAMD_OK) return st;
     launch_aos_to_soa(h->dev_stage, h->d.vx, h->d.vy, h->d.vz, n, h->stream);

@@ -17,12 +17,13 @@ namespace examd {
 using Cuts = std::array<std::vector<int>, 3>;   // T + 1 boundaries per axis
 
 // Block coordinate of a position along one axis (exaMD_identify_block).
+// The clamp is applied before the conversion to int, so that a finite
+// position far outside the box converts without overflow.
 inline int host_block(double x, double bl, int nb) {
   double q = x / bl + 1.0;
-  int v = (int)q;
-  if (v > nb) v = nb;
-  if (v < 1) v = 1;
-  return v - 1;
+  if (q > (double)nb) q = (double)nb;
+  if (q < 1.0) q = 1.0;
+  return (int)q - 1;
 }
 
 // Balanced map: along each axis, boundaries at equal shares of the atoms
