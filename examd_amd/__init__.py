@@ -333,8 +333,13 @@ class _Handle:
                                          self.n), "examd_hip_get_state")
         return (pos, vel, f) if forces else (pos, vel)
 
-    def _forces(self, precision, allow):
-        f = np.zeros((self.n, 3))
+    def _forces(self, precision, allow, out=None):
+        if out is None:
+            f = np.zeros((self.n, 3))
+        else:
+            f = out
+            assert (f.shape == (self.n, 3) and f.dtype == np.float64
+                    and f.flags["C_CONTIGUOUS"])
         s = (Stats * self.nsys)()
         st = _check(lib().examd_hip_forces(self._h, _PREC[precision], _dp(f),
                                            s), "examd_hip_forces", allow)
@@ -503,9 +508,11 @@ class MidpointLJ(_Handle):
         self._h = h
         self._info()
 
-    def forces(self, precision="fp64", allow=()):
-        """(forces (n, 3), Stats, status) at the current positions."""
-        f, s, st = self._forces(precision, allow)
+    def forces(self, precision="fp64", allow=(), out=None):
+        """(forces (n, 3), Stats, status) at the current positions; out: a
+        C-contiguous float64 (n, 3) array to write the forces into (reused
+        by callers that evaluate again and again)."""
+        f, s, st = self._forces(precision, allow, out)
         return f, s[0], st
 
     def run(self, nsteps, dt, precision="fp64", energy_every=0):

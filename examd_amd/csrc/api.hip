@@ -114,6 +114,7 @@ struct examd_handle {
                                   // of the chunk a run is replaying
   int64_t regrows = 0;            // chunks run again with a larger capacity
   double *host_stage = nullptr;   // pinned 3 n doubles, the same layout
+  hipEvent_t piece_ev[4] = {};    // the pieces of a download through the stage
 };
 
 namespace {
@@ -394,14 +395,29 @@ int upload_aos(examd_handle *h, const double *src) {
                         hipMemcpyHostToDevice, h->stream));
   return EXAMD_OK;
 }
+// The download goes in pieces, each followed by an event: the host copies
+// piece k out of the stage while the device still sends the later ones.
 int download_aos(examd_handle *h, double *dst) {
   const int64_t m = 3 * h->n;
-  HIPCHK(hipMemcpyAsync(h->host_stage, h->dev_stage, m * sizeof(double),
-                        hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  host_parallel_for(m, [&](int64_t i0, int64_t i1) {
-    std::memcpy(dst + i0, h->host_stage + i0, (size_t)(i1 - i0) * sizeof(double));
-  });
+  constexpr int kPieces = 4;
+  int64_t off[kPieces + 1];
+  for (int k = 0; k <= kPieces; ++k) off[k] = m * k / kPieces;
+  for (int k = 0; k < kPieces; ++k) {
+    if (!h->piece_ev[k])
+      HIPCHK(hipEventCreateWithFlags(&h->piece_ev[k], hipEventDisableTiming));
+    if (off[k + 1] > off[k])
+      HIPCHK(hipMemcpyAsync(h->host_stage + off[k], h->dev_stage + off[k],
+                            (off[k + 1] - off[k]) * sizeof(double),
+                            hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipEventRecord(h->piece_ev[k], h->stream));
+  }
+  for (int k = 0; k < kPieces; ++k) {
+    HIPCHK(hipEventSynchronize(h->piece_ev[k]));
+    host_parallel_for(off[k + 1] - off[k], [&](int64_t i0, int64_t i1) {
+      std::memcpy(dst + off[k] + i0, h->host_stage + off[k] + i0,
+                  (size_t)(i1 - i0) * sizeof(double));
+    });
+  }
   return EXAMD_OK;
 }
 
@@ -1310,6 +1326,8 @@ int examd_hip_destroy(examd_handle *h) {
   drop_graph(h);
   free_state(h);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->piece_ev)
+    if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return EXAMD_OK;
