@@ -1,12 +1,14 @@
 // kernels.hip -- gfx950 kernels of the midpoint-method LJ path.
 //
 //   k_step_bin   velocity-Verlet kick/drift/wrap fused with cell assignment
-//                (exaMD_identify_block, SRC/exaMD_module.f90:101-125)
-//   k_bin_insert the per-block atom lists kept from step to step: atoms
-//                that changed block take the first free slot of their new
-//                block's list (the slot they left is free); (FP64 path)
-//                k_block_order closes up each changed list and restores
-//                input order inside it (SRC/exaMD_input_formats.f90:80-88)
+//                (exaMD_identify_block, SRC/exaMD_module.f90:101-125) and
+//                with the upkeep of the per-block atom lists kept from step
+//                to step: an atom that changed block frees its slot and
+//                takes the first free slot of its new block's list
+//   k_bin_insert the lists built from scratch (after a kBinOnly pass);
+//                (FP64 path) k_block_order closes up each changed list and
+//                restores input order inside it
+//                (SRC/exaMD_input_formats.f90:80-88)
 //   k_forces     one workgroup per cell block: gathers the home blocks plus
 //                the rc/2 ghost shell into LDS (the halo exchange of
 //                SRC/exaMD_comms.f90:44-376 done on device), evaluates every
@@ -367,21 +369,33 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
     atomicOr(&flag_log->flags, (unsigned long long)flags);
 }
 
-// Atoms without a slot (all of them after a kBinOnly pass) put into the
-// first free slot of their block's list, kInsertPer atoms per thread, as
-// list_insert does for the atoms that change block in the other binning
-// modes.  A block without a free slot keeps its atoms and raises
-// kFlagBinOverflow.
+// Every atom into its block's list after a kBinOnly pass, which leaves the
+// lists empty and the counts zero: the atom takes the next slot of the list
+// from an atomic increment of the block's count.  kInsertPer atoms per
+// thread.  A block with more atoms than slots keeps the first ones in its
+// list and raises kFlagBinOverflow (whose recovery gives every list twice
+// the slots and bins again).
 __global__ __launch_bounds__(kBinThreads) void k_bin_insert(DevState d,
                                                             StepLog *flag_log) {
   const int64_t base =
       (int64_t)blockIdx.x * (kBinThreads * kInsertPer) + threadIdx.x;
+  const int S = 1 << d.bin_shift;
   unsigned flags = 0;
 #pragma unroll
   for (int k = 0; k < kInsertPer; ++k) {
     const int64_t i = base + (int64_t)k * kBinThreads;
-    if (i >= d.n || d.slot[i] >= 0) continue;
-    if (!list_insert(d, i, d.cell[i])) flags |= kFlagBinOverflow;
+    if (i >= d.n) continue;
+    const int c = d.cell[i];
+    const int s = atomicAdd(&d.count[c], 1);
+    if (s < S) {
+      const int64_t e = ((int64_t)c << d.bin_shift) + s;
+      d.sorted[e] = (int)i;
+      d.slot[i] = (int)e;
+      d.bflag[c] = 1;
+    } else {
+      d.slot[i] = -1;
+      flags |= kFlagBinOverflow;
+    }
   }
   flags = wave_or(flags);
   if ((threadIdx.x & 63) == 0 && flags)
@@ -466,7 +480,7 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int *s_wave,
 
 // Input order inside each block (the order the reference's linked list
 // gives, SRC/exaMD_input_formats.f90:80-88): each list that received atoms
-// in this step (flagged by k_bin_insert) closed up (holes removed, the tail
+// in this step (flagged where an atom took a slot) closed up (holes removed, the tail
 // left free) and put in input order by an insertion sort of its few ids,
 // the slots updated and the flag cleared.  A list that only lost atoms
 // keeps the rest in order around its holes.  Only the FP64 path needs it
