@@ -83,6 +83,7 @@ struct examd_handle {
   int nsys = 1;
   std::vector<SysHost> sys;
   long long *atom_seg = nullptr;  // device, nsys + 1
+  int *slab_sys = nullptr;        // device, system of each kSysSlab-atom slab (batches)
   DevTables tab[2];               // per precision
   int64_t nblocks = 0;            // all systems
   int64_t n = 0;                  // all systems
@@ -177,13 +178,14 @@ void free_state(examd_handle *h) {
   void *ptrs[] = {d.x,  d.y,  d.z,    d.vx,    d.vy,     d.vz,    d.fx,
                   d.fy, d.fz, d.qx,   d.qy,    d.qz,     d.cell,  d.slot,
                   d.count, d.sorted, d.bflag, d.home, d.part, h->log,
-                  h->atom_seg, d.type, d.pair32, d.pair64, d.pair_shift,
+                  h->atom_seg, h->slab_sys, d.type, d.pair32, d.pair64, d.pair_shift,
                   d.tab32, d.tab64};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DevState{};
   h->log = nullptr;
   h->atom_seg = nullptr;
+  h->slab_sys = nullptr;
   h->log_cap = 0;
   h->part_cap = 0;
   free_tables(h->tab[0]);
@@ -233,6 +235,7 @@ SysTable table(const examd_handle *h, int prec) {
   t.xcd_start = h->tab[prec].xcd_start;
   t.xcd_next = h->tab[prec].xcd_next;
   t.atom_seg = h->atom_seg;
+  t.slab_sys = h->slab_sys;
   t.cuts = h->tab[prec].cuts;
   t.nsys = h->nsys;
   t.ntiles = h->tab[prec].ntiles;
@@ -1138,6 +1141,8 @@ int create_handle(const examd_params *prm, const examd_system *systems,
   rc |= dalloc(&d.sorted, blocks << kMinBinShift) | dalloc(&d.count, blocks);
   rc |= dalloc(&d.bflag, bin_flag_bytes(blocks));
   rc |= dalloc(&h->atom_seg, nsys + 1);
+  const int64_t nslab = (std::max<int64_t>(n, 1) + kSysSlab - 1) / kSysSlab;
+  if (nsys > 1) rc |= dalloc(&h->slab_sys, nslab);
   if (rc == EXAMD_OK) rc = ensure_log(h, 1);
   if (rc != EXAMD_OK) {
     free_state(h);
@@ -1150,6 +1155,15 @@ int create_handle(const examd_params *prm, const examd_system *systems,
   seg[nsys] = (long long)atoms;
   hipError_t e = hipMemcpy(h->atom_seg, seg.data(), seg.size() * sizeof(long long),
                            hipMemcpyHostToDevice);
+  if (nsys > 1 && e == hipSuccess) {
+    std::vector<int> slab((size_t)nslab);
+    for (int64_t k = 0, q = 0; k < nslab; ++k) {
+      while (q + 1 < nsys && seg[q + 1] <= k * kSysSlab) ++q;
+      slab[k] = (int)q;
+    }
+    e = hipMemcpy(h->slab_sys, slab.data(), slab.size() * sizeof(int),
+                  hipMemcpyHostToDevice);
+  }
   launch_fill_i32(d.count, 0, blocks, h->stream);
   launch_fill_i32(reinterpret_cast<int *>(d.bflag), 0, bin_flag_bytes(blocks) / 4,
                   h->stream);

@@ -131,17 +131,6 @@ __global__ void k_fill_log(StepLog *p, int n) {
 // Atoms in input order, kBinPer per thread.  mode kFinishKickDriftBin closes
 // the previous step (v += h F, kinetic energy) and opens the next one; all
 // modes zero the force accumulators and assign the block.
-// System of atom i: the last k with atom_seg[k] <= i.
-__device__ __forceinline__ int system_of(const SysTable &t, int64_t i) {
-  int lo = 0, hi = t.nsys - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (t.atom_seg[mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // One atom of the binning pass: what it reads before any store (the
 // force as loaded: FP64, or the fixed-point sums converted at the store,
 // so that no load waits for another's conversion).
@@ -319,10 +308,12 @@ __global__ __launch_bounds__(kBinThreads) void k_step_bin(
       if constexpr (SINGLE) {
         c = bin_store<MIXED, MODE>(d, g1, b1, h, dt, i, at[k], ke[k], flags);
       } else {
-        // the system of the wave's first atom of this slab by binary
-        // search, then each lane steps forward (systems are contiguous and
-        // hold hundreds of atoms or more: rarely more than one step)
-        int sys = system_of(t, (int64_t)__builtin_amdgcn_readfirstlane((int)i));
+        // the system of the first atom of the wave's slab of kSysSlab atoms
+        // from the slab table, then each lane steps forward (systems are
+        // contiguous and hold hundreds of atoms or more: rarely more than
+        // one step)
+        const int64_t i0 = (int64_t)__builtin_amdgcn_readfirstlane((int)i);
+        int sys = t.slab_sys[i0 / kSysSlab];
         while (sys + 1 < t.nsys && t.atom_seg[sys + 1] <= i) ++sys;
         // a wave inside one system reads its grid as one (scalar) copy
         const int s0 = __builtin_amdgcn_readfirstlane(sys);

@@ -50,6 +50,7 @@ struct DevState {
 };
 
 constexpr int kProfSlots = 5;   // timestamps per cell block
+constexpr int kSysSlab = 256;   // atoms per entry of SysTable::slab_sys
 
 // Partial slots needed by a launch sequence over n atoms and G tiles.
 int64_t partial_slots(int64_t n, int64_t tiles);
@@ -67,6 +68,8 @@ struct SysTable {
   const int *xcd_start = nullptr;
   int *xcd_next = nullptr;
   const long long *atom_seg = nullptr; // first atom of each system (nsys + 1)
+  // system of the first atom of each slab of kSysSlab atoms (batches)
+  const int *slab_sys = nullptr;
   const int *cuts = nullptr;           // cell-block boundaries of balanced maps
   int nsys = 1;
   int ntiles = 0;
