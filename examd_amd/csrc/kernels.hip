@@ -469,14 +469,33 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int *s_wave,
   return r;
 }
 
+// Sorting network for eight values (Batcher's odd-even merge sort, 19
+// compare-exchanges), ascending.
+__device__ __forceinline__ void sort8(unsigned (&v)[8]) {
+  auto cx = [&](int a, int b) {
+    const unsigned lo = min(v[a], v[b]), hi = max(v[a], v[b]);
+    v[a] = lo;
+    v[b] = hi;
+  };
+  cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+  cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+  cx(1, 2); cx(5, 6);
+  cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+  cx(2, 4); cx(3, 5);
+  cx(1, 2); cx(3, 4); cx(5, 6);
+}
+
 // Input order inside each block (the order the reference's linked list
 // gives, SRC/exaMD_input_formats.f90:80-88): each list that received atoms
-// in this step (flagged where an atom took a slot) closed up (holes removed, the tail
-// left free) and put in input order by an insertion sort of its few ids,
-// the slots updated and the flag cleared.  A list that only lost atoms
-// keeps the rest in order around its holes.  Only the FP64 path needs it
-// (its sums depend on the order of the pairs); the mixed path's fixed-point
-// sums do not, and its lists keep their holes until insertions refill them.
+// in this step (flagged where an atom took a slot) closed up (holes removed,
+// the tail left free) and put in input order, the slots updated and the
+// flag cleared.  A list of eight slots is sorted in registers as unsigned
+// values, so that its free slots (-1) sort last behind the ascending ids;
+// a longer list (after an overflow) by an insertion sort in place.  A list
+// that only lost atoms keeps the rest in order around its holes.  Only the
+// FP64 path needs it (its sums depend on the order of the pairs); the mixed
+// path's fixed-point sums do not, and its lists keep their holes until
+// insertions refill them.
 __global__ void k_block_order(DevState d) {
   const int64_t w4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w4 * 4 >= d.nblocks) return;
@@ -490,6 +509,19 @@ __global__ void k_block_order(DevState d) {
     const int64_t b = w4 * 4 + q;
     const int base = (int)(b << d.bin_shift);
     int *p = d.sorted + base;
+    if (S == 8) {
+      int4 *p4 = reinterpret_cast<int4 *>(p);
+      const int4 lo = p4[0], hi = p4[1];
+      unsigned v[8] = {(unsigned)lo.x, (unsigned)lo.y, (unsigned)lo.z, (unsigned)lo.w,
+                       (unsigned)hi.x, (unsigned)hi.y, (unsigned)hi.z, (unsigned)hi.w};
+      sort8(v);
+      p4[0] = make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+      p4[1] = make_int4((int)v[4], (int)v[5], (int)v[6], (int)v[7]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if ((int)v[k] >= 0) d.slot[v[k]] = base + k;
+      continue;
+    }
     int c = 0, last = -1;
     for (int r = 0; r < S; ++r) {
       const int v = p[r];
