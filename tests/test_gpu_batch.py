@@ -141,7 +141,8 @@ def test_batch_block_ids_and_ownership(E, oracle):
 def test_persistent_grid_matches(E, batch):
     """A persistent force grid (workgroups looping over the cell blocks)
     gives the same mixed forces bit for bit and the same FP64 forces to
-    rounding."""
+    rounding.  Three workgroups run on three of the eight XCDs, so the cell
+    blocks of the other five XCDs' ranges are all taken from other ranges."""
     print(f"[persistent grid] batch={batch}", flush=True)
     if batch:
         sy = _systems(E)
@@ -154,12 +155,12 @@ def test_persistent_grid_matches(E, batch):
         cfg = {"tiles": (4, 4, 3)}
     with h:
         out = {}
-        for wgs in (0, 16, 24):
+        for wgs in (0, 3, 16, 24):
             for prec in ("mixed", "fp64"):
                 h.set_config(prec, block_threads=256, workgroups=wgs, **cfg)
                 assert h.get_config(prec)["workgroups"] == wgs
                 out[wgs, prec] = h.forces(prec)[0]
-    for wgs in (16, 24):
+    for wgs in (3, 16, 24):
         assert np.array_equal(out[wgs, "mixed"], out[0, "mixed"])
         assert _rel(out[wgs, "fp64"], out[0, "fp64"]) < 1e-13
 
