@@ -821,10 +821,27 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       A.d.prof ? A.d.prof + (size_t)tile * kProfSlots : nullptr;
   if (prof && tid == 0) prof[0] = clock64();
 
+  // Region coordinates of region block rb: x fastest.  The divisions by the
+  // region's extents are multiplications by ceil(2^32 / d) taken high,
+  // exact for n * d < 2^32: the maps keep region extents at most 255 per
+  // axis (the 8-bit packed coordinates), and NRB <= maxrb, the region blocks
+  // whose offsets fit the LDS, was checked above.
+  // (the constants by an FP64 division, exact to far below one for these
+  // divisors, 3 <= d <= 255, so that the ceiling is right and below 2^32)
+  const unsigned mRX = (unsigned)__builtin_ceil(0x1p32 / (double)RX);
+  const unsigned mRY = (unsigned)__builtin_ceil(0x1p32 / (double)RY);
+  auto region_coords = [&](int rb, int &cx, int &cy, int &cz) {
+    const int q = (int)__umulhi((unsigned)rb, mRX);
+    cx = rb - q * RX;
+    cz = (int)__umulhi((unsigned)q, mRY);
+    cy = q - cz * RY;
+  };
+
   // 1. region blocks: home blocks of the tile plus a one-block ghost shell,
   //    periodic (SRC/exaMD_partitioning.f90:788-790).
   for (int rb = tid; rb < NRB; rb += NT) {
-    int cx = rb % RX, cy = (rb / RX) % RY, cz = rb / RXY;
+    int cx, cy, cz;
+    region_coords(rb, cx, cy, cz);
     int gx = h0x - 1 + cx, gy = h0y - 1 + cy, gz = h0z - 1 + cz;
     gx += gx < 0 ? nbx : 0;
     gx -= gx >= nbx ? nbx : 0;
@@ -887,7 +904,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
   const float rc2pre =
       A.rc2f + (float)((hxr * hxr + hyr * hyr + hzr * hzr) * 0x1p-17);
   for (int rb = tid; rb < NRB; rb += NT) {
-    const int cx = rb % RX, cy = (rb / RX) % RY, cz = rb / RXY;
+    int cx, cy, cz;
+    region_coords(rb, cx, cy, cz);
     const int sx = (wlx && cx == 0) ? -1 : ((whx && cx == RX - 1) ? 1 : 0);
     const int sy = (wly && cy == 0) ? -1 : ((why && cy == RY - 1) ? 1 : 0);
     const int sz = (wlz && cz == 0) ? -1 : ((whz && cz == RZ - 1) ? 1 : 0);
