@@ -497,11 +497,27 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
   if (c.edge > 0)
     for (int k = 0; k < nsys; ++k)
       c.T[k] = tiles_from_edge(h->sys[k].g, (double)c.edge);
+  // the region atoms of every tile of the current maps in launch order
+  // (systems in order, x fastest), kept for the split of the launch over the
+  // XCDs; returns system k's largest
+  std::vector<int64_t> cost;
+  auto add_costs = [&](int k, const Grid &g, const Cuts &cb) {
+    const size_t t0 = cost.size();
+    cost.resize(t0 + (size_t)g.T[0] * g.T[1] * g.T[2]);
+    int64_t m = 0;
+    for_region_atoms(g, cb, cnt + h->sys[k].block0,
+                     [&](int tx, int ty, int tz, int64_t v) {
+                       cost[t0 + tx + (size_t)g.T[0] * (ty + (size_t)g.T[1] * tz)] = v;
+                       m = std::max(m, v);
+                     });
+    return m;
+  };
   for (int iter = 0; iter < 4096; ++iter) {
     bool ok = true;
     int64_t mra = 0;
     int worst = 0;
     c.maxrb = 0;
+    cost.clear();
     if (c.balance) c.cuts.resize((size_t)nsys);
     for (int k = 0; k < nsys; ++k) {
       const Grid g = grid_of(h, prec, k);
@@ -514,7 +530,7 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
         for (int t = 0; t < g.T[a]; ++t)
           if (cb[a][t + 1] - cb[a][t] + 2 > 255) ok = false;
       if (!ok) break;
-      const int64_t m = max_region_atoms(g, cb, cnt + S.block0);
+      const int64_t m = add_costs(k, g, cb);
       if (m > mra) {
         mra = m;
         worst = k;
@@ -591,11 +607,12 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
         // the launch's largest region after coarsening sets the capacity
         c.maxrb = 0;
         mra = 0;
+        cost.clear();
         for (int k = 0; k < nsys; ++k) {
           const Grid g = grid_of(h, prec, k);
           const Cuts cb = cuts_of(h, prec, k, g);
           c.maxrb = std::max(c.maxrb, max_region_blocks(cb));
-          mra = std::max(mra, max_region_atoms(g, cb, cnt + h->sys[k].block0));
+          mra = std::max(mra, add_costs(k, g, cb));
         }
         int64_t cap2 = std::max<int64_t>(cap, cap_of(mra));
         while (cap2 > std::max<int64_t>(cap, needed_cap(mra)) &&
@@ -604,21 +621,9 @@ int choose_caps_counts(examd_handle *h, int prec, const int *cnt,
           cap2 -= 16;
         c.cap = (int)cap2;
       }
-      // the tiles in launch order (systems in order, x fastest) split into
-      // eight contiguous ranges of about equal region atoms, one per XCD
-      {
-        std::vector<int64_t> cost;
-        for (int k = 0; k < nsys; ++k) {
-          const Grid g = grid_of(h, prec, k);
-          const size_t t0 = cost.size();
-          cost.resize(t0 + (size_t)g.T[0] * g.T[1] * g.T[2]);
-          for_region_atoms(g, cuts_of(h, prec, k, g), cnt + h->sys[k].block0,
-                           [&](int tx, int ty, int tz, int64_t v) {
-                             cost[t0 + tx + (size_t)g.T[0] * (ty + (size_t)g.T[1] * tz)] = v;
-                           });
-        }
-        c.xcd = equal_cost_ranges(cost, kXcds);
-      }
+      // the tiles in launch order split into eight contiguous ranges of
+      // about equal region atoms, one per XCD
+      c.xcd = equal_cost_ranges(cost, kXcds);
       // about three work items per lane: run groups from the mean region
       double reg_atoms = 0.0, reg_tiles = 0.0;
       for (int k = 0; k < nsys; ++k) {

@@ -32,16 +32,24 @@ inline int host_block(double x, double bl, int nb) {
 // tile at least one block wide.  cnt: the system's atoms per block.
 template <class G>
 Cuts balanced_cuts(const G &g, const int *cnt, int64_t n) {
-  std::vector<double> slab[3];
-  for (int a = 0; a < 3; ++a) slab[a].assign((size_t)g.nb[a], 0.0);
-  for (int z = 0, i = 0; z < g.nb[2]; ++z)
-    for (int y = 0; y < g.nb[1]; ++y)
-      for (int x = 0; x < g.nb[0]; ++x, ++i) {
-        const double c = cnt[i];
-        slab[0][x] += c;
-        slab[1][y] += c;
-        slab[2][z] += c;
+  // slab sums in integers (each row of x summed first, so that the y and z
+  // slabs take one addition per row), then as doubles
+  std::vector<int64_t> isl[3];
+  for (int a = 0; a < 3; ++a) isl[a].assign((size_t)g.nb[a], 0);
+  const int nx = g.nb[0];
+  for (int z = 0; z < g.nb[2]; ++z)
+    for (int y = 0; y < g.nb[1]; ++y) {
+      const int *row = cnt + (size_t)nx * (y + (size_t)g.nb[1] * z);
+      int64_t r = 0;
+      for (int x = 0; x < nx; ++x) {
+        isl[0][(size_t)x] += row[x];
+        r += row[x];
       }
+      isl[1][(size_t)y] += r;
+      isl[2][(size_t)z] += r;
+    }
+  std::vector<double> slab[3];
+  for (int a = 0; a < 3; ++a) slab[a].assign(isl[a].begin(), isl[a].end());
   Cuts b;
   for (int a = 0; a < 3; ++a) {
     const int nb = g.nb[a], T = g.T[a];
@@ -105,11 +113,19 @@ void for_region_atoms(const G &g, const Cuts &cb, const int *cnt, F f) {
   std::vector<int64_t> pre;
   // A[(z * ny + y) * Tx + tx]: x-region sums of row (y, z)
   std::vector<int64_t> A((size_t)Tx * ny * nz);
+  pre.resize((size_t)nx + 3);
   for (int z = 0; z < nz; ++z)
     for (int y = 0; y < ny; ++y) {
+      // the one pass over every block: the row's prefix sum in place
       const int *row = cnt + (size_t)nx * (y + (size_t)ny * z);
-      region_line(nx, cb[0], Tx, [&](int x) { return (int64_t)row[x]; }, pre,
-                  &A[((size_t)z * ny + y) * Tx], 1);
+      int64_t *q = pre.data();
+      q[0] = 0;
+      q[1] = row[nx - 1];
+      int64_t acc = q[1];
+      for (int i = 0; i < nx; ++i) q[i + 2] = (acc += row[i]);
+      q[nx + 2] = acc + row[0];
+      int64_t *out = &A[((size_t)z * ny + y) * Tx];
+      for (int t = 0; t < Tx; ++t) out[t] = q[cb[0][t + 1] + 2] - q[cb[0][t]];
     }
   // B[(z * Ty + ty) * Tx + tx]: then over y
   std::vector<int64_t> B((size_t)Tx * Ty * nz);
