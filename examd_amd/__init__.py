@@ -546,9 +546,10 @@ class MidpointLJBatch(_Handle):
         self._h = h
         self._info()
 
-    def forces(self, precision="fp64", allow=()):
-        """(forces (n, 3), [Stats per system], status)."""
-        return self._forces(precision, allow)
+    def forces(self, precision="fp64", allow=(), out=None):
+        """(forces (n, 3), [Stats per system], status); out: a C-contiguous
+        float64 (n, 3) array to write the forces into."""
+        return self._forces(precision, allow, out)
 
     def run(self, nsteps, dt, precision="mixed", energy_every=0):
         """Velocity-Verlet steps of every system; one list of per-system
