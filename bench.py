@@ -433,7 +433,8 @@ def table_bench(E, steps, warmup, tune_after=300, npoints=4096, r_lo=0.6):
     return {
         "workload": "LJ liquid 2,048,000 atoms (the headline system) with the "
                     f"LJ pair as a table of {npoints} points uniform in r^2 "
-                    f"from {r_lo} sigma to rc 2.5, linear in r^2, mixed "
+                    f"from {r_lo} sigma to rc 2.5, cubic Hermite pieces in "
+                    "r^2, mixed "
                     "precision, dt 0.005",
         "atoms": n, "steps": steps,
         "atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,

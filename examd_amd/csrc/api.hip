@@ -1535,17 +1535,50 @@ int examd_hip_set_pair_table(examd_handle *h, int32_t npoints, double r2_lo,
       if (std::fabs(g[k]) * std::sqrt(r2) >= 16384.0)
         d2fix = std::max(d2fix, r2_lo + (rc2 - r2_lo) * (k + 1) / (npoints - 1));
     }
+    // per interval the cubic (in t, the position between the points) of
+    // g and of v through the two points with the points' slopes: Hermite
+    // interpolation, exact for cubics in r^2; slopes per point spacing by
+    // the fourth-order differences (centred inside, one-sided at the two
+    // ends), exact for quartics.  Fewer than five points: linear.
     const int ni = npoints - 1;
-    t32.resize(ni);
-    t64.resize(4 * (size_t)ni);
+    auto slopes = [&](const double *y) {
+      std::vector<double> m((size_t)npoints, 0.0);
+      const int n = npoints;
+      if (n < 5) return m;   // unused: the intervals are linear
+      for (int k = 2; k < n - 2; ++k)
+        m[k] = (y[k - 2] - 8.0 * y[k - 1] + 8.0 * y[k + 1] - y[k + 2]) / 12.0;
+      m[0] = (-25.0 * y[0] + 48.0 * y[1] - 36.0 * y[2] + 16.0 * y[3] - 3.0 * y[4]) / 12.0;
+      m[1] = (-3.0 * y[0] - 10.0 * y[1] + 18.0 * y[2] - 6.0 * y[3] + y[4]) / 12.0;
+      m[n - 2] = (-y[n - 5] + 6.0 * y[n - 4] - 18.0 * y[n - 3] + 10.0 * y[n - 2] +
+                  3.0 * y[n - 1]) / 12.0;
+      m[n - 1] = (3.0 * y[n - 5] - 16.0 * y[n - 4] + 36.0 * y[n - 3] - 48.0 * y[n - 2] +
+                  25.0 * y[n - 1]) / 12.0;
+      return m;
+    };
+    auto cubic = [&](const double *y, const std::vector<double> &m, int k, double c[4]) {
+      if (npoints < 5) {
+        c[0] = y[k];
+        c[1] = y[k + 1] - y[k];
+        c[2] = c[3] = 0.0;
+        return;
+      }
+      c[0] = y[k];
+      c[1] = m[k];
+      c[2] = 3.0 * (y[k + 1] - y[k]) - 2.0 * m[k] - m[k + 1];
+      c[3] = 2.0 * (y[k] - y[k + 1]) + m[k] + m[k + 1];
+    };
+    const std::vector<double> mg = slopes(g), mv = slopes(v);
+    t32.resize(2 * (size_t)ni);
+    t64.resize(8 * (size_t)ni);
+    constexpr double kS = 4294967296.0;   // the mixed path's 2^32 scale
     for (int k = 0; k < ni; ++k) {
-      const double dg = g[k + 1] - g[k], dv = v[k + 1] - v[k];
-      t64[4 * k + 0] = g[k];
-      t64[4 * k + 1] = dg;
-      t64[4 * k + 2] = v[k];
-      t64[4 * k + 3] = dv;
-      t32[k] = make_float4((float)(g[k] * 4294967296.0), (float)(dg * 4294967296.0),
-                           (float)(v[k] * 4294967296.0), (float)(dv * 4294967296.0));
+      double *cg = &t64[8 * (size_t)k], *cv = cg + 4;
+      cubic(g, mg, k, cg);
+      cubic(v, mv, k, cv);
+      t32[2 * (size_t)k] = make_float4((float)(cg[0] * kS), (float)(cg[1] * kS),
+                                       (float)(cg[2] * kS), (float)(cg[3] * kS));
+      t32[2 * (size_t)k + 1] = make_float4((float)(cv[0] * kS), (float)(cv[1] * kS),
+                                           (float)(cv[2] * kS), (float)(cv[3] * kS));
     }
   }
   HIPCHK(hipSetDevice(h->dev));
@@ -1558,7 +1591,7 @@ int examd_hip_set_pair_table(examd_handle *h, int32_t npoints, double r2_lo,
   d.tab_n = 0;
   h->forces_valid = -1;
   if (npoints == 0) return EXAMD_OK;
-  if (dalloc(&d.tab32, npoints - 1) || dalloc(&d.tab64, 4 * (npoints - 1))) {
+  if (dalloc(&d.tab32, 2 * (npoints - 1)) || dalloc(&d.tab64, 8 * (npoints - 1))) {
     if (d.tab32) (void)hipFree(d.tab32);
     d.tab32 = nullptr;
     return EXAMD_ERR_NOMEM;

@@ -1154,8 +1154,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       if constexpr (ENERGY) pefix += fix32_scaled(P.z * s6 * (s6 - 1.0f));
       d2min32 = min(d2min32, __float_as_uint(d2));
     };
-    // a tabulated pair function, linear in r^2 between its points (the
-    // pair force f12 = g d as the reference's user function returns it)
+    // a tabulated pair function, a cubic in r^2 on each interval between its
+    // points (Hermite, from the points' values and slopes), evaluated by
+    // Horner's rule in t across the interval (the pair force f12 = g d as
+    // the reference's user function returns it)
     auto pair64tab = [&](int j, double dx, double dy, double dz, double d2) {
       double u = (d2 - A.d.tab_lo) * A.d.tab_inv_h;
       if (u < 0.0) {
@@ -1165,8 +1167,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       int k = (int)u;
       if (k > A.d.tab_n - 2) k = A.d.tab_n - 2;
       const double t = u - (double)k;
-      const double *e = A.d.tab64 + 4 * k;
-      const double g = e[0] + t * e[1];
+      const double *e = A.d.tab64 + 8 * k;
+      const double g = ((e[3] * t + e[2]) * t + e[1]) * t + e[0];
       const double f1 = g * dx, f2 = g * dy, f3 = g * dz;
       fax += f1;
       fay += f2;
@@ -1176,7 +1178,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       lds_add(fj, -f1);
       lds_add(fj + 1, -f2);
       lds_add(fj + 2, -f3);
-      if constexpr (ENERGY) pe64 += e[2] + t * e[3];
+      if constexpr (ENERGY) pe64 += ((e[7] * t + e[6]) * t + e[5]) * t + e[4];
       if (d2 < A.d2zero) flags |= kFlagZero;
     };
     auto pair32tab = [&](int j, float dx, float dy, float dz, float d2) {
@@ -1187,8 +1189,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       }
       const int k = min((int)u, A.d.tab_n - 2);
       const float t = u - (float)k;
-      const float4 e = A.d.tab32[k];   // g, dg, V, dV, times 2^32
-      const double gs = (double)__builtin_fmaf(t, e.y, e.x);
+      // g's cubic, times 2^32 (the V cubic follows it)
+      const float4 e = A.d.tab32[2 * k];
+      const double gs = (double)__builtin_fmaf(
+          __builtin_fmaf(__builtin_fmaf(e.w, t, e.z), t, e.y), t, e.x);
       const long long q1 = fix32_prod_raw(gs, dx);
       const long long q2 = fix32_prod_raw(gs, dy);
       const long long q3 = fix32_prod_raw(gs, dz);
@@ -1200,7 +1204,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
       lds_sub(qj, q1);
       lds_sub(qj + 1, q2);
       lds_sub(qj + 2, q3);
-      if constexpr (ENERGY) pefix += fix32_scaled(__builtin_fmaf(t, e.w, e.z));
+      if constexpr (ENERGY) {
+        const float4 ev = A.d.tab32[2 * k + 1];
+        pefix += fix32_scaled(__builtin_fmaf(
+            __builtin_fmaf(__builtin_fmaf(ev.w, t, ev.z), t, ev.y), t, ev.x));
+      }
       d2min32 = min(d2min32, __float_as_uint(d2));
     };
     auto home_count = [&](int j) {

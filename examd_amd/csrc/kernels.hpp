@@ -39,8 +39,10 @@ struct DevState {
   int ntypes = 0;
   double max_eps_sig12 = 0.0; // largest eps sig^12 of the table
   // tabulated pair function (tab_n > 0): tab_n - 1 intervals uniform in r^2
-  // from tab_lo; per interval (g, dg, V, dV) with f12 = g * d, the mixed
-  // copy scaled by 2^32
+  // from tab_lo; per interval the cubics c0 + c1 t + c2 t^2 + c3 t^3 (t in
+  // [0, 1] across the interval) of g, with f12 = g * d, and of V: FP64 as
+  // (g c0..c3, V c0..c3), the mixed copy as two float4 (g, then V) scaled
+  // by 2^32
   float4 *tab32 = nullptr;
   double *tab64 = nullptr;
   int tab_n = 0;

@@ -128,8 +128,8 @@ int or_lennardjones_12(const or_lj *p, double dx, double dy, double dz,
     return 0;
 }
 
-/* A tabulated pair function at d2 (within the cutoff): the linear
- * interpolation of or_table, the result in the form of user_func. */
+/* A tabulated pair function at d2 (within the cutoff): the interval's
+ * cubic of or_table, the result in the form of user_func. */
 static int or_table_pair(const or_table *tab, double rc, double dx, double dy,
                          double dz, double d2, double f12[3], double *e12,
                          int64_t *below)
@@ -143,11 +143,12 @@ static int or_table_pair(const or_table *tab, double rc, double dx, double dy,
     int32_t k = (int32_t)u;
     if (k > tab->n - 2) k = tab->n - 2;
     double tt = u - (double)k;
-    double g = tab->g[k] + tt * tab->dg[k];
+    const double *cg = tab->gc + 4 * (int64_t)k, *cv = tab->vc + 4 * (int64_t)k;
+    double g = ((cg[3] * tt + cg[2]) * tt + cg[1]) * tt + cg[0];
     f12[0] = g * dx;
     f12[1] = g * dy;
     f12[2] = g * dz;
-    *e12 = tab->v[k] + tt * tab->dv[k];
+    *e12 = ((cv[3] * tt + cv[2]) * tt + cv[1]) * tt + cv[0];
     return 0;
 }
 

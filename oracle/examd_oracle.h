@@ -46,17 +46,20 @@ typedef struct or_species {
     const or_lj *pair;
 } or_species;
 
-/* A tabulated pair function: g(r^2) and V(r^2) at n >= 2 points uniform in
- * r^2 from r2_lo to r2_lo + (n - 1) h (the cutoff squared), g the factor
- * of the pair force f12 = g * (dx, dy, dz) and V the pair energy, as the
+/* A tabulated pair function: n points uniform in r^2 from r2_lo to the
+ * cutoff squared (inv_h = (n - 1) / (rc^2 - r2_lo)), g the factor of the
+ * pair force f12 = g * (dx, dy, dz) and V the pair energy, as the
  * reference's user function returns them (user_func.f90:40-75).  Between
  * points: u = (d2 - r2_lo) * inv_h, k = min(floor(u), n - 2), t = u - k,
- * g = g[k] + t * dg[k] with dg[k] = g[k+1] - g[k] (V likewise).  A pair
- * closer than r2_lo is counted in `below` and uses the first point. */
+ * g = ((gc[4k+3] t + gc[4k+2]) t + gc[4k+1]) t + gc[4k] (V likewise from
+ * vc): per interval the cubic Hermite interpolant of the two points' values
+ * and slopes (the caller builds the coefficients; linear below five
+ * points).  A pair closer than r2_lo is counted in `below` and uses the
+ * first point. */
 typedef struct or_table {
     int32_t n;
     double r2_lo, inv_h;
-    const double *g, *dg, *v, *dv;   /* n - 1 intervals each (g, v: n) */
+    const double *gc, *vc;   /* 4 (n - 1) coefficients each */
 } or_table;
 
 /* Per-evaluation tallies.  cand = candidate pairs visited by the midpoint

@@ -36,8 +36,7 @@ class _Species(C.Structure):
 
 class _Table(C.Structure):
     _fields_ = [("n", C.c_int32), ("r2_lo", C.c_double), ("inv_h", C.c_double),
-                ("g", C.POINTER(C.c_double)), ("dg", C.POINTER(C.c_double)),
-                ("v", C.POINTER(C.c_double)), ("dv", C.POINTER(C.c_double))]
+                ("gc", C.POINTER(C.c_double)), ("vc", C.POINTER(C.c_double))]
 
 
 class _Tally(C.Structure):
@@ -187,16 +186,44 @@ def _species(species, n):
     return sp, (types, table)
 
 
+def hermite_coefficients(y):
+    """Per interval of the samples y (uniform spacing), the coefficients
+    (c0, c1, c2, c3) of the cubic in t in [0, 1] through the two points with
+    the points' slopes per spacing: fourth-order differences (centred
+    inside, one-sided at the two ends), exact for quartics, so that the
+    interpolant is exact for cubics.  Fewer than five points: the linear
+    interpolant.  Shape (len(y) - 1, 4)."""
+    y = np.asarray(y, dtype=np.float64)
+    n = len(y)
+    c = np.zeros((n - 1, 4))
+    c[:, 0] = y[:-1]
+    if n < 5:
+        c[:, 1] = y[1:] - y[:-1]
+        return c
+    m = np.empty(n)
+    m[2:n - 2] = (y[:n - 4] - 8 * y[1:n - 3] + 8 * y[3:n - 1] - y[4:]) / 12
+    m[0] = (-25 * y[0] + 48 * y[1] - 36 * y[2] + 16 * y[3] - 3 * y[4]) / 12
+    m[1] = (-3 * y[0] - 10 * y[1] + 18 * y[2] - 6 * y[3] + y[4]) / 12
+    m[n - 2] = (-y[n - 5] + 6 * y[n - 4] - 18 * y[n - 3] + 10 * y[n - 2]
+                + 3 * y[n - 1]) / 12
+    m[n - 1] = (3 * y[n - 5] - 16 * y[n - 4] + 36 * y[n - 3] - 48 * y[n - 2]
+                + 25 * y[n - 1]) / 12
+    c[:, 1] = m[:-1]
+    c[:, 2] = 3 * (y[1:] - y[:-1]) - 2 * m[:-1] - m[1:]
+    c[:, 3] = 2 * (y[:-1] - y[1:]) + m[:-1] + m[1:]
+    return c
+
+
 def table_arrays(table, cutoff):
     """(r2_lo, g, v) sampled at len(g) points uniform in r^2 up to cutoff^2
-    -> (n, r2_lo, inv_h, g, dg, v, dv) as the interpolation uses them."""
+    -> (n, r2_lo, inv_h, gc, vc): the cubic coefficients of g and V per
+    interval, as the interpolation uses them."""
     r2_lo, g, v = table
-    g = np.ascontiguousarray(g, dtype=np.float64)
-    v = np.ascontiguousarray(v, dtype=np.float64)
     n = len(g)
     inv_h = float(n - 1) / (cutoff * cutoff - float(r2_lo))
-    return n, float(r2_lo), inv_h, g, np.ascontiguousarray(g[1:] - g[:-1]), \
-        v, np.ascontiguousarray(v[1:] - v[:-1])
+    gc = np.ascontiguousarray(hermite_coefficients(g).reshape(-1))
+    vc = np.ascontiguousarray(hermite_coefficients(v).reshape(-1))
+    return n, float(r2_lo), inv_h, gc, vc
 
 
 def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
@@ -216,8 +243,8 @@ def forces(pos, box, cutoff, eps=1.0, sig=1.0, home_counts=False,
     t = _Tally()
     p = _LJ(eps, sig, cutoff)
     if table is not None:
-        tn, lo, ih, g, dg, v, dv = table_arrays(table, cutoff)
-        tab = _Table(tn, lo, ih, _dp(g), _dp(dg), _dp(v), _dp(dv))
+        tn, lo, ih, gc, vc = table_arrays(table, cutoff)
+        tab = _Table(tn, lo, ih, _dp(gc), _dp(vc))
         rc = lib().or_forces_midpoint_table(
             C.byref(dom._c), C.byref(p), C.byref(tab), n, _dp(pos), _dp(f),
             _ip(hc) if hc is not None else None, C.byref(t))

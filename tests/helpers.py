@@ -199,13 +199,46 @@ def lj_table(npoints, r_lo, rc, eps=1.0, sig=1.0):
     return r_lo * r_lo, g, v
 
 
+def _quartic_slopes(y):
+    """Slope per spacing at every sample: the derivative of the quartic
+    through the five nearest samples (centred, or the first or last five at
+    the ends)."""
+    n = len(y)
+    m = np.zeros(n)
+    for k in range(n):
+        a = min(max(k - 2, 0), n - 5)
+        c = np.polyfit(np.arange(5.0) - (k - a), y[a:a + 5], 4)
+        m[k] = np.polyval(np.polyder(c), 0.0)
+    return m
+
+
+def table_interpolate(y, u, m=None):
+    """A table's value at fractional index u: on each interval the cubic
+    Hermite interpolant of the two samples and their slopes m (the Hermite
+    basis functions; m from _quartic_slopes when not given), or the linear
+    one below five samples."""
+    y = np.asarray(y, dtype=np.float64)
+    n = len(y)
+    k = np.minimum(np.asarray(u).astype(np.int64), n - 2)
+    t = u - k
+    if n < 5:
+        return y[k] + t * (y[k + 1] - y[k])
+    if m is None:
+        m = _quartic_slopes(y)
+    h00, h10 = 2 * t ** 3 - 3 * t ** 2 + 1, t ** 3 - 2 * t ** 2 + t
+    h01, h11 = -2 * t ** 3 + 3 * t ** 2, t ** 3 - t ** 2
+    return h00 * y[k] + h10 * m[k] + h01 * y[k + 1] + h11 * m[k + 1]
+
+
 def table_brute_force(pos, box, rc, table):
-    """All-pairs minimum-image sum of a tabulated pair function, linear in
-    r^2 between the points: (forces, pe, n_cut)."""
+    """All-pairs minimum-image sum of a tabulated pair function, cubic
+    Hermite in r^2 between the points: (forces, pe, n_cut)."""
     r2_lo, g, v = table
-    g, v = np.asarray(g), np.asarray(v)
+    g, v = np.asarray(g, dtype=np.float64), np.asarray(v, dtype=np.float64)
     n_pts = len(g)
     inv_h = (n_pts - 1) / (rc * rc - r2_lo)
+    mg = _quartic_slopes(g) if n_pts >= 5 else None
+    mv = _quartic_slopes(v) if n_pts >= 5 else None
     pos = np.asarray(pos, dtype=np.float64)
     box = np.asarray(box, dtype=np.float64).reshape(3)
     n = len(pos)
@@ -219,11 +252,9 @@ def table_brute_force(pos, box, rc, table):
         r2[i] = np.inf
         m = r2 <= rc * rc
         u = np.clip((r2[m] - r2_lo) * inv_h, 0.0, None)
-        k = np.minimum(u.astype(np.int64), n_pts - 2)
-        t = u - k
-        gg = g[k] + t * (g[k + 1] - g[k])
+        gg = table_interpolate(g, u, mg)
         f[i] += (gg[:, None] * d[m]).sum(0)
-        pe += 0.5 * (v[k] + t * (v[k + 1] - v[k])).sum()
+        pe += 0.5 * table_interpolate(v, u, mv).sum()
         ncut += int(m.sum())
     return f, pe, ncut // 2
 

@@ -1,12 +1,13 @@
 """A tabulated pair function on the GPU (examd_hip_set_pair_table): the
 general user function of exaMD_forces_midpoint_all sampled on a grid in r^2
-and interpolated linearly, against the oracle's evaluation of the same
-table.
+and interpolated by cubic Hermite pieces in r^2, against the oracle's
+evaluation of the same table.
 
 Bars: FP64 forces within 1e-10 relative of the oracle's table evaluation
 (the same interpolation), pair counts exact, energy within 1e-10; the mixed
 path within its FP32 bars; the table of the LJ pair within the
-interpolation error of the analytic LJ evaluation.
+interpolation error of the analytic LJ evaluation; a pair function cubic in
+r^2 reproduced exactly (FP64 to rounding).
 """
 import numpy as np
 import pytest
@@ -93,8 +94,31 @@ def test_table_of_lj_close_to_lj_and_cleared(E, oracle):
         m.set_pair_table()
         f1, s1, _ = m.forces("mixed")
     assert _rel(f64, ref) < 1.01 * bound + 1e-12
-    assert bound < 1e-4
+    assert bound < 1e-8
     assert np.array_equal(f0, f1) and s0.pe == s1.pe
+
+
+def test_table_cubic_in_r2_is_exact(E):
+    """g and V cubic in r^2 come out exactly from seven points: two atoms at
+    several distances give the closed form, FP64 to rounding and mixed
+    within FP32 rounding of the coefficients."""
+    print("[table cubic]", flush=True)
+    r2 = np.linspace(0.25, 6.25, 7)
+    gp = lambda x: 3.0 - 0.5 * x + 0.25 * x * x - 0.03 * x ** 3
+    vp = lambda x: 1.0 + 2.0 * x - 0.4 * x * x + 0.02 * x ** 3
+    box = np.array([13.0, 13.0, 13.0])
+    for r in (0.7, 1.3, 2.2, 2.5):
+        pos = np.array([[4.0, 6.0, 6.0], [4.0 + r, 6.0, 6.0]])
+        with E.MidpointLJ(len(pos), box, 2.5) as m:
+            m.set_state(pos)
+            m.set_pair_table(0.25, gp(r2), vp(r2))
+            f64, s64, _ = m.forces("fp64")
+            f32, s32, _ = m.forces("mixed")
+        want = gp(r * r) * r
+        assert abs(f64[0, 0] - want) < 1e-12 and abs(f64[1, 0] + want) < 1e-12, r
+        assert abs(s64.pe - vp(r * r)) < 1e-12, r
+        assert abs(f32[0, 0] - want) < 1e-6 * max(1.0, abs(want)), r
+        assert abs(s32.pe - vp(r * r)) < 1e-6 * max(1.0, abs(vp(r * r))), r
 
 
 def test_table_range_and_arguments(E):

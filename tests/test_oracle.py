@@ -368,7 +368,7 @@ def test_table_forces_match_all_pairs(oracle):
 
 def test_table_of_lj_approaches_lj(oracle):
     """The LJ pair tabulated on a fine grid reproduces the analytic
-    evaluation within the linear-interpolation bound, which falls as h^2."""
+    evaluation within the cubic interpolation's bound, which falls as h^4."""
     pos, L = H.perturbed_fcc(6, 0.8442, 0.1, 32)
     ref, t0, _ = oracle.forces(pos, [L] * 3, 2.5)
     scale = np.sqrt((ref ** 2).sum(1).mean())
@@ -377,25 +377,41 @@ def test_table_of_lj_approaches_lj(oracle):
         f, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=H.lj_table(npts, 0.8, 2.5))
         errs.append(np.abs(f - ref).max() / scale)
         assert t.cut == t0.cut
-    assert errs[1] < 1e-4
-    assert 10 < errs[0] / errs[1] < 22            # second order: about 16x
+    assert errs[1] < 1e-9
+    assert 180 < errs[0] / errs[1] < 360          # fourth order: about 256x
 
 
-def test_table_linear_in_r2_is_exact(oracle):
-    """g and V linear in r^2 are interpolated exactly: two atoms at r give
-    the closed form, and a pair closer than the table is counted."""
+def test_table_cubic_in_r2_is_exact(oracle):
+    """g and V cubic in r^2 are interpolated exactly (the slopes of the
+    fourth-order differences are exact for them): two atoms at r give the
+    closed form, and a pair closer than the table is counted."""
     L = 13.0
     r2 = np.linspace(0.25, 6.25, 7)
-    g, v = 3.0 - 0.5 * r2, 1.0 + 2.0 * r2
-    for r in (0.7, 1.3, 2.2):
+    gp = lambda x: 3.0 - 0.5 * x + 0.25 * x * x - 0.03 * x ** 3
+    vp = lambda x: 1.0 + 2.0 * x - 0.4 * x * x + 0.02 * x ** 3
+    g, v = gp(r2), vp(r2)
+    for r in (0.7, 1.3, 2.2, 2.5):
         pos = np.array([[4.0, 6.0, 6.0], [4.0 + r, 6.0, 6.0]])
         f, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=(0.25, g, v))
-        want = (3.0 - 0.5 * r * r) * r
-        assert abs(f[0, 0] - want) < 1e-13 and abs(f[1, 0] + want) < 1e-13
-        assert abs(t.pe - (1.0 + 2.0 * r * r)) < 1e-13
+        want = gp(r * r) * r
+        assert abs(f[0, 0] - want) < 1e-12 and abs(f[1, 0] + want) < 1e-12
+        assert abs(t.pe - vp(r * r)) < 1e-12
     pos = np.array([[4.0, 6.0, 6.0], [4.3, 6.0, 6.0]])    # r^2 = 0.09 < 0.25
     _, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=(0.25, g, v))
     assert t.below == 1
+
+
+def test_short_table_is_linear(oracle):
+    """Below five points a table is interpolated linearly: g and V linear in
+    r^2 on three points come out exactly."""
+    L = 13.0
+    r2 = np.linspace(0.25, 6.25, 3)
+    g, v = 3.0 - 0.5 * r2, 1.0 + 2.0 * r2
+    for r in (0.7, 1.9):
+        pos = np.array([[4.0, 6.0, 6.0], [4.0 + r, 6.0, 6.0]])
+        f, t, _ = oracle.forces(pos, [L] * 3, 2.5, table=(0.25, g, v))
+        assert abs(f[0, 0] - (3.0 - 0.5 * r * r) * r) < 1e-13
+        assert abs(t.pe - (1.0 + 2.0 * r * r)) < 1e-13
 
 
 def test_golden_kats(oracle):
