@@ -172,9 +172,11 @@ int examd_hip_set_species(examd_handle *h, int32_t ntypes,
  * call, sampled by the host.  g[k] and v[k] are the force factor (the pair
  * force is f12 = g * (dx, dy, dz), as user_func returns it) and the pair
  * energy at r^2 = r2_lo + k h, k = 0 .. npoints - 1, h = (cutoff^2 - r2_lo) /
- * (npoints - 1), with the handle's cutoff.  Between points both are linear
- * in r^2; a pair closer than sqrt(r2_lo) uses the first point and the call
- * returns EXAMD_ERR_TABLE_RANGE.  A user function's retval = 1 (skip) is
+ * (npoints - 1), with the handle's cutoff.  Between points both are cubic
+ * in r^2: on each interval the Hermite cubic through the two points with
+ * slopes from fourth-order differences of the samples, exact for cubics in
+ * r^2 (linear below five points); a pair closer than sqrt(r2_lo) uses the
+ * first point and the call returns EXAMD_ERR_TABLE_RANGE.  A user function's retval = 1 (skip) is
  * g = v = 0.  2 <= npoints <= EXAMD_MAX_TABLE_POINTS, 0 <= r2_lo <
  * cutoff^2; npoints = 0 returns to the Lennard-Jones pair.  Not combined
  * with several species.  The energy shift of pe_shifted is v[npoints - 1]. */
