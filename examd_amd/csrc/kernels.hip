@@ -491,11 +491,13 @@ __device__ __forceinline__ void sort8(unsigned (&v)[8]) {
 // the tail left free) and put in input order, the slots updated and the
 // flag cleared.  A list of eight slots is sorted in registers as unsigned
 // values, so that its free slots (-1) sort last behind the ascending ids;
-// a longer list (after an overflow) by an insertion sort in place.  A list
-// that only lost atoms keeps the rest in order around its holes.  Only the
-// FP64 path needs it (its sums depend on the order of the pairs); the mixed
-// path's fixed-point sums do not, and its lists keep their holes until
-// insertions refill them.
+// a longer list by an insertion sort in place.  A list that only lost
+// atoms keeps the rest in order around its holes.  Only the FP64 path needs
+// the order (its sums depend on the order of the pairs), and only for lists
+// longer than eight slots (after an overflow): the force kernel's gather
+// sorts an 8-slot list itself.  The mixed path's fixed-point sums do not
+// depend on it, and its lists keep their holes until insertions refill
+// them.
 __global__ void k_block_order(DevState d) {
   const int64_t w4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w4 * 4 >= d.nblocks) return;
@@ -922,7 +924,22 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     for (int s0 = 0; w < c && s0 < S; s0 += 8) {
       const int4 q0 = *reinterpret_cast<const int4 *>(src + s0);
       const int4 q1 = *reinterpret_cast<const int4 *>(src + s0 + 4);
-      const int ids[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+      int ids[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+      if constexpr (FP64) {
+        // the FP64 sums follow the order of the atoms in a block, which is
+        // input order (SRC/exaMD_input_formats.f90:80-88): an 8-slot list
+        // is sorted here, as unsigned values so that free slots (-1) come
+        // last; a longer list (after an overflow) is kept in input order
+        // by k_block_order
+        if (S == 8) {
+          unsigned u[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) u[k] = (unsigned)ids[k];
+          sort8(u);
+#pragma unroll
+          for (int k = 0; k < 8; ++k) ids[k] = (int)u[k];
+        }
+      }
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (ids[k] >= 0 && w < c) {
