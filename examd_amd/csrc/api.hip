@@ -848,7 +848,7 @@ void evaluate(examd_handle *h, int prec, bool energy, bool homecount,
               StepLog *log) {
   launch_step_bin(h->d, table(h, prec), prec == EXAMD_PREC_MIXED, kBinOnly,
                   0.0, 0.0, nullptr, log, h->stream);
-  launch_bin_update(h->d, true, prec == EXAMD_PREC_FP64 && h->d.bin_shift > 3, log, h->stream);
+  launch_bin_update(h->d, true, prec == EXAMD_PREC_FP64 && h->d.bin_shift > 4, log, h->stream);
   forces_timed(h, prec, energy, homecount, log);
 }
 
@@ -890,7 +890,7 @@ void enqueue_steps(examd_handle *h, int prec, int64_t count, double dt,
       launch_step_bin(h->d, t, mixed, kKickDriftBin, hk, dt, nullptr, Lc,
                       h->stream);
     }
-    if (!mixed && h->d.bin_shift > 3) launch_bin_update(h->d, false, true, Lc, h->stream);
+    if (!mixed && h->d.bin_shift > 4) launch_bin_update(h->d, false, true, Lc, h->stream);
     // with timing on, every timing_every-th step's force launch is
     // bracketed by events (mid-stride, away from the tally steps)
     const int k = h->timing_every;
@@ -1787,7 +1787,7 @@ int examd_hip_forces_static(const examd_params *prm, int precision,
   launch_fill_log(log, 1, s);
   // a kBinOnly pass clears the counts and flags itself
   launch_step_bin(d, t, mixed, kBinOnly, 0.0, 0.0, nullptr, log, s);
-  launch_bin_update(d, true, !mixed && d.bin_shift > 3, log, s);
+  launch_bin_update(d, true, !mixed && d.bin_shift > 4, log, s);
   launch_forces(d, t, p, cfg, mixed, true, false, log, s);
   launch_forces_to_fp64(d, forces, mixed, s);
   HIPCHK(hipGetLastError());

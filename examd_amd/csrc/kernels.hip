@@ -485,19 +485,45 @@ __device__ __forceinline__ void sort8(unsigned (&v)[8]) {
   cx(1, 2); cx(3, 4); cx(5, 6);
 }
 
+// Sixteen values, ascending: both halves sorted by sort8, the second
+// reversed behind the first (a bitonic sequence), then merged by four
+// stages of eight compare-exchanges.
+__device__ __forceinline__ void sort16(unsigned (&v)[16]) {
+  unsigned a[8], b[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = v[k];
+    b[k] = v[8 + k];
+  }
+  sort8(a);
+  sort8(b);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = a[k];
+    v[15 - k] = b[k];
+  }
+#pragma unroll
+  for (int j = 8; j >= 1; j >>= 1)
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if (!(i & j)) {
+        const unsigned lo = min(v[i], v[i + j]), hi = max(v[i], v[i + j]);
+        v[i] = lo;
+        v[i + j] = hi;
+      }
+}
+
 // Input order inside each block (the order the reference's linked list
 // gives, SRC/exaMD_input_formats.f90:80-88): each list that received atoms
 // in this step (flagged where an atom took a slot) closed up (holes removed,
 // the tail left free) and put in input order, the slots updated and the
-// flag cleared.  A list of eight slots is sorted in registers as unsigned
-// values, so that its free slots (-1) sort last behind the ascending ids;
-// a longer list by an insertion sort in place.  A list that only lost
+// flag cleared, by an insertion sort in place.  A list that only lost
 // atoms keeps the rest in order around its holes.  Only the FP64 path needs
 // the order (its sums depend on the order of the pairs), and only for lists
-// longer than eight slots (after an overflow): the force kernel's gather
-// sorts an 8-slot list itself.  The mixed path's fixed-point sums do not
-// depend on it, and its lists keep their holes until insertions refill
-// them.
+// longer than sixteen slots (after repeated overflows): the force kernel's
+// gather sorts a list of 8 or 16 slots itself.  The mixed path's
+// fixed-point sums do not depend on it, and its lists keep their holes
+// until insertions refill them.
 __global__ void k_block_order(DevState d) {
   const int64_t w4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w4 * 4 >= d.nblocks) return;
@@ -511,19 +537,6 @@ __global__ void k_block_order(DevState d) {
     const int64_t b = w4 * 4 + q;
     const int base = (int)(b << d.bin_shift);
     int *p = d.sorted + base;
-    if (S == 8) {
-      int4 *p4 = reinterpret_cast<int4 *>(p);
-      const int4 lo = p4[0], hi = p4[1];
-      unsigned v[8] = {(unsigned)lo.x, (unsigned)lo.y, (unsigned)lo.z, (unsigned)lo.w,
-                       (unsigned)hi.x, (unsigned)hi.y, (unsigned)hi.z, (unsigned)hi.w};
-      sort8(v);
-      p4[0] = make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
-      p4[1] = make_int4((int)v[4], (int)v[5], (int)v[6], (int)v[7]);
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if ((int)v[k] >= 0) d.slot[v[k]] = base + k;
-      continue;
-    }
     int c = 0, last = -1;
     for (int r = 0; r < S; ++r) {
       const int v = p[r];
@@ -915,38 +928,54 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
                    ((sy + 1) << 26) | ((sz + 1) << 28);
     const int o = s_off[rb], c = s_off[rb + 1] - o;
     // the block's c atoms are the valid entries of its slots (a list may
-    // hold holes, -1, where atoms moved away); the slots start 32-byte
-    // aligned (at least 8 per block) and come eight at a time in two
-    // 16-byte loads issued together
+    // hold holes, -1, where atoms moved away); a list of S slots starts
+    // 4*S-byte aligned (S at least 8) and comes in 16-byte loads issued
+    // together, eight slots at a time (all sixteen of a 16-slot FP64 list)
     const int *src = A.d.sorted + s_gst[rb];
     const int S = 1 << A.d.bin_shift;
     int w = 0;
-    for (int s0 = 0; w < c && s0 < S; s0 += 8) {
-      const int4 q0 = *reinterpret_cast<const int4 *>(src + s0);
-      const int4 q1 = *reinterpret_cast<const int4 *>(src + s0 + 4);
-      int ids[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-      if constexpr (FP64) {
-        // the FP64 sums follow the order of the atoms in a block, which is
-        // input order (SRC/exaMD_input_formats.f90:80-88): an 8-slot list
-        // is sorted here, as unsigned values so that free slots (-1) come
-        // last; a longer list (after an overflow) is kept in input order
-        // by k_block_order
-        if (S == 8) {
-          unsigned u[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) u[k] = (unsigned)ids[k];
-          sort8(u);
-#pragma unroll
-          for (int k = 0; k < 8; ++k) ids[k] = (int)u[k];
-        }
+    auto put = [&](int id) {
+      if (id >= 0 && w < c) {
+        s_id[o + w] = id;
+        s_bc[o + w] = bc;
+        ++w;
       }
+    };
+    if (FP64 && (S == 8 || S == 16)) {
+      // the FP64 sums follow the order of the atoms in a block, which is
+      // input order (SRC/exaMD_input_formats.f90:80-88): a list of 8 or 16
+      // slots is sorted here, as unsigned values so that free slots (-1)
+      // come last.  The second eight slots of a 16-slot list are read (and
+      // all sixteen sorted) only when the first eight hold fewer than the
+      // block's count.  A longer list (after repeated overflows) is kept in
+      // input order by k_block_order.
+      const int4 *q = reinterpret_cast<const int4 *>(src);
+      const int4 q0 = q[0], q1 = q[1];
+      unsigned u[8] = {(unsigned)q0.x, (unsigned)q0.y, (unsigned)q0.z, (unsigned)q0.w,
+                       (unsigned)q1.x, (unsigned)q1.y, (unsigned)q1.z, (unsigned)q1.w};
+      int v8 = 0;
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (ids[k] >= 0 && w < c) {
-          s_id[o + w] = ids[k];
-          s_bc[o + w] = bc;
-          ++w;
-        }
+      for (int k = 0; k < 8; ++k) v8 += (int)u[k] >= 0;
+      if (S == 16 && v8 < c) {
+        const int4 q2 = q[2], q3 = q[3];
+        unsigned u16[16] = {u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7],
+                            (unsigned)q2.x, (unsigned)q2.y, (unsigned)q2.z, (unsigned)q2.w,
+                            (unsigned)q3.x, (unsigned)q3.y, (unsigned)q3.z, (unsigned)q3.w};
+        sort16(u16);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) put((int)u16[k]);
+      } else {
+        sort8(u);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) put((int)u[k]);
+      }
+    } else {
+      for (int s0 = 0; w < c && s0 < S; s0 += 8) {
+        const int4 q0 = *reinterpret_cast<const int4 *>(src + s0);
+        const int4 q1 = *reinterpret_cast<const int4 *>(src + s0 + 4);
+        put(q0.x); put(q0.y); put(q0.z); put(q0.w);
+        put(q1.x); put(q1.y); put(q1.z); put(q1.w);
+      }
     }
     // a count above the list's valid entries would leave LDS entries
     // unset: they get the system's first atom (so that no read goes out of
