@@ -396,3 +396,45 @@ def test_persistent_lists_match_fresh_binning(E, prec):
         dp = np.minimum(dp, box - dp).max()
         assert dp < 1e-10 * box.max(), dp
         assert np.abs(v_run - v).max() < 1e-10 * np.abs(v).max()
+
+
+def test_fp64_dense_block_of_twelve_matches_oracle(E, oracle):
+    """A block that holds 12 atoms (a 3 x 2 x 2 cluster 0.4-0.45 sigma apart
+    in a lattice with a hole around it): its list has 16 slots with ids in
+    the second eight, which the FP64 force kernel's gather sorts into input
+    order together with the first eight.  The cluster atoms sit at random
+    places of the input order.  Every atom's force is held to the oracle at
+    1e-10 of its own magnitude (cluster) or of the lattice's RMS force
+    (the rest); pair counts and the energy as everywhere."""
+    pos, L = H.perturbed_fcc(7, RHO, 0.05, 31)
+    box = np.array([L, L, L])
+    nb = E.domain(box, 1.25)[0][0]
+    bl = L / nb
+    g = np.stack(np.meshgrid(4 * bl + 0.25 + 0.4 * np.arange(3),
+                             4 * bl + 0.35 + 0.45 * np.arange(2),
+                             4 * bl + 0.35 + 0.45 * np.arange(2), indexing="ij"),
+                 -1).reshape(-1, 3)
+    lo, hi = g.min(0) - 0.9, g.max(0) + 0.9
+    keep = ~np.all((pos > lo) & (pos < hi), axis=1)
+    lat = pos[keep]
+    rng = np.random.default_rng(32)
+    at = np.sort(rng.choice(len(lat) + len(g), len(g), replace=False))
+    is_cl = np.zeros(len(lat) + len(g), bool)
+    is_cl[at] = True
+    allp = np.empty((len(is_cl), 3))
+    allp[is_cl] = g
+    allp[~is_cl] = lat
+    b = np.floor(allp / bl).astype(int)
+    occ = np.bincount(b[:, 0] + nb * (b[:, 1] + nb * b[:, 2]), minlength=nb ** 3)
+    assert occ.max() == 12
+    print(f"[dense block fp64] n={len(allp)} nb={nb}", flush=True)
+    with _system(E, allp, box) as m:
+        f, s, _ = m.forces("fp64")
+    ref, t, _ = oracle.forces(allp, box, 2.5)
+    assert (s.pairs_cand, s.pairs_cut) == (t.cand, t.cut)
+    assert abs(s.pe - t.pe) < FP64_TOL * abs(t.pe)
+    df = np.linalg.norm(f - ref, axis=1)
+    fc = np.linalg.norm(ref[is_cl], axis=1)
+    assert np.all(df[is_cl] < FP64_TOL * fc)
+    rms = np.sqrt((ref[~is_cl] ** 2).sum(1).mean())
+    assert df[~is_cl].max() < FP64_TOL * rms
