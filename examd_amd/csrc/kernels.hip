@@ -930,7 +930,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &A, const int tile,
     // the block's c atoms are the valid entries of its slots (a list may
     // hold holes, -1, where atoms moved away); a list of S slots starts
     // 4*S-byte aligned (S at least 8) and comes in 16-byte loads issued
-    // together, eight slots at a time (all sixteen of a 16-slot FP64 list)
+    // together, eight slots at a time
     const int *src = A.d.sorted + s_gst[rb];
     const int S = 1 << A.d.bin_shift;
     int w = 0;
